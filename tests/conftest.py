@@ -1,0 +1,32 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP C-ABI library)")
+
+
+@pytest.fixture(scope="session")
+def root():
+    return ROOT
+
+
+@pytest.fixture(scope="session")
+def golden():
+    import numpy as np
+
+    return dict(np.load(os.path.join(ROOT, "tests", "golden", "golden_small.npz")))
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import cpuref
+
+    cpuref.build()
+    return cpuref
