@@ -1,0 +1,154 @@
+/*
+ * tinyvlm_hip.h -- C ABI of the MI355X (gfx950) HIP backend for tinyvlm.
+ *
+ * Drop-in boundary: every entry point below is one method of the reference's plugin interface
+ * (class vlm::Backend, vlm::LU, vlm::BLAS, vlm::Memory in vlm/include/vlm_backend.hpp and
+ * vlm/include/vlm_memory.hpp), flattened to plain C: device pointers, shapes and strides
+ * (tvlm_view), an explicit HIP stream, and an int status instead of exit()/exceptions.
+ * The citation beside each declaration names the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every tensor is column-major like the reference TensorView (vlm/include/vlm_memory.hpp:306-319):
+ *    element (i,j,k) lives at ptr + i*stride[0] + j*stride[1] + k*stride[2]; stride[0] must be 1
+ *    for 2-D/3-D operands.  Unused trailing dimensions have shape 1.
+ *  - Multi-surface arguments (the reference's MultiTensorView = std::vector<TensorView>) are
+ *    host arrays of n_surf tvlm_view, one per surface, 1 <= n_surf <= TVLM_MAX_SURFACES.
+ *  - Suffix _f64 computes in double, _f32 in float.  `stream` is a hipStream_t (NULL = default).
+ *  - All compute entry points are asynchronous on `stream`.  Scalar results (CL, CM, sums,
+ *    norms) are written to DEVICE memory (`*_dev` arguments) so that callers can keep whole
+ *    solver loops on the device; the C++ adapter (tinyvlm_amd/vlm) turns them back into the
+ *    reference's by-value returns.
+ *  - Outputs the reference accumulates into (lhs, rhs, forces_unsteady) are accumulated here too;
+ *    the caller zero-fills them, as the reference drivers do (vlm/src/vlm.cpp:99-100, :437).
+ *  - Return value: TVLM_OK or an error code; tvlm_last_error() gives the message.
+ */
+#ifndef TINYVLM_HIP_H
+#define TINYVLM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TVLM_MAX_SURFACES 16
+
+typedef enum tvlm_status {
+    TVLM_OK = 0,
+    TVLM_ERR_INVALID = 1,     /* bad shape / stride / argument */
+    TVLM_ERR_HIP = 2,         /* HIP runtime error */
+    TVLM_ERR_SINGULAR = 3,    /* getrf: exactly zero pivot (info > 0) */
+    TVLM_ERR_UNSUPPORTED = 4, /* request outside what this build implements */
+    TVLM_ERR_NOMEM = 5        /* device allocation failed */
+} tvlm_status;
+
+typedef struct tvlm_view {
+    void* ptr;         /* address of element (0,0,0) */
+    int64_t shape[3];  /* extents; 1 for unused trailing dimensions */
+    int64_t stride[3]; /* in elements */
+} tvlm_view;
+
+/* ---- library / device ---------------------------------------------------------------------- */
+/* BackendCUDA::BackendCUDA device selection and banner (vlm/backends/cuda/src/vlm_backend_cuda.cu:132-142) */
+int tvlm_init(int32_t device);
+/* error text of the last failing call on this thread (replaces the reference's printf + exit,
+ * vlm/backends/cuda/src/vlm_backend_cuda.cu:21-29) */
+const char* tvlm_last_error(void);
+/* create_backend name registry (vlm/src/vlm_backend.cpp:15-38): "hip" */
+const char* tvlm_backend_name(void);
+int tvlm_stream_synchronize(void* stream);
+
+/* ---- Memory (vlm/include/vlm_memory.hpp:22-35) ---------------------------------------------- */
+/* location: 0 = Device, 1 = Host (pinned) -- enum class Location { Device, Host } */
+int tvlm_malloc(void** ptr, int64_t bytes, int32_t location);
+int tvlm_free(void* ptr, int32_t location);
+/* Memory::copy: n elements of elem_size bytes, strided in elements */
+int tvlm_copy(int32_t dst_loc, void* dst, int64_t dst_stride, int32_t src_loc, const void* src,
+              int64_t src_stride, int64_t elem_size, int64_t n, void* stream);
+/* Memory::fill (float / double overloads) */
+int tvlm_fill_f32(int32_t location, float* ptr, int64_t stride, float value, int64_t n, void* stream);
+int tvlm_fill_f64(int32_t location, double* ptr, int64_t stride, double value, int64_t n, void* stream);
+
+/* ---- Backend virtuals ----------------------------------------------------------------------- */
+#define TVLM_DECLARE_BACKEND(T, S)                                                                  \
+    /* Backend::lhs_assemble (vlm_backend.hpp:31/:44): lhs{N,N} += ring influence; TE columns also \
+     * get wake rows -2-k, k < iteration */                                                         \
+    int tvlm_lhs_assemble_##S(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc,      \
+                              const tvlm_view* normals, const tvlm_view* verts_wing,                \
+                              const tvlm_view* verts_wake, int32_t iteration, T sigma,             \
+                              void* stream);                                                        \
+    /* Backend::rhs_assemble_velocities (vlm_backend.hpp:32/:45): rhs += -(V . n) */                \
+    int tvlm_rhs_assemble_velocities_##S(const tvlm_view* rhs, int32_t n_surf,                     \
+                                         const tvlm_view* normals, const tvlm_view* velocities,     \
+                                         void* stream);                                             \
+    /* Backend::rhs_assemble_wake_influence (vlm_backend.hpp:33/:46) */                             \
+    int tvlm_rhs_assemble_wake_influence_##S(const tvlm_view* rhs, int32_t n_surf,                 \
+                                             const tvlm_view* gamma_wake, const tvlm_view* colloc,  \
+                                             const tvlm_view* normals,                              \
+                                             const tvlm_view* verts_wake, const uint8_t* lifting,   \
+                                             int32_t iteration, T sigma, void* stream);             \
+    /* Backend::forces_steady (vlm_backend.hpp:34/:47) */                                           \
+    int tvlm_forces_steady_##S(const tvlm_view* verts_wing, const tvlm_view* gamma_delta,          \
+                               const tvlm_view* velocities, const tvlm_view* forces, void* stream); \
+    /* Backend::forces_unsteady (vlm_backend.hpp:35/:48) */                                         \
+    int tvlm_forces_unsteady_##S(const tvlm_view* verts_wing, const tvlm_view* gamma_delta,        \
+                                 const tvlm_view* dgamma_dt, const tvlm_view* velocities,           \
+                                 const tvlm_view* areas, const tvlm_view* normals,                  \
+                                 const tvlm_view* forces, void* stream);                            \
+    /* Backend::coeff_cl (vlm_backend.hpp:36/:49); result -> cl_dev[0] */                           \
+    int tvlm_coeff_cl_##S(const tvlm_view* forces, const T* lift_axis, const T* freestream, T rho, \
+                          T area, T* cl_dev, void* stream);                                         \
+    /* Backend::coeff_cm (vlm_backend.hpp:37/:50); result -> cm_dev[0..2] */                        \
+    int tvlm_coeff_cm_##S(const tvlm_view* forces, const tvlm_view* verts_wing, const T* ref_pt,   \
+                          const T* freestream, T rho, T area, T mac, T* cm_dev, void* stream);      \
+    /* Backend::mesh_metrics (vlm_backend.hpp:38/:51) */                                            \
+    int tvlm_mesh_metrics_##S(T alpha_rad, int32_t n_surf, const tvlm_view* verts_wing,            \
+                              const tvlm_view* colloc, const tvlm_view* normals,                    \
+                              const tvlm_view* areas, void* stream);                                \
+    /* Backend::mesh_mac (vlm_backend.hpp:39/:52); result -> mac_dev[0] */                          \
+    int tvlm_mesh_mac_##S(const tvlm_view* verts_wing, const tvlm_view* areas, T* mac_dev,         \
+                          void* stream);                                                            \
+    /* Backend::gamma_wake_from_coeffs (vlm_backend.hpp:40/:53) */                                  \
+    int tvlm_gamma_wake_from_coeffs_##S(const tvlm_view* gamma_wake,                               \
+                                        const tvlm_view* gamma_coeffs, int32_t harmonics, T tn,     \
+                                        T omega, T dt, int64_t iteration, void* stream);            \
+    /* Backend::sum (TV1 / TV2 overloads, vlm_backend.hpp:41-42/:54-55); ndim 1 or 2 */             \
+    int tvlm_sum_##S(const tvlm_view* x, int32_t ndim, T* sum_dev, void* stream);                  \
+    /* Backend::wake_shed (vlm_backend.hpp:59/:66, impl vlm/src/vlm_backend.cpp:42-47) */           \
+    int tvlm_wake_shed_##S(int32_t n_surf, const tvlm_view* verts_wing,                            \
+                           const tvlm_view* verts_wake, int32_t iteration, void* stream);           \
+    /* Backend::displace_wing (vlm_backend.hpp:58/:65, impl vlm/src/vlm_backend.cpp:49-69) */       \
+    int tvlm_displace_wing_##S(int32_t n_surf, const tvlm_view* transforms,                        \
+                               const tvlm_view* verts_wing, const tvlm_view* verts_wing_init,       \
+                               void* stream);                                                       \
+    /* LU::factorize (vlm_backend.hpp:105/:112): in-place L\U with row pivots; ipiv_dev{N} is      \
+     * 1-based like LAPACK; info_dev[0] = 0 or k (1-based) of the first exactly zero pivot */       \
+    int tvlm_getrf_##S(const tvlm_view* a, int32_t* ipiv_dev, int32_t* info_dev, void* stream);    \
+    /* LU::solve (vlm_backend.hpp:106/:113): x{N,nrhs} <- A^{-1} x ('N') */                         \
+    int tvlm_getrs_##S(const tvlm_view* a, const int32_t* ipiv_dev, const tvlm_view* x,            \
+                       void* stream);                                                               \
+    /* BLAS::gemv (vlm_backend.hpp:84/:91), trans: 0 = No, 1 = Yes */                               \
+    int tvlm_gemv_##S(T alpha, const tvlm_view* a, const tvlm_view* x, T beta, const tvlm_view* y, \
+                      int32_t trans, void* stream);                                                 \
+    /* BLAS::gemm (vlm_backend.hpp:85/:92) */                                                       \
+    int tvlm_gemm_##S(T alpha, const tvlm_view* a, const tvlm_view* b, T beta, const tvlm_view* c, \
+                      int32_t trans_a, int32_t trans_b, void* stream);                              \
+    /* BLAS::axpy 1-D and column-wise 2-D (vlm_backend.hpp:86-87/:93-94); ndim 1 or 2 */            \
+    int tvlm_axpy_##S(T alpha, const tvlm_view* x, const tvlm_view* y, int32_t ndim, void* stream);\
+    /* BLAS::scal (vlm_backend.hpp:88/:95) */                                                       \
+    int tvlm_scal_##S(T alpha, const tvlm_view* x, void* stream);                                  \
+    /* BLAS::norm (vlm_backend.hpp:89/:96) -> nrm_dev[0] */                                         \
+    int tvlm_nrm2_##S(const tvlm_view* x, T* nrm_dev, void* stream);                              \
+    /* Device form of UVLM::run's host velocity loop (vlm/src/vlm.cpp:419-436): for surface m,     \
+     * velocities(p) = -(dT_m [x_p; 1]) with x_p = colloc(p); dT holds n_surf column-major 4x4      \
+     * time derivatives of the surface transforms (host memory, 16*n_surf values) */                \
+    int tvlm_kinematic_velocities_##S(int32_t n_surf, const T* dT, const tvlm_view* colloc,       \
+                                      const tvlm_view* velocities, void* stream);
+
+TVLM_DECLARE_BACKEND(double, f64)
+TVLM_DECLARE_BACKEND(float, f32)
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TINYVLM_HIP_H */

@@ -28,5 +28,6 @@ def golden():
 def oracle():
     from oracle import cpuref
 
-    cpuref.build()
+    if not os.path.exists(cpuref._LIB_PATH):
+        cpuref.build()
     return cpuref

@@ -1,0 +1,206 @@
+// tvlm_internal.hpp -- shared device/host helpers of the gfx950 backend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/tinyvlm_hip.h"
+
+namespace tvlm {
+
+// ---- error reporting --------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+
+#define TVLM_HIP(expr)                                                                              \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess) {                                                                     \
+            ::tvlm::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__,      \
+                              __LINE__);                                                            \
+            return e_ == hipErrorOutOfMemory ? TVLM_ERR_NOMEM : TVLM_ERR_HIP;                       \
+        }                                                                                           \
+    } while (0)
+
+#define TVLM_REQUIRE(cond, ...)                                                                     \
+    do {                                                                                            \
+        if (!(cond)) {                                                                              \
+            ::tvlm::set_error(__VA_ARGS__);                                                         \
+            return TVLM_ERR_INVALID;                                                                \
+        }                                                                                           \
+    } while (0)
+
+#define TVLM_TRY(expr)                                                                              \
+    do {                                                                                            \
+        int s_ = (expr);                                                                            \
+        if (s_ != TVLM_OK) return s_;                                                               \
+    } while (0)
+
+#define TVLM_LAUNCHED() TVLM_HIP(hipGetLastError())
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---- view validation --------------------------------------------------------------------------
+// 3-D per-surface tensors {a, b, k} must be "plane contiguous": stride[0] == 1 and
+// stride[1] == shape[0] (what the reference kernels assume, vlm_backend_cpu.cpp:111, :202).
+inline bool plane_contiguous(const tvlm_view& v) {
+    return v.ptr != nullptr && v.stride[0] == 1 && v.stride[1] == v.shape[0];
+}
+
+// Per-device scratch buffer owned by the library (grow-only).  Used for deterministic two-stage
+// reductions.  Not thread-safe across concurrent callers on one device, like the reference's
+// CUDA scratch scalars (vlm_backend_cuda.hpp:45-54).
+int device_workspace(size_t bytes, void** out);
+
+// ---- small vector type ------------------------------------------------------------------------
+template <typename T>
+struct v3 {
+    T x, y, z;
+};
+template <typename T>
+__host__ __device__ inline v3<T> operator-(const v3<T>& a, const v3<T>& b) {
+    return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <typename T>
+__host__ __device__ inline v3<T> operator+(const v3<T>& a, const v3<T>& b) {
+    return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+template <typename T>
+__host__ __device__ inline v3<T> operator*(T s, const v3<T>& a) {
+    return {s * a.x, s * a.y, s * a.z};
+}
+template <typename T>
+__host__ __device__ inline T dot(const v3<T>& a, const v3<T>& b) {
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <typename T>
+__host__ __device__ inline v3<T> cross(const v3<T>& a, const v3<T>& b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// ---- Biot-Savart constants (vlm_backend_cpu_kernels.ispc:9-12, :80, :87, :393, :400) -----------
+template <typename T>
+struct BS;
+template <>
+struct BS<double> {
+    static constexpr double fourpi = (double)(4.0f * 3.141593f);  // the float product 4.0f*PI_f
+    static constexpr double cut_cross = (double)1e-10f;            // RCUT
+    static constexpr double cut_norm = (double)1e-10f;             // FP64 uses RCUT for the norms too
+};
+template <>
+struct BS<float> {
+    static constexpr float fourpi = 4.0f * 3.141593f;
+    static constexpr float cut_cross = 1e-10f;  // RCUT on |r1 x r2|^2
+    static constexpr float cut_norm = 1e-5f;    // RCUT2 on |r1|, |r2|
+};
+
+// Exact form of one unit straight segment a->b at c (Katz-Plotkin eq. 10.115), operation for
+// operation as kernel_biosavart / dkernel_biosavart (kernels.ispc:69-89 / :382-402) with IEEE
+// sqrt and division.  With sigma == 0 the core term sqrt(sq*sq + 0) equals sq exactly in IEEE
+// arithmetic, so the SIGMA=false instantiation skips that square root.
+template <typename T, bool SIGMA>
+__device__ __forceinline__ v3<T> segment_exact(const v3<T>& c, const v3<T>& a, const v3<T>& b, T sigma) {
+    const v3<T> r0 = b - a;
+    const v3<T> r1 = c - a;
+    const v3<T> r2 = c - b;
+    const v3<T> cr = cross(r1, r2);
+    const T r1n = sqrt(dot(r1, r1));
+    const T r2n = sqrt(dot(r2, r2));
+    const T sq = dot(cr, cr);
+    if ((sq < BS<T>::cut_cross) || (r1n < BS<T>::cut_norm) || (r2n < BS<T>::cut_norm)) return {T(0), T(0), T(0)};
+    T core;
+    if constexpr (SIGMA) {
+        const T sm = sigma * sigma * dot(r0, r0);
+        core = sqrt(sq * sq + sm * sm);
+    } else {
+        core = sq;
+    }
+    const T coeff = (dot(r0, r1) * r2n - dot(r0, r2) * r1n) / (BS<T>::fourpi * core * r1n * r2n);
+    return coeff * cr;
+}
+
+// Segment + its image across y = 0 (kernel_symmetry, kernels.ispc:91-102).
+template <typename T, bool SIGMA>
+__device__ __forceinline__ void symmetry_exact(v3<T>& inf, const v3<T>& c, const v3<T>& a, const v3<T>& b, T sigma) {
+    const v3<T> s = segment_exact<T, SIGMA>(c, a, b, sigma);
+    inf.x += s.x;
+    inf.y += s.y;
+    inf.z += s.z;
+    const v3<T> cm = {c.x, -c.y, c.z};
+    const v3<T> t = segment_exact<T, SIGMA>(cm, a, b, sigma);
+    inf.x += t.x;
+    inf.y -= t.y;
+    inf.z += t.z;
+}
+
+// Unit vortex ring v0->v1->v2->v3->v0 with images (kernel_influence body, kernels.ispc:127-133).
+template <typename T, bool SIGMA>
+__device__ __forceinline__ v3<T> ring_exact(const v3<T>& c, const v3<T>& v0, const v3<T>& v1, const v3<T>& v2,
+                                            const v3<T>& v3_, T sigma) {
+    v3<T> inf = {T(0), T(0), T(0)};
+    symmetry_exact<T, SIGMA>(inf, c, v0, v1, sigma);
+    symmetry_exact<T, SIGMA>(inf, c, v1, v2, sigma);
+    symmetry_exact<T, SIGMA>(inf, c, v2, v3_, sigma);
+    symmetry_exact<T, SIGMA>(inf, c, v3_, v0, sigma);
+    return inf;
+}
+
+// ---- FP32 fast form ---------------------------------------------------------------------------
+// Same mathematics with the hardware reciprocal square root: coeff = (d1/|r1| - d2/|r2|) / (4 pi sq),
+// which equals the reference's (d1 |r2| - d2 |r1|) / (4 pi sq |r1| |r2|).  Cutoffs are tested on
+// squared norms: |r| < 1e-5  <=>  |r|^2 < 1e-10, and sq < 1e-10 as in the reference.  Each ring
+// vertex difference and its rsqrt is computed once and shared by the two segments that meet there.
+struct FastVertex {
+    float x, y, z, r2, rinv;  // c - v, |c - v|^2, 1/|c - v|
+};
+
+__device__ __forceinline__ FastVertex fast_vertex(float cx, float cy, float cz, float vx, float vy, float vz) {
+    FastVertex f;
+    f.x = cx - vx;
+    f.y = cy - vy;
+    f.z = cz - vz;
+    f.r2 = f.x * f.x + f.y * f.y + f.z * f.z;
+    f.rinv = __builtin_amdgcn_rsqf(f.r2);
+    return f;
+}
+
+// Accumulates s * (velocity of segment a->b) into acc; s = +1 / -1 mixes the y image sign.
+__device__ __forceinline__ void fast_segment(float& ax, float& ay, float& az, const FastVertex& r1, const FastVertex& r2,
+                                             float sy) {
+    const float crx = r1.y * r2.z - r1.z * r2.y;
+    const float cry = r1.z * r2.x - r1.x * r2.z;
+    const float crz = r1.x * r2.y - r1.y * r2.x;
+    const float sq = crx * crx + cry * cry + crz * crz;
+    // r0 = b - a = r1 - r2 (as points: (c-a) - (c-b))
+    const float r0x = r1.x - r2.x, r0y = r1.y - r2.y, r0z = r1.z - r2.z;
+    const float d1 = r0x * r1.x + r0y * r1.y + r0z * r1.z;
+    const float d2 = r0x * r2.x + r0y * r2.y + r0z * r2.z;
+    const bool ok = (sq >= 1e-10f) & (r1.r2 >= 1e-10f) & (r2.r2 >= 1e-10f);
+    const float t = (d1 * r1.rinv - d2 * r2.rinv) * __builtin_amdgcn_rcpf(BS<float>::fourpi * sq);
+    const float coeff = ok ? t : 0.0f;
+    ax += coeff * crx;
+    ay += sy * (coeff * cry);
+    az += coeff * crz;
+}
+
+// Ring with image; vertices given as coordinates.
+__device__ __forceinline__ void fast_ring(float& ax, float& ay, float& az, float cx, float cy, float cz, const float* v0,
+                                          const float* v1, const float* v2, const float* v3) {
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+        const float y = m == 0 ? cy : -cy;
+        const float sy = m == 0 ? 1.0f : -1.0f;
+        const FastVertex a = fast_vertex(cx, y, cz, v0[0], v0[1], v0[2]);
+        const FastVertex b = fast_vertex(cx, y, cz, v1[0], v1[1], v1[2]);
+        const FastVertex c = fast_vertex(cx, y, cz, v2[0], v2[1], v2[2]);
+        const FastVertex d = fast_vertex(cx, y, cz, v3[0], v3[1], v3[2]);
+        fast_segment(ax, ay, az, a, b, sy);
+        fast_segment(ax, ay, az, b, c, sy);
+        fast_segment(ax, ay, az, c, d, sy);
+        fast_segment(ax, ay, az, d, a, sy);
+    }
+}
+
+}  // namespace tvlm
