@@ -168,13 +168,16 @@ def test_wake_multi_surface_and_tiles(H, torch, oracle, prec):
     rng = np.random.default_rng(7)
     specs = [(70, 3, 40, 0.0), (5, 2, 9, 2.5), (12, 4, 17, 5.0)]
     lifting = [True, False, True]
-    osurf, dev = [], []
+    osurf, osurf64, dev = [], [], []
     for (ns, nc, rows, y0), lift in zip(specs, lifting):
         v, wake, gw = _random_wake_surface(rng, ns, nc, rows, y0)
-        s = oracle.Surface(verts=np.asfortranarray(v.astype(npdt)), verts_wake=np.asfortranarray(wake.astype(npdt)),
-                           gamma_wake=np.asfortranarray(gw.astype(npdt)), lifting=lift)
-        osurf.append(s)
+        arrs = [np.asfortranarray(x.astype(npdt)) for x in (v, wake, gw)]
+        osurf.append(oracle.Surface(verts=arrs[0], verts_wake=arrs[1], gamma_wake=arrs[2], lifting=lift))
+        # FP64 evaluation of exactly the same (rounded) inputs: the accuracy yardstick for FP32
+        a64 = [np.asfortranarray(x.astype(np.float64)) for x in arrs]
+        osurf64.append(oracle.Surface(verts=a64[0], verts_wake=a64[1], gamma_wake=a64[2], lifting=lift))
     oracle.mesh_metrics(0.0, osurf)
+    oracle.mesh_metrics(0.0, osurf64)
     for s in osurf:
         dev.append(dict(c=H.to_device(s.colloc, dt), n=H.to_device(s.normals, dt), w=H.to_device(s.verts_wake, dt),
                         g=H.to_device(s.gamma_wake, dt)))
@@ -185,7 +188,15 @@ def test_wake_multi_surface_and_tiles(H, torch, oracle, prec):
         rhs = torch.zeros(N, dtype=dt, device="cuda")
         H.rhs_assemble_wake_influence(rhs, [d["g"] for d in dev], [d["c"] for d in dev], [d["n"] for d in dev],
                                       [d["w"] for d in dev], lifting, it)
-        assert rel(H.to_host(rhs), rhs_o) < (1e-13 if prec == "f64" else 2e-5)
+        if prec == "f64":
+            assert rel(H.to_host(rhs), rhs_o) < 1e-13
+        else:
+            # FP32: no worse than 3x the FP32 restatement's own error against FP64 on the same inputs
+            rhs64 = np.zeros(N)
+            oracle.rhs_assemble_wake_influence(rhs64, osurf64, it)
+            e_gpu, e_orc = rel(H.to_host(rhs), rhs64), rel(rhs_o, rhs64)
+            print(f"[case] f32 wake it={it}: gpu err {e_gpu:.2e} fp32-oracle err {e_orc:.2e}", flush=True)
+            assert e_gpu < max(3 * e_orc, 1e-6)
         # deterministic: the same call gives bitwise the same result
         rhs2 = torch.zeros(N, dtype=dt, device="cuda")
         H.rhs_assemble_wake_influence(rhs2, [d["g"] for d in dev], [d["c"] for d in dev], [d["n"] for d in dev],
@@ -313,7 +324,7 @@ def test_shed_displace_coeff_wake_kinematics(H, torch, oracle, prec):
     oracle.mesh_metrics(0.0, [s])
     c = H.to_device(s.colloc, dt)
     vel = H.colmajor(s.colloc.shape, dt)
-    H.kinematic_velocities([dT], [c], vel)
+    H.kinematic_velocities([dT], [c], [vel])
     pts = s.colloc.reshape(-1, 3, order="F").astype(np.float64)
     ref = -(pts @ dT[:3, :3].T + dT[:3, 3])
     assert rel(H.to_host(vel).reshape(-1, 3, order="F"), ref) < (1e-15 if prec == "f64" else 1e-6)
