@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the hot path on one MI355X: the unsteady pitch + heave workload.
+
+Workload (the unsteady configuration the headline metric is quoted on): flat rectangular wing,
+AR 4 (chord 1, half span 2), 10 chordwise x 40 spanwise vortex-ring panels (N = 400), heave
+0.1 c sin(t), pitch 2 deg sin(t + pi/2) about the quarter chord, reduced frequency k = 0.5,
+dt = c / (10 u_inf) = last panel chord / u_inf, 500 time steps; the wake grows to 500 x 40 rings.
+
+A bench "step" is one whole simulation (motion evaluation, pose, influence matrix, LU, and 500
+UVLM time steps with wake induction, solve, forces, CL and CM), all inside the HIP library.
+value = UVLM time steps per second = 500 * K / (time of K simulations), inputs resident on the
+device.  The served path evaluates the wake induction with FP64 arithmetic and FP32-seeded
+rsqrt/reciprocal (one Newton step), solves with the FP64 inverse and replays the loop as one
+hipGraph; it is accepted only if |dCL| <= 1e-6 at every step against the plain FP64 path
+(exact FP64 Biot-Savart, LU substitution each step), which is timed too.
+
+Also reported: the roofline of the dominant kernel (wake induction, 398 algorithmic flops per
+receiver x wake-ring pair, timed with HIP events on the plan's stream) and the CPU baseline (the
+oracle's FP64 restatement of the reference CPU backend, OpenMP on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOPS_PER_PAIR = 398  # algorithmic flops per (receiver, ring) pair, both mirror images
+PEAK_FP32_VALU_TFLOPS = 157.3  # MI355X dense FP32 (packed) nominal
+NS, NC, HALF_SPAN, CHORD = 40, 10, 2.0, 1.0
+T_STEPS = 501  # UVLM::run loops t_steps-1 times -> 500 time steps
+A_AMP = float(np.float32(2) * np.float32(3.141593) / np.float32(180))
+MOTION = dict(u=1.0, h_amp=0.1, h_phase=0.0, a_amp=A_AMP, a_phase=math.pi / 2, omega=1.0, x_pivot=0.25)
+
+
+def rectangular_wing_f32(ns, nc, chord, half_span):
+    """Flat rectangular half wing as a plot3d file would hold it (float32), quarter-chord shifted
+    in float32 like every reference mesh (vlm/src/vlm_mesh.cpp:15-32), widened to float64."""
+    f = np.float32
+    v = np.zeros((ns + 1, nc + 1, 4), dtype=f)
+    ys = np.linspace(0.0, half_span, ns + 1)
+    xs = np.linspace(0.0, chord, nc + 1)
+    for i in range(ns + 1):
+        v[i, :, 0] = xs
+        v[i, :, 1] = ys[i]
+    v[:, :, 3] = 1
+    for j in range(nc):
+        v[:, j, :3] = f(0.75) * v[:, j, :3] + f(0.25) * v[:, j + 1, :3]
+    v[:, nc, :3] = f(4.0 / 3.0) * v[:, nc, :3] - f(1.0 / 3.0) * v[:, nc - 1, :3]
+    return np.asfortranarray(v.astype(np.float64))
+
+
+def run_plan(plan, k, use_graph):
+    import torch
+
+    lat = []
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(plan.stream):
+        start.record(plan.stream)
+        for _ in range(k):
+            t1 = time.perf_counter()
+            plan.set_pitch_heave(**MOTION)
+            plan.run(use_graph=use_graph)
+            plan.stream.synchronize()
+            lat.append((time.perf_counter() - t1) * 1e3)
+        stop.record(plan.stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, start.elapsed_time(stop) / 1e3, lat
+
+
+def cpu_baseline(verts, budget_steps):
+    from oracle import cpuref as C
+
+    mot = [MOTION[k] for k in ("u", "h_amp", "h_phase", "a_amp", "a_phase", "omega", "x_pivot")]
+    C.uvlm_run([verts], mot, t_steps=3)  # warm-up (thread pool)
+    t0 = time.perf_counter()
+    cl, _, _, n = C.uvlm_run([verts], mot, t_steps=budget_steps + 1)
+    el = time.perf_counter() - t0
+    return budget_steps / el, el, C.lib().orc_num_threads(), cl
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed simulations of 500 UVLM steps")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=3, help="timed plain FP64 simulations")
+    ap.add_argument("--cpu-steps", type=int, default=500, help="UVLM steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the workload is independent per request and would shard by request, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from tinyvlm_amd import capi
+    from tinyvlm_amd.uvlm_plan import MODE_FAST, MODE_PLAIN_FP64, UVLMPlan, time_step
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    capi.init(0)
+    verts = rectangular_wing_f32(NS, NC, CHORD, HALF_SPAN)
+    dt = time_step(verts, MOTION["u"])
+    steps_per_sim = T_STEPS - 1
+
+    # served path: seeded FP64 wake induction, FP64 inverse solve, hipGraph replay
+    fast = UVLMPlan([verts], T_STEPS, dt, mode=MODE_FAST)
+    _, _, _ = run_plan(fast, max(args.warmup, 1), use_graph=True)
+    wall, gpu_s, lat = run_plan(fast, args.steps, use_graph=True)
+    res_fast = fast.results()
+    value = steps_per_sim * args.steps / wall
+
+    # plain FP64 path (exact FP64 kernels, LU substitution each step, no graph)
+    plain = UVLMPlan([verts], T_STEPS, dt, mode=MODE_PLAIN_FP64)
+    run_plan(plain, 1, use_graph=False)
+    wall_p, _, lat_p = run_plan(plain, args.plain_steps, use_graph=False)
+    res_plain = plain.results()
+    plain_value = steps_per_sim * args.plain_steps / wall_p
+    dcl = float(np.max(np.abs(res_fast["cl"] - res_plain["cl"])))
+    dcm = float(np.max(np.abs(res_fast["cm"] - res_plain["cm"])))
+
+    # roofline of the dominant kernel: live HIP events around each wake launch on the plan stream
+    wake_ms, launches, pairs = fast.time_wake()
+    avg_ms = wake_ms / launches
+    achieved = FLOPS_PER_PAIR * (pairs / launches) / (avg_ms * 1e-3) / 1e12
+    wake_ms_p, launches_p, pairs_p = plain.time_wake()
+    roofline = {
+        "bound": "valu",
+        "kernel": "k_wake_partials (FP64 seeded rsqrt/rcp form)",
+        "achieved": round(achieved, 3),
+        "peak": PEAK_FP32_VALU_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP32_VALU_TFLOPS, 4),
+        "traffic": None,
+        "avg_launch_us": round(avg_ms * 1e3, 3),
+        "launches": launches,
+        "pairs_per_launch": pairs / launches,
+        "flops_per_pair": FLOPS_PER_PAIR,
+        "wake_share_of_sim": round(wake_ms / 1e3 / (wall / args.steps), 4),
+        "plain_fp64_wake_tflops": round(FLOPS_PER_PAIR * pairs_p / (wake_ms_p * 1e-3) / 1e12, 3),
+    }
+
+    out = {
+        "metric": "steady solves/s & UVLM steps/s vs plain FP64 at |dCL|<=1e-6; p50/p99 latency",
+        "value": round(value, 2),
+        "unit": "UVLM steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (f32-seeded rsqrt/rcp in the wake kernel)",
+        "data": "synthetic",
+        "config": {
+            "workload": "uvlm_pitch_heave_ar4_10c_x_40s_500steps",
+            "panels": NS * NC, "chordwise": NC, "spanwise": NS, "time_steps_per_sim": steps_per_sim,
+            "wake_rings_final": NS * steps_per_sim, "heave": "0.1c sin(t)", "pitch": "2deg sin(t+pi/2)",
+            "reduced_frequency": 0.5, "dt": dt, "bench_step": "one whole 500-step simulation",
+        },
+        "plain_fp64": {"value": round(plain_value, 2), "unit": "UVLM steps/s", "sims": args.plain_steps,
+                       "speedup_of_served_path": round(value / plain_value, 3)},
+        "accuracy": {"max_abs_dCL_vs_plain_fp64": dcl, "max_abs_dCM_vs_plain_fp64": dcm, "bar": 1e-6,
+                     "accepted": bool(dcl <= 1e-6)},
+        "latency_ms_per_sim": {"p50": round(float(np.percentile(lat, 50)), 4),
+                               "p99": round(float(np.percentile(lat, 99)), 4),
+                               "plain_fp64_p50": round(float(np.percentile(lat_p, 50)), 4)},
+        "gpu_event_s": round(gpu_s, 6),
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        cpu_sps, cpu_el, threads, cl_cpu = cpu_baseline(verts, args.cpu_steps)
+        n = min(len(cl_cpu), len(res_plain["cl"]))
+        out["cpu_baseline"] = {
+            "value": round(cpu_sps, 3), "unit": "UVLM steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 restatement of the reference CPU backend, first {args.cpu_steps} steps of the "
+                      f"same workload ({cpu_el:.1f} s, OpenMP {threads} threads)",
+            "max_abs_dCL_plain_gpu_vs_cpu": float(np.max(np.abs(cl_cpu[:n] - res_plain["cl"][:n]))),
+        }
+    else:
+        out["cpu_baseline"] = None
+    fast.close()
+    plain.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -148,6 +148,47 @@ int tvlm_fill_f64(int32_t location, double* ptr, int64_t stride, double value, i
 TVLM_DECLARE_BACKEND(double, f64)
 TVLM_DECLARE_BACKEND(float, f32)
 
+/* ---- device-resident UVLM time loop ---------------------------------------------------------
+ * The caller of the hot path, UVLM::run (vlm/src/vlm.cpp:358-509), kept on the device: the host
+ * evaluates the prescribed kinematics once for all steps (tvlm_uvlm_set_motion) and the whole
+ * loop -- wake shed, kinematic velocities, rhs, wake induction, solve, gamma post-processing,
+ * unsteady forces, CL/CM, displacement, panel metrics -- runs without host round trips.
+ * mode 0: plain FP64 (exact FP64 Biot-Savart, LU forward/backward substitution every step).
+ * mode 1: wake induction in the FP64 seeded form (FP64 arithmetic, 1/sqrt and 1/x from the FP32
+ *         hardware estimates plus one Newton step), gamma = A^{-1} rhs with the FP64 inverse formed
+ *         once from the LU; use_graph replays the loop as a hipGraph (needs a non-default stream).
+ * All surfaces are lifting; LHS assembly uses iteration 0 as the reference does (vlm.cpp:402). */
+typedef struct tvlm_uvlm_plan tvlm_uvlm_plan;
+/* verts_init[m]: host {ns+1, nc+1, 4} column-major double (w = 1); t_steps as UVLM::run's
+ * (the loop runs t_steps-1 steps); dt the fixed time step (vlm.cpp:380-386) */
+int tvlm_uvlm_create(tvlm_uvlm_plan** plan, int32_t n_surf, const int64_t* ns, const int64_t* nc,
+                     const double* const* verts_init, int64_t t_steps, double dt, int32_t mode,
+                     void* stream);
+/* T, dT: host [t_steps][n_surf][16] column-major value / time derivative of each surface's
+ * transform at t_i = i*dt; freestream, ref_pt: host [t_steps][3] */
+int tvlm_uvlm_set_motion(tvlm_uvlm_plan* plan, const double* T, const double* dT,
+                         const double* freestream, const double* ref_pt);
+/* prescribed motion evaluated natively for all steps: freestream translation(-u t), heave
+ * h_amp sin(omega t + h_phase) after it, pitch a_amp sin(omega t + a_phase) about (x_pivot,0,0)
+ * and the y axis after that (tests/uvlm_theodorsen.cpp:184-208 style chain); CM about
+ * T(t) (0.5,0,0) (vlm/src/vlm.cpp:464-466) */
+int tvlm_uvlm_set_pitch_heave(tvlm_uvlm_plan* plan, double u, double h_amp, double h_phase,
+                              double a_amp, double a_phase, double omega, double x_pivot);
+/* host-only evaluation of the same chain into caller arrays laid out as tvlm_uvlm_set_motion
+ * takes them (no device work) */
+int tvlm_motion_pitch_heave(double u, double h_amp, double h_phase, double a_amp, double a_phase,
+                            double omega, double x_pivot, int32_t n_surf, int64_t t_steps, double dt,
+                            double* T, double* dT, double* freestream, double* ref_pt);
+/* one whole simulation (setup + all steps), asynchronous on the plan's stream */
+int tvlm_uvlm_run(tvlm_uvlm_plan* plan, int32_t use_graph);
+/* synchronises; CL[t_steps-1], CM[t_steps-1][3] (step 0 = 0 as the reference skips it),
+ * last gamma[N], getrf info; null pointers are skipped */
+int tvlm_uvlm_results(tvlm_uvlm_plan* plan, double* cl, double* cm, double* gamma, int32_t* info);
+int tvlm_uvlm_destroy(tvlm_uvlm_plan* plan);
+/* measurement: one simulation without capture, HIP events around each wake-induction launch on
+ * the plan's stream; returns summed kernel ms, launch count and (receiver, wake ring) pairs */
+int tvlm_uvlm_time_wake(tvlm_uvlm_plan* plan, double* total_ms, int32_t* launches, double* pairs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,18 @@ void orc_motion_transform_d(const double* motion, double t, double* T, double* d
     }
 }
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+/* OpenMP threads the parallel loops use (the CPU baseline reports it as its core count). */
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 /* ---- FP64 instantiation ---------------------------------------------------------------------- */
 #define REAL double
 #define FN(name) name##_d

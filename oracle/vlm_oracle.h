@@ -101,6 +101,7 @@ ORC_DECLARE(float, f)
  * motion[] = {u, h_amp, h_phase, a_amp, a_phase, omega, x_pivot}.
  * Writes the 4x4 column-major value T and its time derivative dT. */
 void orc_motion_transform_d(const double* motion, double t, double* T, double* dT);
+int orc_num_threads(void);
 
 #ifdef __cplusplus
 }

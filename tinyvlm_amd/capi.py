@@ -52,6 +52,11 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise TvlmError(f"HIP backend library not built: {LIB_PATH} (run __graft_entry__.build())")
+        # PyTorch ships its own libamdhip64.so.7 / libhsa-runtime64.so.1 (same sonames as /opt/rocm).
+        # Load it first so that this library binds to the runtime PyTorch uses: one HIP runtime
+        # per process, and device pointers from torch tensors are valid in our kernels.
+        import torch  # noqa: F401
+
         _lib = ctypes.CDLL(LIB_PATH)
         _lib.tvlm_last_error.restype = ctypes.c_char_p
         _lib.tvlm_backend_name.restype = ctypes.c_char_p

@@ -147,6 +147,34 @@ __device__ __forceinline__ v3<T> ring_exact(const v3<T>& c, const v3<T>& v0, con
     return inf;
 }
 
+// ---- panel metrics (vlm_backend_cpu.cpp:391-433, vlm_backend_cpud.cpp:258-300) -----------------
+// Normal normalize((v3-v1) x (v2-v0)); area 0.5(|f x b| + |b x e|); collocation point
+// 0.5(v0+v1) + k * 0.5(v2+v3-v0-v1), k = 1/2 below EPS_f else alpha / (2 (sin alpha + EPS_f)).
+template <typename T>
+__device__ __forceinline__ void panel_metrics(T alpha, const v3<T>& v0, const v3<T>& v1, const v3<T>& v2,
+                                              const v3<T>& v3_, v3<T>& colloc, v3<T>& normal, T& area) {
+    const v3<T> nv = cross(v3_ - v1, v2 - v0);
+    const T l = sqrt(dot(nv, nv));
+    normal = {nv.x / l, nv.y / l, nv.z / l};
+    const v3<T> f = v3_ - v0, b = v2 - v0, e = v1 - v0;
+    const v3<T> fb = cross(f, b), be = cross(b, e);
+    area = T(0.5) * (sqrt(dot(fb, fb)) + sqrt(dot(be, be)));
+    const T eps = (T)1.19209e-07f;  // EPS_f (vlm_types.hpp:24) in both precisions
+    const T factor = (alpha < eps) ? T(0.5) : T(0.5) * (alpha / (sin(alpha) + eps));
+    const v3<T> chord = T(0.5) * (((v2 + v3_) - v0) - v1);
+    colloc = T(0.5) * (v0 + v1) + factor * chord;
+}
+
+// One homogeneous point times a column-major 4x4 (row r of verts {P,4} * T^T, vlm_backend.cpp:59-67).
+template <typename T>
+__device__ __forceinline__ T displace_row(const T* M, int64_t m0, int64_t m1, int r, const T* x) {
+    T s = x[0] * M[r * m0 + 0 * m1];
+    s += x[1] * M[r * m0 + 1 * m1];
+    s += x[2] * M[r * m0 + 2 * m1];
+    s += x[3] * M[r * m0 + 3 * m1];
+    return s;
+}
+
 // ---- FP32 fast form ---------------------------------------------------------------------------
 // Same mathematics with the hardware reciprocal square root: coeff = (d1/|r1| - d2/|r2|) / (4 pi sq),
 // which equals the reference's (d1 |r2| - d2 |r1|) / (4 pi sq |r1| |r2|).  Cutoffs are tested on
@@ -183,6 +211,70 @@ __device__ __forceinline__ void fast_segment(float& ax, float& ay, float& az, co
     ax += coeff * crx;
     ay += sy * (coeff * cry);
     az += coeff * crz;
+}
+
+// ---- FP64 seeded form ---------------------------------------------------------------------------
+// FP64 arithmetic throughout, but 1/sqrt and 1/x from the single-precision hardware estimates
+// (v_rsq_f32, v_rcp_f32, relative error ~2^-23) refined by one Newton step in FP64 (relative error
+// ~2^-45) instead of the multi-instruction correctly rounded FP64 sqrt and division.  MI355X runs
+// FP64 add/mul/fma at the rate of non-packed FP32, so this costs about what the FP32 form costs
+// while keeping FP64 accuracy.  Cutoffs as the FP64 reference: |r| < RCUT <=> |r|^2 < RCUT^2.
+__device__ __forceinline__ double rsqrt_seeded(double x) {
+    const double y = (double)__builtin_amdgcn_rsqf((float)x);
+    return y * (1.5 - 0.5 * x * y * y);
+}
+__device__ __forceinline__ double rcp_seeded(double x) {
+    const double y = (double)__builtin_amdgcn_rcpf((float)x);
+    return y * (2.0 - x * y);
+}
+
+struct SeedVertex {
+    double x, y, z, r2, rinv;
+};
+
+__device__ __forceinline__ SeedVertex seed_vertex(double cx, double cy, double cz, double vx, double vy, double vz) {
+    SeedVertex f;
+    f.x = cx - vx;
+    f.y = cy - vy;
+    f.z = cz - vz;
+    f.r2 = f.x * f.x + f.y * f.y + f.z * f.z;
+    f.rinv = rsqrt_seeded(f.r2);
+    return f;
+}
+
+__device__ __forceinline__ void seed_segment(double& ax, double& ay, double& az, const SeedVertex& r1,
+                                             const SeedVertex& r2, double sy) {
+    const double crx = r1.y * r2.z - r1.z * r2.y;
+    const double cry = r1.z * r2.x - r1.x * r2.z;
+    const double crz = r1.x * r2.y - r1.y * r2.x;
+    const double sq = crx * crx + cry * cry + crz * crz;
+    const double r0x = r1.x - r2.x, r0y = r1.y - r2.y, r0z = r1.z - r2.z;
+    const double d1 = r0x * r1.x + r0y * r1.y + r0z * r1.z;
+    const double d2 = r0x * r2.x + r0y * r2.y + r0z * r2.z;
+    constexpr double cut2 = BS<double>::cut_norm * BS<double>::cut_norm;
+    const bool ok = (sq >= BS<double>::cut_cross) & (r1.r2 >= cut2) & (r2.r2 >= cut2);
+    const double t = (d1 * r1.rinv - d2 * r2.rinv) * rcp_seeded(BS<double>::fourpi * sq);
+    const double coeff = ok ? t : 0.0;
+    ax += coeff * crx;
+    ay += sy * (coeff * cry);
+    az += coeff * crz;
+}
+
+__device__ __forceinline__ void seed_ring(double& ax, double& ay, double& az, double cx, double cy, double cz,
+                                          const double* v0, const double* v1, const double* v2, const double* v3) {
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+        const double y = m == 0 ? cy : -cy;
+        const double sy = m == 0 ? 1.0 : -1.0;
+        const SeedVertex a = seed_vertex(cx, y, cz, v0[0], v0[1], v0[2]);
+        const SeedVertex b = seed_vertex(cx, y, cz, v1[0], v1[1], v1[2]);
+        const SeedVertex c = seed_vertex(cx, y, cz, v2[0], v2[1], v2[2]);
+        const SeedVertex d = seed_vertex(cx, y, cz, v3[0], v3[1], v3[2]);
+        seed_segment(ax, ay, az, a, b, sy);
+        seed_segment(ax, ay, az, b, c, sy);
+        seed_segment(ax, ay, az, c, d, sy);
+        seed_segment(ax, ay, az, d, a, sy);
+    }
 }
 
 // Ring with image; vertices given as coordinates.

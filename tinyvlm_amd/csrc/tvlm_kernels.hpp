@@ -1,0 +1,235 @@
+// tvlm_kernels.hpp -- Biot-Savart assembly kernels shared by the Backend entry points and the
+// device-resident solver loops.
+//
+// lhs block: a work-group owns 64 receivers (one per lane, colloc + normal in registers) and TC
+// rings whose 12 vertex coordinates are staged in LDS; its 4 waves take every 4th ring, so each
+// wave reads LDS as a broadcast and writes one coalesced 64-element column segment per ring.
+//
+// wake partials: the wake rings are cut into tiles of RH rows x SW spans; a work-group stages one
+// tile's vertex rows and circulations in LDS, each wave takes every 4th ring of the tile for the
+// work-group's 64 receivers, and the 4 waves reduce in LDS to one partial per (tile, receiver).
+// k_wake_reduce then sums the partials of all tiles in a fixed order (bitwise reproducible).
+#pragma once
+
+#include "tvlm_internal.hpp"
+
+namespace tvlm {
+namespace {
+
+constexpr int kRecv = 64;      // receivers per work-group (one per lane)
+constexpr int kWaves = 4;      // waves per work-group
+constexpr int kThreads = kRecv * kWaves;
+
+template <typename T>
+struct RingSource {
+    const T* verts;   // {ns+1, nc+1, 4} plane contiguous
+    int64_t vs2;      // plane stride of verts
+    int32_t ns, nc;   // panels spanwise / chordwise of the ring surface
+    const T* wake;    // {ns+1, nwv, 4} plane contiguous (TE columns only)
+    int64_t ws2;
+    int32_t nwv;
+    int32_t iteration;
+};
+
+template <typename T>
+struct Receivers {
+    const T* colloc;  // {ns, nc, 3}: element p + k*cs2
+    int64_t cs2;
+    const T* normals;
+    int64_t ns2;
+    int32_t m;        // number of receivers (panels)
+};
+
+// Velocity of the ring (v0, v1, v2, v3) with image at c in the form selected by MODE:
+// 0 exact (sigma = 0), 1 exact with core, 2 FP32 rsqrt form, 3 FP64 seeded form.
+template <typename T, int MODE>
+__device__ __forceinline__ v3<T> ring_vel(const v3<T>& c, const T* v0, const T* v1, const T* v2, const T* v3_,
+                                          T sigma) {
+    if constexpr (MODE == 2) {
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        fast_ring(ax, ay, az, c.x, c.y, c.z, v0, v1, v2, v3_);
+        return {ax, ay, az};
+    } else if constexpr (MODE == 3) {
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        seed_ring(ax, ay, az, c.x, c.y, c.z, v0, v1, v2, v3_);
+        return {ax, ay, az};
+    } else {
+        return ring_exact<T, MODE == 1>(c, {v0[0], v0[1], v0[2]}, {v1[0], v1[1], v1[2]}, {v2[0], v2[1], v2[2]},
+                                        {v3_[0], v3_[1], v3_[2]}, sigma);
+    }
+}
+
+// Loads ring r's vertices (v0, v1 = +1 span, v2 = +1 span +1 chord, v3 = +1 chord) from a vertex
+// grid whose row (chordwise) stride is ns+1.
+template <typename T>
+__device__ __forceinline__ void load_ring(T* dst, const T* grid, int64_t s2, int64_t base, int32_t ns) {
+    const int64_t off[4] = {base, base + 1, base + ns + 2, base + ns + 1};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        dst[3 * q + 0] = grid[off[q]];
+        dst[3 * q + 1] = grid[off[q] + s2];
+        dst[3 * q + 2] = grid[off[q] + 2 * s2];
+    }
+}
+
+// lhs block of one (receiver surface, ring surface) pair.  lhs points at (offset_i, offset_j).
+template <typename T, int TC, int MODE>  // MODE as ring_vel
+__global__ __launch_bounds__(kThreads) void k_lhs_block(T* __restrict__ lhs, int64_t ld, Receivers<T> rc,
+                                                        RingSource<T> src, T sigma) {
+    __shared__ T sv[TC][12];
+    const int n_rings = src.ns * src.nc;
+    const int col0 = blockIdx.y * TC;
+    for (int e = threadIdx.x; e < TC * 4; e += kThreads) {
+        const int r = e >> 2, q = e & 3;
+        const int lidx = col0 + r;
+        if (lidx < n_rings) {
+            const int64_t base = lidx + lidx / src.ns;  // vertex 0 of ring lidx (vlm_backend_cpu.cpp:111)
+            const int64_t off = q == 0 ? base : q == 1 ? base + 1 : q == 2 ? base + src.ns + 2 : base + src.ns + 1;
+            sv[r][3 * q + 0] = src.verts[off];
+            sv[r][3 * q + 1] = src.verts[off + src.vs2];
+            sv[r][3 * q + 2] = src.verts[off + 2 * src.vs2];
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kRecv - 1);
+    const int wave = threadIdx.x / kRecv;
+    const int i = blockIdx.x * kRecv + lane;
+    if (i >= rc.m) return;
+    const v3<T> c = {rc.colloc[i], rc.colloc[i + rc.cs2], rc.colloc[i + 2 * rc.cs2]};
+    const v3<T> n = {rc.normals[i], rc.normals[i + rc.ns2], rc.normals[i + 2 * rc.ns2]};
+    const int ncols = min(TC, n_rings - col0);
+    const int te0 = (src.nc - 1) * src.ns;
+    for (int r = wave; r < ncols; r += kWaves) {
+        const int lidx = col0 + r;
+        T* dst = lhs + i + (int64_t)lidx * ld;
+        T acc = *dst;
+        acc += dot(ring_vel<T, MODE>(c, sv[r], sv[r] + 3, sv[r] + 6, sv[r] + 9, sigma), n);
+        if (src.iteration > 0 && lidx >= te0) {
+            // trailing-edge column: wake rings of rows nwv-2-k, k < iteration (vlm_backend_cpu.cpp:121-139)
+            const int j = lidx - te0;
+            for (int k = 0; k < src.iteration; k++) {
+                T w[12];
+                load_ring(w, src.wake, src.ws2, (int64_t)j + (int64_t)(src.nwv - 2 - k) * (src.ns + 1), src.ns);
+                acc += dot(ring_vel<T, MODE>(c, w, w + 3, w + 6, w + 9, sigma), n);
+            }
+        }
+        *dst = acc;
+    }
+}
+
+// ---- wake influence ---------------------------------------------------------------------------
+constexpr int kMaxWakes = TVLM_MAX_SURFACES;
+
+template <typename T>
+struct WakeSurface {
+    const T* verts;   // {ns+1, nwv, 4} plane contiguous
+    int64_t ws2;
+    const T* gamma;   // {ns, nwv-1} contiguous
+    int32_t ns, nwv;
+    int32_t row_begin, row_end;  // ring rows [row_begin, row_end)
+    int32_t sw, rh;              // tile extent: spans, rows
+    int32_t tiles_span, tile_begin;
+};
+
+template <typename T>
+struct WakeTable {
+    int32_t n;
+    int32_t total_tiles;
+    WakeSurface<T> s[kMaxWakes];
+};
+
+template <typename T>
+struct RecvTable {
+    int32_t n;
+    int32_t total;
+    int32_t offset[kMaxWakes + 1];
+    Receivers<T> r[kMaxWakes];
+};
+
+constexpr int kTileRings = 256;  // target rings per tile
+constexpr int kMaxSpanTile = 64;
+
+template <typename T>
+__device__ __forceinline__ int find_surface(const RecvTable<T>& t, int p) {
+    int s = 0;
+    while (s + 1 < t.n && p >= t.offset[s + 1]) s++;
+    return s;
+}
+
+// TS: storage type of the inputs and partials, T: compute type (float for MODE 2, double for MODE 3)
+template <typename TS, typename T, int MODE>
+__global__ __launch_bounds__(kThreads) void k_wake_partials(TS* __restrict__ partial, RecvTable<TS> rt,
+                                                            WakeTable<TS> wt, T sigma) {
+    // LDS: vertex rows of the tile (rh+1) x (sw+1) x 3 and circulations rh x sw.  With
+    // rh = floor(kTileRings / sw) and 1 <= sw <= kMaxSpanTile, (rh+1)(sw+1) <= 2 (kTileRings+1).
+    __shared__ T sverts[2 * (kTileRings + 1) * 3];
+    __shared__ T sgam[kTileRings];
+    __shared__ T sred[kWaves][kRecv];
+
+    const int tile = blockIdx.y;
+    int w = 0;
+    while (w + 1 < wt.n && tile >= wt.s[w + 1].tile_begin) w++;
+    const WakeSurface<TS>& ws = wt.s[w];
+    const int local = tile - ws.tile_begin;
+    const int ts = local % ws.tiles_span, tr = local / ws.tiles_span;
+    const int s0 = ts * ws.sw, r0 = ws.row_begin + tr * ws.rh;
+    const int swe = min(ws.sw, ws.ns - s0);
+    const int rhe = min(ws.rh, ws.row_end - r0);
+    const int vw = swe + 1;
+    // stage vertex rows r0 .. r0+rhe and circulations
+    for (int e = threadIdx.x; e < (rhe + 1) * vw; e += kThreads) {
+        const int q = e / vw, sp = e % vw;
+        const int64_t g = (int64_t)(s0 + sp) + (int64_t)(r0 + q) * (ws.ns + 1);
+        sverts[3 * e + 0] = (T)ws.verts[g];
+        sverts[3 * e + 1] = (T)ws.verts[g + ws.ws2];
+        sverts[3 * e + 2] = (T)ws.verts[g + 2 * ws.ws2];
+    }
+    for (int e = threadIdx.x; e < rhe * swe; e += kThreads) {
+        const int q = e / swe, sp = e % swe;
+        sgam[e] = (T)ws.gamma[(int64_t)(r0 + q) * ws.ns + s0 + sp];
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & (kRecv - 1);
+    const int wave = threadIdx.x / kRecv;
+    const int p = blockIdx.x * kRecv + lane;
+    T part = T(0);
+    if (p < rt.total) {
+        const int si = find_surface(rt, p);
+        const Receivers<TS>& rc = rt.r[si];
+        const int li = p - rt.offset[si];
+        const v3<T> c = {(T)rc.colloc[li], (T)rc.colloc[li + rc.cs2], (T)rc.colloc[li + 2 * rc.cs2]};
+        const v3<T> n = {(T)rc.normals[li], (T)rc.normals[li + rc.ns2], (T)rc.normals[li + 2 * rc.ns2]};
+        v3<T> acc = {T(0), T(0), T(0)};
+        for (int q = wave; q < rhe * swe; q += kWaves) {
+            const int row = q / swe, sp = q % swe;
+            const T* a = sverts + 3 * (row * vw + sp);
+            const T* d = sverts + 3 * ((row + 1) * vw + sp);
+            const T g = sgam[q];
+            const v3<T> ind = ring_vel<T, MODE>(c, a, a + 3, d + 3, d, sigma);
+            acc.x += ind.x * g;
+            acc.y += ind.y * g;
+            acc.z += ind.z * g;
+        }
+        part = dot(acc, n);
+    }
+    sred[wave][lane] = part;
+    __syncthreads();
+    if (wave == 0 && p < rt.total) {
+        const T s = (sred[0][lane] + sred[1][lane]) + (sred[2][lane] + sred[3][lane]);
+        partial[(int64_t)tile * rt.total + p] = (TS)s;
+    }
+}
+
+template <typename T>
+__global__ void k_wake_reduce(T* __restrict__ rhs, int64_t rhs_stride, const T* __restrict__ partial, int tiles,
+                              int total) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= total) return;
+    T s = T(0);
+    for (int t = 0; t < tiles; t++) s += partial[(int64_t)t * total + p];
+    rhs[(int64_t)p * rhs_stride] -= s;
+}
+
+}  // namespace
+}  // namespace tvlm

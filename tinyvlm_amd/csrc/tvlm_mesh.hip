@@ -92,18 +92,13 @@ __global__ void k_mesh_metrics(T alpha, const T* __restrict__ V, int64_t vs2, in
         return {V[o], V[o + vs2], V[o + 2 * vs2]};
     };
     const v3<T> v0 = vtx(i, j), v1 = vtx(i + 1, j), v2 = vtx(i + 1, j + 1), v3_ = vtx(i, j + 1);
-    const v3<T> nv = cross(v3_ - v1, v2 - v0);
-    const T l = sqrt(dot(nv, nv));
-    normals[p] = nv.x / l;
-    normals[p + ns2] = nv.y / l;
-    normals[p + 2 * ns2] = nv.z / l;
-    const v3<T> f = v3_ - v0, b = v2 - v0, e = v1 - v0;
-    const v3<T> fb = cross(f, b), be = cross(b, e);
-    areas[i + (int64_t)j * as1] = T(0.5) * (sqrt(dot(fb, fb)) + sqrt(dot(be, be)));
-    const T eps = (T)1.19209e-07f;  // EPS_f (vlm_types.hpp:24) in both precisions
-    const T factor = (alpha < eps) ? T(0.5) : T(0.5) * (alpha / (sin(alpha) + eps));
-    const v3<T> chord = T(0.5) * (((v2 + v3_) - v0) - v1);
-    const v3<T> cp = T(0.5) * (v0 + v1) + factor * chord;
+    v3<T> cp, nrm;
+    T area;
+    panel_metrics(alpha, v0, v1, v2, v3_, cp, nrm, area);
+    normals[p] = nrm.x;
+    normals[p + ns2] = nrm.y;
+    normals[p + 2 * ns2] = nrm.z;
+    areas[i + (int64_t)j * as1] = area;
     colloc[p] = cp.x;
     colloc[p + cs2] = cp.y;
     colloc[p + 2 * cs2] = cp.z;
@@ -236,14 +231,7 @@ __global__ void k_displace(const T* __restrict__ M, int64_t m0, int64_t m1, cons
     if (p >= nverts) return;
     const T x[4] = {X[p], X[p + xs2], X[p + 2 * xs2], X[p + 3 * xs2]};
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        // verts {P,4} = verts_init {P,4} * T^T  (vlm/src/vlm_backend.cpp:59-67)
-        T s = x[0] * M[r * m0 + 0 * m1];
-        s += x[1] * M[r * m0 + 1 * m1];
-        s += x[2] * M[r * m0 + 2 * m1];
-        s += x[3] * M[r * m0 + 3 * m1];
-        Y[p + r * ys2] = s;
-    }
+    for (int r = 0; r < 4; r++) Y[p + r * ys2] = displace_row(M, m0, m1, r, x);
 }
 
 // ---- gamma_wake_from_coeffs (vlm_backend_cpu.cpp:470-503) -----------------------------------------

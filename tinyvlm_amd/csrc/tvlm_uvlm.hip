@@ -1,0 +1,711 @@
+// tvlm_uvlm.hip -- device-resident unsteady VLM time loop (the caller of the hot path, UVLM::run,
+// vlm/src/vlm.cpp:358-509, kept on the GPU end to end).
+//
+// The prescribed kinematics are a function of time only, so the host evaluates the dual 4x4 of
+// every surface at every step once (value T and derivative dT) and uploads them; the loop then
+// needs no host interaction: per step
+//   k_pre    : wake shed of the trailing-edge vertex row, kinematic velocities -(dT [x;1]) at the
+//              t = 0 collocation points, rhs = -(V . n)                        (vlm.cpp:412-438)
+//   k_wake_partials + k_wake_reduce : rhs -= wake induction                   (vlm.cpp:439)
+//   solve    : plain mode: LU forward/backward substitution (getrs);
+//              fast mode : gamma = A^{-1} rhs with the FP64 inverse formed once from the LU
+//   k_post   : gamma post-processing, shed of Gamma_TE into the wake, unsteady forces, CL and CM
+//              of all surfaces (one work-group, fixed-order reductions)       (vlm.cpp:441-487)
+//   k_move   : displacement by T(t+dt) and panel metrics                      (vlm.cpp:489-507)
+// The fast mode evaluates the wake induction in the FP64 seeded form (FP64 arithmetic, rsqrt and
+// reciprocal from the FP32 hardware estimates plus one Newton step) and replays the whole loop as
+// one hipGraph.
+#include <cstring>
+#include <vector>
+
+#include "tvlm_kernels.hpp"
+
+struct tvlm_uvlm_plan;
+
+namespace tvlm {
+namespace {
+
+constexpr int kMaxS = TVLM_MAX_SURFACES;
+
+struct USurf {
+    double *verts_pos, *verts, *colloc, *colloc_h, *normals, *areas, *vel, *forces;
+    double *gprev, *gdelta, *gdt, *wake, *gamma_wake;
+    int32_t ns, nc, off, v_off, te_off;
+};
+
+struct UTab {
+    int32_t n, N, NV, NTE;
+    int32_t t_steps;
+    USurf s[kMaxS];
+};
+
+__device__ __forceinline__ int surf_of(const UTab& t, int idx, int which) {
+    // which: 0 panel offsets, 1 vertex offsets, 2 trailing-edge copy offsets
+    int m = 0;
+    while (m + 1 < t.n) {
+        const int next = which == 0 ? t.s[m + 1].off : which == 1 ? t.s[m + 1].v_off : t.s[m + 1].te_off;
+        if (idx < next) break;
+        m++;
+    }
+    return m;
+}
+
+// wake shed + kinematic velocities + rhs velocities
+__global__ void k_pre(UTab tab, const double* __restrict__ dT_all, int step, double* __restrict__ rhs) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < tab.N) {
+        const int m = surf_of(tab, e, 0);
+        const USurf& S = tab.s[m];
+        const int p = e - S.off;
+        const int np = S.ns * S.nc;
+        const double* d = dT_all + ((int64_t)step * tab.n + m) * 16;
+        const double x = S.colloc_h[p], y = S.colloc_h[p + np], z = S.colloc_h[p + 2 * np];
+        double V[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            V[r] = -(d[r] * x + d[r + 4] * y + d[r + 8] * z + d[r + 12]);
+            S.vel[p + r * np] = V[r];
+        }
+        rhs[e] = -(V[0] * S.normals[p] + V[1] * S.normals[p + np] + V[2] * S.normals[p + 2 * np]);
+        return;
+    }
+    const int q = e - tab.N;
+    if (q >= tab.NTE) return;
+    const int m = surf_of(tab, q, 2);
+    const USurf& S = tab.s[m];
+    const int l = q - S.te_off;
+    const int vs1 = S.ns + 1;
+    const int i = l % vs1, k = l / vs1;
+    const int64_t vs2 = (int64_t)vs1 * (S.nc + 1), ws2 = (int64_t)vs1 * tab.t_steps;
+    const int row = tab.t_steps - 1 - step;
+    S.wake[i + (int64_t)row * vs1 + k * ws2] = S.verts[i + (int64_t)S.nc * vs1 + k * vs2];
+}
+
+// y(partial cb) = A(:, cols of block cb) x(cols); 64 rows per work-group, 4 waves split the columns
+__global__ __launch_bounds__(256) void k_gemv_part(const double* __restrict__ A, int64_t lda, int n, int cbs,
+                                                   const double* __restrict__ x, double* __restrict__ part) {
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = blockIdx.x * 64 + lane;
+    const int c0 = blockIdx.y * cbs, c1 = min(n, c0 + cbs);
+    double s = 0.0;
+    if (row < n)
+        for (int c = c0 + wave; c < c1; c += 4) s += A[row + (int64_t)c * lda] * x[c];
+    red[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && row < n) part[(int64_t)blockIdx.y * n + row] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+}
+
+constexpr int kPostThreads = 1024;
+
+template <int NT>
+__device__ __forceinline__ double bsum(double v, double* lds) {
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (t < s) lds[t] += lds[t + s];
+        __syncthreads();
+    }
+    const double r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kPostThreads) void k_post(UTab tab, const double* __restrict__ gsrc, int cb, int step,
+                                                       double inv_dt, const double* __restrict__ fs_all,
+                                                       const double* __restrict__ ref_all, double rho,
+                                                       double* __restrict__ gamma, double* __restrict__ cl_out,
+                                                       double* __restrict__ cm_out) {
+    __shared__ double lds[kPostThreads];
+    const int t = threadIdx.x;
+    const int N = tab.N;
+    for (int P = t; P < N; P += kPostThreads) {
+        double g = 0.0;
+        for (int c = 0; c < cb; c++) g += gsrc[(int64_t)c * N + P];
+        gamma[P] = g;
+    }
+    __syncthreads();
+    const v3<double> fs = {fs_all[3 * step], fs_all[3 * step + 1], fs_all[3 * step + 2]};
+    const v3<double> ref = {ref_all[3 * step], ref_all[3 * step + 1], ref_all[3 * step + 2]};
+    // lift axis = normalize(freestream x (0,1,0))  (vlm/src/vlm_backend.cpp:86)
+    v3<double> la = cross(fs, v3<double>{0.0, 1.0, 0.0});
+    const double ll = sqrt(dot(la, la));
+    la = {la.x / ll, la.y / ll, la.z / ll};
+    const double q = 0.5 * rho * dot(fs, fs);
+    double cl_num = 0.0, tot_a = 0.0, tot_mac = 0.0;
+    double cmn[3] = {0.0, 0.0, 0.0};
+    const bool forces = step > 0;  // the reference skips the coefficients of step 0 (vlm.cpp:461)
+    for (int m = 0; m < tab.n; m++) {
+        const USurf& S = tab.s[m];
+        const int ns = S.ns, nc = S.nc, np = ns * nc;
+        const int64_t vs1 = ns + 1, vs2 = vs1 * (nc + 1);
+        double a_acc = 0.0, c_acc = 0.0, mx = 0.0, my = 0.0, mz = 0.0, mac_acc = 0.0;
+        for (int p = t; p < np; p += kPostThreads) {
+            const int i = p % ns, j = p / ns;
+            const double g = gamma[S.off + p];
+            const double gd = j > 0 ? g + (-1.0) * gamma[S.off + p - ns] : g;
+            const double gt = (g + (-1.0) * S.gprev[p]) * inv_dt;
+            S.gdelta[p] = gd;
+            S.gdt[p] = gt;
+            S.gprev[p] = g;
+            if (j == nc - 1) S.gamma_wake[i + (int64_t)(tab.t_steps - 2 - step) * ns] = g;
+            if (forces) {
+                const int64_t a = i + (int64_t)j * vs1, b = a + 1;
+                const v3<double> U = {S.vel[p], S.vel[p + np], S.vel[p + 2 * np]};
+                const v3<double> va = {S.verts[a], S.verts[a + vs2], S.verts[a + 2 * vs2]};
+                const v3<double> vb = {S.verts[b], S.verts[b + vs2], S.verts[b + 2 * vs2]};
+                const v3<double> cr = cross(U, vb - va);
+                const double s1 = rho * gd;
+                const double u = rho * gt * S.areas[p];
+                const v3<double> F = {s1 * cr.x + u * S.normals[p], s1 * cr.y + u * S.normals[p + np],
+                                      s1 * cr.z + u * S.normals[p + 2 * np]};
+                S.forces[p] = F.x;
+                S.forces[p + np] = F.y;
+                S.forces[p + 2 * np] = F.z;
+                c_acc += dot(F, la);
+                const v3<double> lever = {0.5 * (va.x + vb.x) - ref.x, 0.5 * (va.y + vb.y) - ref.y,
+                                          0.5 * (va.z + vb.z) - ref.z};
+                const v3<double> mo = cross(lever, F);
+                mx += mo.x;
+                my += mo.y;
+                mz += mo.z;
+                a_acc += S.areas[p];
+            }
+        }
+        if (forces) {
+            for (int i = t; i < ns; i += kPostThreads) {
+                const int64_t te = (int64_t)nc * vs1;
+                const double* V = S.verts;
+                const v3<double> d0 = {V[i] - V[i + te], V[i + vs2] - V[i + te + vs2], V[i + 2 * vs2] - V[i + te + 2 * vs2]};
+                const v3<double> d1 = {V[i + 1] - V[i + 1 + te], V[i + 1 + vs2] - V[i + 1 + te + vs2],
+                                       V[i + 1 + 2 * vs2] - V[i + 1 + te + 2 * vs2]};
+                const v3<double> d3 = {V[i + 1] - V[i], V[i + 1 + vs2] - V[i + vs2], V[i + 1 + 2 * vs2] - V[i + 2 * vs2]};
+                const double c0 = sqrt(dot(d0, d0)), c1 = sqrt(dot(d1, d1)), w = sqrt(dot(d3, d3));
+                mac_acc += 0.5 * (c0 * c0 + c1 * c1) * w;
+            }
+            const double area = bsum<kPostThreads>(a_acc, lds);
+            const double cs = bsum<kPostThreads>(c_acc, lds);
+            const double sx = bsum<kPostThreads>(mx, lds);
+            const double sy = bsum<kPostThreads>(my, lds);
+            const double sz = bsum<kPostThreads>(mz, lds);
+            const double macs = bsum<kPostThreads>(mac_acc, lds);
+            // coeff_cl / coeff_cm / mesh_mac / *_multibody (vlm_backend_cpu.cpp:323-362, vlm_backend.cpp:71-139)
+            const double mac = macs / area;
+            const double cl_m = cs / (q * area);
+            const double den = q * area * mac;
+            cl_num += cl_m * area;
+            cmn[0] += (sx / den) * area * mac;
+            cmn[1] += (sy / den) * area * mac;
+            cmn[2] += (sz / den) * area * mac;
+            tot_a += area;
+            tot_mac += mac;
+        }
+    }
+    if (t == 0) {
+        cl_out[step] = forces ? cl_num / tot_a : 0.0;
+        for (int k = 0; k < 3; k++) cm_out[3 * step + k] = forces ? cmn[k] / (tot_a * tot_mac) : 0.0;
+    }
+}
+
+// displacement by T(step) (verts = T verts_pos) and panel metrics at alpha = 0
+__global__ void k_move(UTab tab, const double* __restrict__ T_all, int step) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < tab.NV) {
+        const int m = surf_of(tab, e, 1);
+        const USurf& S = tab.s[m];
+        const int p = e - S.v_off;
+        const int64_t nv = (int64_t)(S.ns + 1) * (S.nc + 1);
+        const double* M = T_all + ((int64_t)step * tab.n + m) * 16;
+        const double x[4] = {S.verts_pos[p], S.verts_pos[p + nv], S.verts_pos[p + 2 * nv], S.verts_pos[p + 3 * nv]};
+#pragma unroll
+        for (int r = 0; r < 4; r++) S.verts[p + r * nv] = displace_row(M, 1, 4, r, x);
+        return;
+    }
+    const int P = e - tab.NV;
+    if (P >= tab.N) return;
+    const int m = surf_of(tab, P, 0);
+    const USurf& S = tab.s[m];
+    const int p = P - S.off;
+    const int ns = S.ns, nc = S.nc, np = ns * nc;
+    const int i = p % ns, j = p / ns;
+    const int64_t vs1 = ns + 1, nv = vs1 * (nc + 1);
+    const double* M = T_all + ((int64_t)step * tab.n + m) * 16;
+    auto vtx = [&](int ii, int jj) -> v3<double> {
+        const int64_t o = ii + jj * vs1;
+        const double x[4] = {S.verts_pos[o], S.verts_pos[o + nv], S.verts_pos[o + 2 * nv], S.verts_pos[o + 3 * nv]};
+        return {displace_row(M, 1, 4, 0, x), displace_row(M, 1, 4, 1, x), displace_row(M, 1, 4, 2, x)};
+    };
+    v3<double> c, n;
+    double area;
+    panel_metrics(0.0, vtx(i, j), vtx(i + 1, j), vtx(i + 1, j + 1), vtx(i, j + 1), c, n, area);
+    S.colloc[p] = c.x;
+    S.colloc[p + np] = c.y;
+    S.colloc[p + 2 * np] = c.z;
+    S.normals[p] = n.x;
+    S.normals[p + np] = n.y;
+    S.normals[p + 2 * np] = n.z;
+    S.areas[p] = area;
+}
+
+__global__ void k_fill64(double* x, int64_t n, double v) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q < n) x[q] = v;
+}
+
+__global__ void k_eye(double* a, int n) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q < (int64_t)n * n) a[q] = (q % n) == (q / n) ? 1.0 : 0.0;
+}
+
+inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+}  // namespace tvlm
+
+using namespace tvlm;
+
+struct tvlm_uvlm_plan {
+    int32_t mode = 0;
+    int32_t n_surf = 0;
+    int32_t N = 0;
+    int64_t t_steps = 0;
+    double dt = 0.0;
+    hipStream_t stream = nullptr;
+    UTab tab{};
+    std::vector<void*> allocs;
+    double *init[kMaxS] = {};
+    double *lhs = nullptr, *ainv = nullptr, *rhs = nullptr, *gamma = nullptr, *gpart = nullptr, *wpart = nullptr;
+    double *cl = nullptr, *cm = nullptr, *T_all = nullptr, *dT_all = nullptr, *fs_all = nullptr, *ref_all = nullptr;
+    int32_t *ipiv = nullptr, *info = nullptr;
+    int cb = 1, cbs = 0;
+    size_t wpart_elems = 0;
+    bool motion_set = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> wake_events;  // [2 * step]: start/stop around each wake launch
+    bool instrument = false;
+
+    int alloc(void** p, size_t bytes) {
+        TVLM_HIP(hipMalloc(p, bytes));
+        allocs.push_back(*p);
+        return TVLM_OK;
+    }
+};
+
+namespace tvlm {
+namespace {
+
+int plan_views(tvlm_uvlm_plan* P, int m, tvlm_view& verts, tvlm_view& colloc, tvlm_view& normals, tvlm_view& areas,
+               tvlm_view& wake) {
+    const USurf& S = P->tab.s[m];
+    const int64_t ns = S.ns, nc = S.nc;
+    verts = {S.verts, {ns + 1, nc + 1, 4}, {1, ns + 1, (ns + 1) * (nc + 1)}};
+    colloc = {S.colloc, {ns, nc, 3}, {1, ns, ns * nc}};
+    normals = {S.normals, {ns, nc, 3}, {1, ns, ns * nc}};
+    areas = {S.areas, {ns, nc, 1}, {1, ns, ns * nc}};
+    wake = {S.wake, {ns + 1, P->t_steps, 4}, {1, ns + 1, (ns + 1) * P->t_steps}};
+    return TVLM_OK;
+}
+
+// The wake tile table for iteration `it` (rows [t_steps-1-it, t_steps-1) of every surface).
+void wake_tables(const tvlm_uvlm_plan* P, int it, RecvTable<double>& rt, WakeTable<double>& wt, int& tiles) {
+    rt = {};
+    wt = {};
+    rt.n = P->n_surf;
+    for (int m = 0; m < P->n_surf; m++) {
+        const USurf& S = P->tab.s[m];
+        const int np = S.ns * S.nc;
+        rt.offset[m] = S.off;
+        rt.r[m] = {S.colloc, np, S.normals, np, np};
+    }
+    rt.offset[P->n_surf] = P->N;
+    rt.total = P->N;
+    tiles = 0;
+    for (int m = 0; m < P->n_surf; m++) {
+        const USurf& S = P->tab.s[m];
+        WakeSurface<double>& w = wt.s[wt.n];
+        w.verts = S.wake;
+        w.ws2 = (int64_t)(S.ns + 1) * P->t_steps;
+        w.gamma = S.gamma_wake;
+        w.ns = S.ns;
+        w.nwv = (int32_t)P->t_steps;
+        w.row_begin = (int32_t)P->t_steps - 1 - it;
+        w.row_end = (int32_t)P->t_steps - 1;
+        w.sw = S.ns < kMaxSpanTile ? S.ns : kMaxSpanTile;
+        w.rh = kTileRings / w.sw < 1 ? 1 : kTileRings / w.sw;
+        w.tiles_span = (S.ns + w.sw - 1) / w.sw;
+        w.tile_begin = tiles;
+        tiles += w.tiles_span * ((w.row_end - w.row_begin + w.rh - 1) / w.rh);
+        wt.n++;
+    }
+    wt.total_tiles = tiles;
+}
+
+int setup(tvlm_uvlm_plan* P) {
+    hipStream_t st = P->stream;
+    const int n = P->n_surf;
+    // pose at t = 0 (vlm.cpp:366-373): verts_pos = T(0) verts_init; verts = verts_pos; metrics; colloc_h
+    for (int m = 0; m < n; m++) {
+        USurf& S = P->tab.s[m];
+        const int64_t nv = (int64_t)(S.ns + 1) * (S.nc + 1);
+        tvlm_view tr = {P->T_all + (int64_t)m * 16, {4, 4, 1}, {1, 4, 16}};
+        tvlm_view vi = {P->init[m], {S.ns + 1, S.nc + 1, 4}, {1, S.ns + 1, nv}};
+        tvlm_view vp = {S.verts_pos, {S.ns + 1, S.nc + 1, 4}, {1, S.ns + 1, nv}};
+        TVLM_TRY(tvlm_displace_wing_f64(1, &tr, &vp, &vi, st));
+        TVLM_HIP(hipMemcpyAsync(S.verts, S.verts_pos, sizeof(double) * nv * 4, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_fill64, dim3(nblk(S.ns * S.nc)), dim3(256), 0, st, S.gprev, (int64_t)S.ns * S.nc, 0.0);
+        TVLM_LAUNCHED();
+    }
+    std::vector<tvlm_view> V(n), C(n), Nn(n), A(n), W(n);
+    for (int m = 0; m < n; m++) plan_views(P, m, V[m], C[m], Nn[m], A[m], W[m]);
+    TVLM_TRY(tvlm_mesh_metrics_f64(0.0, n, V.data(), C.data(), Nn.data(), A.data(), st));
+    for (int m = 0; m < n; m++) {
+        USurf& S = P->tab.s[m];
+        TVLM_HIP(hipMemcpyAsync(S.colloc_h, S.colloc, sizeof(double) * S.ns * S.nc * 3, hipMemcpyDeviceToDevice, st));
+    }
+    const int64_t NN = (int64_t)P->N * P->N;
+    hipLaunchKernelGGL(k_fill64, dim3(nblk(NN)), dim3(256), 0, st, P->lhs, NN, 0.0);
+    TVLM_LAUNCHED();
+    hipLaunchKernelGGL(k_fill64, dim3(nblk(P->t_steps)), dim3(256), 0, st, P->cl, (int64_t)P->t_steps, 0.0);
+    TVLM_LAUNCHED();
+    // LHS once, iteration 0, and its LU (vlm.cpp:402-403)
+    tvlm_view lhs = {P->lhs, {P->N, P->N, 1}, {1, P->N, NN}};
+    TVLM_TRY(tvlm_lhs_assemble_f64(&lhs, n, C.data(), Nn.data(), V.data(), W.data(), 0, 0.0, st));
+    TVLM_TRY(tvlm_getrf_f64(&lhs, P->ipiv, P->info, st));
+    if (P->mode == 1) {
+        hipLaunchKernelGGL(k_eye, dim3(nblk(NN)), dim3(256), 0, st, P->ainv, P->N);
+        TVLM_LAUNCHED();
+        tvlm_view x = {P->ainv, {P->N, P->N, 1}, {1, P->N, NN}};
+        TVLM_TRY(tvlm_getrs_f64(&lhs, P->ipiv, &x, st));
+    }
+    return TVLM_OK;
+}
+
+int enqueue_step(tvlm_uvlm_plan* P, int i) {
+    hipStream_t st = P->stream;
+    UTab& tab = P->tab;
+    hipLaunchKernelGGL(k_pre, dim3(nblk(tab.N + tab.NTE)), dim3(256), 0, st, tab, P->dT_all, i, P->rhs);
+    TVLM_LAUNCHED();
+    if (i > 0) {
+        RecvTable<double> rt;
+        WakeTable<double> wt;
+        int tiles = 0;
+        wake_tables(P, i, rt, wt, tiles);
+        TVLM_REQUIRE((size_t)tiles * P->N <= P->wpart_elems, "wake partial buffer too small");
+        dim3 grid((P->N + kRecv - 1) / kRecv, tiles);
+        if (P->instrument) TVLM_HIP(hipEventRecord(P->wake_events[2 * i], st));
+        if (P->mode == 1)
+            hipLaunchKernelGGL((k_wake_partials<double, double, 3>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt, 0.0);
+        else
+            hipLaunchKernelGGL((k_wake_partials<double, double, 0>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt, 0.0);
+        TVLM_LAUNCHED();
+        if (P->instrument) TVLM_HIP(hipEventRecord(P->wake_events[2 * i + 1], st));
+        hipLaunchKernelGGL(k_wake_reduce<double>, dim3(nblk(P->N)), dim3(256), 0, st, P->rhs, (int64_t)1, P->wpart,
+                           tiles, P->N);
+        TVLM_LAUNCHED();
+    }
+    const double* gsrc;
+    int cb;
+    if (P->mode == 1) {
+        hipLaunchKernelGGL(k_gemv_part, dim3((P->N + 63) / 64, P->cb), dim3(256), 0, st, P->ainv, (int64_t)P->N, P->N,
+                           P->cbs, P->rhs, P->gpart);
+        TVLM_LAUNCHED();
+        gsrc = P->gpart;
+        cb = P->cb;
+    } else {
+        tvlm_view lhs = {P->lhs, {P->N, P->N, 1}, {1, P->N, (int64_t)P->N * P->N}};
+        tvlm_view x = {P->rhs, {P->N, 1, 1}, {1, P->N, P->N}};
+        TVLM_TRY(tvlm_getrs_f64(&lhs, P->ipiv, &x, st));
+        gsrc = P->rhs;
+        cb = 1;
+    }
+    hipLaunchKernelGGL(k_post, dim3(1), dim3(kPostThreads), 0, st, tab, gsrc, cb, i, 1.0 / P->dt, P->fs_all,
+                       P->ref_all, 1.0, P->gamma, P->cl, P->cm);
+    TVLM_LAUNCHED();
+    hipLaunchKernelGGL(k_move, dim3(nblk(tab.NV + tab.N)), dim3(256), 0, st, tab, P->T_all, i + 1);
+    TVLM_LAUNCHED();
+    return TVLM_OK;
+}
+
+}  // namespace
+}  // namespace tvlm
+
+extern "C" {
+
+int tvlm_uvlm_create(tvlm_uvlm_plan** out, int32_t n_surf, const int64_t* ns, const int64_t* nc,
+                     const double* const* verts_init, int64_t t_steps, double dt, int32_t mode, void* stream) {
+    TVLM_REQUIRE(out && ns && nc && verts_init, "null argument");
+    TVLM_REQUIRE(n_surf >= 1 && n_surf <= TVLM_MAX_SURFACES, "n_surf %d out of range", n_surf);
+    TVLM_REQUIRE(t_steps >= 2 && t_steps < (1 << 24), "t_steps %lld out of range", (long long)t_steps);
+    TVLM_REQUIRE(dt > 0.0, "dt must be positive");
+    TVLM_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (plain FP64) or 1 (FP32 wake, inverse solve, graph)");
+    auto* P = new tvlm_uvlm_plan();
+    P->mode = mode;
+    P->n_surf = n_surf;
+    P->t_steps = t_steps;
+    P->dt = dt;
+    P->stream = as_stream(stream);
+    UTab& tab = P->tab;
+    tab.n = n_surf;
+    tab.t_steps = (int32_t)t_steps;
+    int off = 0, voff = 0, teoff = 0;
+    int rc = TVLM_OK;
+    auto fail = [&](int code) {
+        for (void* p : P->allocs) (void)hipFree(p);
+        delete P;
+        return code;
+    };
+    size_t max_tiles = 0;
+    for (int m = 0; m < n_surf; m++) {
+        if (ns[m] < 1 || nc[m] < 1 || !verts_init[m]) {
+            set_error("surface %d: bad dimensions", m);
+            return fail(TVLM_ERR_INVALID);
+        }
+        USurf& S = tab.s[m];
+        S.ns = (int32_t)ns[m];
+        S.nc = (int32_t)nc[m];
+        S.off = off;
+        S.v_off = voff;
+        S.te_off = teoff;
+        const size_t np = (size_t)ns[m] * nc[m], nv = (size_t)(ns[m] + 1) * (nc[m] + 1);
+        off += (int)np;
+        voff += (int)nv;
+        teoff += 4 * (int)(ns[m] + 1);
+        double** bufs3[] = {&S.colloc, &S.colloc_h, &S.normals, &S.vel, &S.forces};
+        for (double** b : bufs3)
+            if ((rc = P->alloc((void**)b, sizeof(double) * np * 3)) != TVLM_OK) return fail(rc);
+        double** bufs1[] = {&S.areas, &S.gprev, &S.gdelta, &S.gdt};
+        for (double** b : bufs1)
+            if ((rc = P->alloc((void**)b, sizeof(double) * np)) != TVLM_OK) return fail(rc);
+        if ((rc = P->alloc((void**)&S.verts_pos, sizeof(double) * nv * 4)) != TVLM_OK) return fail(rc);
+        if ((rc = P->alloc((void**)&S.verts, sizeof(double) * nv * 4)) != TVLM_OK) return fail(rc);
+        if ((rc = P->alloc((void**)&P->init[m], sizeof(double) * nv * 4)) != TVLM_OK) return fail(rc);
+        if ((rc = P->alloc((void**)&S.wake, sizeof(double) * (size_t)(ns[m] + 1) * t_steps * 4)) != TVLM_OK)
+            return fail(rc);
+        if ((rc = P->alloc((void**)&S.gamma_wake, sizeof(double) * (size_t)ns[m] * (t_steps - 1))) != TVLM_OK)
+            return fail(rc);
+        if (hipMemcpyAsync(P->init[m], verts_init[m], sizeof(double) * nv * 4, hipMemcpyHostToDevice, P->stream) !=
+            hipSuccess) {
+            set_error("upload of surface %d failed", m);
+            return fail(TVLM_ERR_HIP);
+        }
+        const int sw = ns[m] < kMaxSpanTile ? (int)ns[m] : kMaxSpanTile;
+        const int rh = kTileRings / sw < 1 ? 1 : kTileRings / sw;
+        max_tiles += (size_t)((ns[m] + sw - 1) / sw) * (size_t)((t_steps + rh - 1) / rh);
+    }
+    tab.N = off;
+    tab.NV = voff;
+    tab.NTE = teoff;
+    P->N = off;
+    const size_t N = (size_t)off;
+    P->cbs = 64;
+    P->cb = (int)((N + P->cbs - 1) / P->cbs);
+    P->wpart_elems = max_tiles * N;
+    if ((rc = P->alloc((void**)&P->lhs, sizeof(double) * N * N)) != TVLM_OK) return fail(rc);
+    if (mode == 1 && (rc = P->alloc((void**)&P->ainv, sizeof(double) * N * N)) != TVLM_OK) return fail(rc);
+    if ((rc = P->alloc((void**)&P->rhs, sizeof(double) * N)) != TVLM_OK) return fail(rc);
+    if ((rc = P->alloc((void**)&P->gamma, sizeof(double) * N)) != TVLM_OK) return fail(rc);
+    if ((rc = P->alloc((void**)&P->gpart, sizeof(double) * N * P->cb)) != TVLM_OK) return fail(rc);
+    if ((rc = P->alloc((void**)&P->wpart, sizeof(double) * (P->wpart_elems ? P->wpart_elems : 1))) != TVLM_OK)
+        return fail(rc);
+    if ((rc = P->alloc((void**)&P->cl, sizeof(double) * t_steps)) != TVLM_OK) return fail(rc);
+    if ((rc = P->alloc((void**)&P->cm, sizeof(double) * 3 * t_steps)) != TVLM_OK) return fail(rc);
+    if ((rc = P->alloc((void**)&P->T_all, sizeof(double) * 16 * n_surf * t_steps)) != TVLM_OK) return fail(rc);
+    if ((rc = P->alloc((void**)&P->dT_all, sizeof(double) * 16 * n_surf * t_steps)) != TVLM_OK) return fail(rc);
+    if ((rc = P->alloc((void**)&P->fs_all, sizeof(double) * 3 * t_steps)) != TVLM_OK) return fail(rc);
+    if ((rc = P->alloc((void**)&P->ref_all, sizeof(double) * 3 * t_steps)) != TVLM_OK) return fail(rc);
+    if ((rc = P->alloc((void**)&P->ipiv, sizeof(int32_t) * N)) != TVLM_OK) return fail(rc);
+    if ((rc = P->alloc((void**)&P->info, sizeof(int32_t))) != TVLM_OK) return fail(rc);
+    if (hipStreamSynchronize(P->stream) != hipSuccess) {
+        set_error("stream synchronize failed");
+        return fail(TVLM_ERR_HIP);
+    }
+    *out = P;
+    return TVLM_OK;
+}
+
+// T, dT: [t_steps][n_surf][16] column-major 4x4 value and time derivative of each surface transform
+// at t_i = i dt; freestream, ref_pt: [t_steps][3] (the CM reference point moves with surface 0).
+int tvlm_uvlm_set_motion(tvlm_uvlm_plan* P, const double* T, const double* dT, const double* freestream,
+                         const double* ref_pt) {
+    TVLM_REQUIRE(P && T && dT && freestream && ref_pt, "null argument");
+    const size_t n16 = sizeof(double) * 16 * P->n_surf * P->t_steps, n3 = sizeof(double) * 3 * P->t_steps;
+    TVLM_HIP(hipMemcpyAsync(P->T_all, T, n16, hipMemcpyHostToDevice, P->stream));
+    TVLM_HIP(hipMemcpyAsync(P->dT_all, dT, n16, hipMemcpyHostToDevice, P->stream));
+    TVLM_HIP(hipMemcpyAsync(P->fs_all, freestream, n3, hipMemcpyHostToDevice, P->stream));
+    TVLM_HIP(hipMemcpyAsync(P->ref_all, ref_pt, n3, hipMemcpyHostToDevice, P->stream));
+    TVLM_HIP(hipStreamSynchronize(P->stream));
+    P->motion_set = true;
+    return TVLM_OK;
+}
+
+// One full simulation: setup (pose, LHS, LU[, inverse]) then steps 0 .. t_steps-2.  use_graph
+// replays the time loop as one captured hipGraph (captured on the first call).  Asynchronous.
+int tvlm_uvlm_run(tvlm_uvlm_plan* P, int32_t use_graph) {
+    TVLM_REQUIRE(P && P->motion_set, "plan has no motion");
+    TVLM_TRY(setup(P));
+    const int last = (int)P->t_steps - 1;
+    if (!use_graph) {
+        for (int i = 0; i < last; i++) TVLM_TRY(enqueue_step(P, i));
+        return TVLM_OK;
+    }
+    TVLM_REQUIRE(P->stream != nullptr, "hipGraph capture needs a non-default stream");
+    if (!P->exec) {
+        TVLM_HIP(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+        int rc = TVLM_OK;
+        for (int i = 0; i < last && rc == TVLM_OK; i++) rc = enqueue_step(P, i);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(P->stream, &g);
+        if (rc != TVLM_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        TVLM_HIP(e);
+        P->graph = g;
+        TVLM_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    }
+    TVLM_HIP(hipGraphLaunch(P->exec, P->stream));
+    return TVLM_OK;
+}
+
+// One simulation without graph capture, with HIP events recorded on the plan's stream around every
+// wake-induction launch; returns the summed kernel time, the number of launches and the number of
+// (receiver, wake ring) pairs they evaluated.  Synchronous.
+int tvlm_uvlm_time_wake(tvlm_uvlm_plan* P, double* total_ms, int32_t* launches, double* pairs) {
+    TVLM_REQUIRE(P && P->motion_set && total_ms && launches && pairs, "bad argument");
+    const int last = (int)P->t_steps - 1;
+    if (P->wake_events.empty()) {
+        P->wake_events.resize(2 * (size_t)last, nullptr);
+        for (auto& e : P->wake_events) TVLM_HIP(hipEventCreate(&e));
+    }
+    TVLM_TRY(setup(P));
+    P->instrument = true;
+    int rc = TVLM_OK;
+    for (int i = 0; i < last && rc == TVLM_OK; i++) rc = enqueue_step(P, i);
+    P->instrument = false;
+    TVLM_TRY(rc);
+    TVLM_HIP(hipStreamSynchronize(P->stream));
+    int64_t rings_per_row = 0;
+    for (int m = 0; m < P->n_surf; m++) rings_per_row += P->tab.s[m].ns;
+    double ms = 0.0, pr = 0.0;
+    for (int i = 1; i < last; i++) {
+        float e = 0.f;
+        TVLM_HIP(hipEventElapsedTime(&e, P->wake_events[2 * i], P->wake_events[2 * i + 1]));
+        ms += e;
+        pr += (double)P->N * (double)rings_per_row * (double)i;
+    }
+    *total_ms = ms;
+    *launches = last - 1;
+    *pairs = pr;
+    return TVLM_OK;
+}
+
+// Copies (synchronously) CL[t_steps-1], CM[t_steps-1][3] (step 0 entries are 0), the last
+// circulation gamma[N] and the LU info; any pointer may be null.
+int tvlm_uvlm_results(tvlm_uvlm_plan* P, double* cl, double* cm, double* gamma, int32_t* info) {
+    TVLM_REQUIRE(P, "null plan");
+    TVLM_HIP(hipStreamSynchronize(P->stream));
+    const size_t n = (size_t)(P->t_steps - 1);
+    if (cl) TVLM_HIP(hipMemcpy(cl, P->cl, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (cm) TVLM_HIP(hipMemcpy(cm, P->cm, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    if (gamma) TVLM_HIP(hipMemcpy(gamma, P->gamma, sizeof(double) * P->N, hipMemcpyDeviceToHost));
+    if (info) TVLM_HIP(hipMemcpy(info, P->info, sizeof(int32_t), hipMemcpyDeviceToHost));
+    return TVLM_OK;
+}
+
+// Prescribed freestream + heave + pitch chain shared by all surfaces, evaluated with dual numbers
+// on the host for every step (vlm/include/vlm_kinematics.hpp:149-239, headeronly/tinyad.hpp):
+//   fs    = translation(-u t, 0, 0)
+//   heave = translation(0, 0, h_amp sin(omega t + h_phase))        after fs
+//   pitch = rotation((x_pivot,0,0), (0,1,0), a_amp sin(omega t + a_phase))  after heave
+// T_all[0] = T(0), T_all[k] = T((k-1) dt + dt) (the t + dt of vlm.cpp:500); dT_all[i] = dT(i dt);
+// freestream = (u, 0, 0); CM reference = T(t) (0.5, 0, 0) (vlm.cpp:464-466).
+int tvlm_motion_pitch_heave(double u, double h_amp, double h_phase, double a_amp, double a_phase, double omega,
+                            double x_pivot, int32_t n_surf, int64_t t_steps, double dt, double* T_out, double* dT_out,
+                            double* fs_out, double* ref_out) {
+    TVLM_REQUIRE(T_out && dT_out && fs_out && ref_out && n_surf >= 1 && t_steps >= 1, "bad argument");
+    struct D {
+        double v, d;
+    };
+    auto mul = [](D a, D b) { return D{a.v * b.v, b.v * a.d + a.v * b.d}; };
+    auto add = [](D a, D b) { return D{a.v + b.v, a.d + b.d}; };
+    struct M4 {
+        D m[16];  // column-major
+    };
+    auto ident = []() {
+        M4 r;
+        for (int k = 0; k < 16; k++) r.m[k] = D{(k % 5) == 0 ? 1.0 : 0.0, 0.0};
+        return r;
+    };
+    auto mm = [&](const M4& a, const M4& b) {
+        M4 r;
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 4; i++) {
+                D acc = mul(a.m[0 * 4 + i], b.m[j * 4 + 0]);
+                for (int k = 1; k < 4; k++) acc = add(acc, mul(a.m[k * 4 + i], b.m[j * 4 + k]));
+                r.m[j * 4 + i] = acc;
+            }
+        return r;
+    };
+    auto eval = [&](double t, double* T, double* dT) {
+        M4 fs = ident(), hv = ident(), pt = ident();
+        fs.m[12] = D{-u * t, -u};
+        hv.m[14] = D{h_amp * sin(omega * t + h_phase), h_amp * omega * cos(omega * t + h_phase)};
+        const D a = {a_amp * sin(omega * t + a_phase), a_amp * omega * cos(omega * t + a_phase)};
+        const D sn = {sin(a.v), cos(a.v) * a.d}, cs = {cos(a.v), -sin(a.v) * a.d};
+        pt.m[0] = cs;
+        pt.m[2] = D{-sn.v, -sn.d};
+        pt.m[8] = sn;
+        pt.m[10] = cs;
+        pt.m[12] = D{(1.0 - cs.v) * x_pivot, -cs.d * x_pivot};
+        pt.m[14] = D{sn.v * x_pivot, sn.d * x_pivot};
+        const M4 r = mm(fs, mm(hv, pt));
+        for (int k = 0; k < 16; k++) {
+            T[k] = r.m[k].v;
+            dT[k] = r.m[k].d;
+        }
+    };
+    const int64_t ts = t_steps;
+    const int n = n_surf;
+    double *T = T_out, *dT = dT_out, *fs = fs_out, *ref = ref_out;
+    double t16[16], d16[16];
+    for (int64_t k = 0; k < ts; k++) {
+        const double ti = (double)k * dt;
+        // value for displacement: T(0), then T(t_{k-1} + dt)
+        eval(k == 0 ? 0.0 : (double)(k - 1) * dt + dt, t16, d16);
+        for (int m = 0; m < n; m++) std::memcpy(&T[(k * n + m) * 16], t16, sizeof t16);
+        eval(ti, t16, d16);
+        for (int m = 0; m < n; m++) std::memcpy(&dT[(k * n + m) * 16], d16, sizeof d16);
+        fs[3 * k] = u;
+        fs[3 * k + 1] = 0.0;
+        fs[3 * k + 2] = 0.0;
+        for (int r = 0; r < 3; r++) ref[3 * k + r] = t16[r] * 0.5 + t16[r + 12];
+    }
+    return TVLM_OK;
+}
+
+int tvlm_uvlm_set_pitch_heave(tvlm_uvlm_plan* P, double u, double h_amp, double h_phase, double a_amp,
+                              double a_phase, double omega, double x_pivot) {
+    TVLM_REQUIRE(P, "null plan");
+    const int64_t ts = P->t_steps;
+    std::vector<double> T(16 * P->n_surf * ts), dT(16 * P->n_surf * ts), fs(3 * ts), ref(3 * ts);
+    TVLM_TRY(tvlm_motion_pitch_heave(u, h_amp, h_phase, a_amp, a_phase, omega, x_pivot, P->n_surf, ts, P->dt, T.data(),
+                                     dT.data(), fs.data(), ref.data()));
+    return tvlm_uvlm_set_motion(P, T.data(), dT.data(), fs.data(), ref.data());
+}
+
+int tvlm_uvlm_destroy(tvlm_uvlm_plan* P) {
+    if (!P) return TVLM_OK;
+    (void)hipStreamSynchronize(P->stream);
+    for (auto e : P->wake_events)
+        if (e) (void)hipEventDestroy(e);
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    for (void* p : P->allocs) (void)hipFree(p);
+    delete P;
+    return TVLM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+"""Device-resident UVLM time loop (tvlm_uvlm_* in include/tinyvlm_hip.h) from Python.
+
+The whole of UVLM::run (vlm/src/vlm.cpp:358-509) -- setup, every time step, CL/CM history --
+runs inside libtinyvlm_hip.so on one stream; Python only passes the mesh and the motion.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import capi as H
+
+MODE_PLAIN_FP64 = 0
+MODE_FAST = 1
+
+
+def time_step(verts_init, u):
+    """dt = last panel chord of surface 0 at span station 0 / |u| (vlm.cpp:380-386)."""
+    d = np.asarray(verts_init)[0, -1, :3] - np.asarray(verts_init)[0, -2, :3]
+    return math.sqrt(float(np.dot(d, d))) / abs(u)
+
+
+def motion_pitch_heave(n_surf, t_steps, dt, u, h_amp=0.0, h_phase=0.0, a_amp=0.0, a_phase=0.0, omega=0.0,
+                       x_pivot=0.25):
+    """Host-side evaluation through tvlm_motion_pitch_heave; returns (T, dT, fs, ref) arrays."""
+    L = H.lib()
+    T = np.zeros((t_steps, n_surf, 16))
+    dT = np.zeros((t_steps, n_surf, 16))
+    fs = np.zeros((t_steps, 3))
+    ref = np.zeros((t_steps, 3))
+    d = ctypes.c_double
+    P = ctypes.c_void_p
+    H.check(L.tvlm_motion_pitch_heave(d(u), d(h_amp), d(h_phase), d(a_amp), d(a_phase), d(omega), d(x_pivot),
+                                      ctypes.c_int32(n_surf), ctypes.c_int64(t_steps), d(dt), P(T.ctypes.data),
+                                      P(dT.ctypes.data), P(fs.ctypes.data), P(ref.ctypes.data)),
+            "tvlm_motion_pitch_heave")
+    return T, dT, fs, ref
+
+
+class UVLMPlan:
+    """One reusable device-resident simulation (buffers, LU, wake, optional captured hipGraph)."""
+
+    def __init__(self, verts_init_list, t_steps, dt, mode=MODE_FAST, stream=None):
+        import torch
+
+        self.n_surf = len(verts_init_list)
+        self.t_steps = int(t_steps)
+        self.dt = float(dt)
+        self.mode = mode
+        self.stream = stream if stream is not None else torch.cuda.Stream()
+        self.N = sum((v.shape[0] - 1) * (v.shape[1] - 1) for v in verts_init_list)
+        self._verts = [np.asfortranarray(np.asarray(v, dtype=np.float64)) for v in verts_init_list]
+        ns = (ctypes.c_int64 * self.n_surf)(*[v.shape[0] - 1 for v in self._verts])
+        nc = (ctypes.c_int64 * self.n_surf)(*[v.shape[1] - 1 for v in self._verts])
+        ptrs = (ctypes.c_void_p * self.n_surf)(*[v.ctypes.data for v in self._verts])
+        self._plan = ctypes.c_void_p()
+        H.check(H.lib().tvlm_uvlm_create(ctypes.byref(self._plan), ctypes.c_int32(self.n_surf), ns, nc, ptrs,
+                                         ctypes.c_int64(self.t_steps), ctypes.c_double(self.dt),
+                                         ctypes.c_int32(mode), H.stream_ptr(self.stream)), "tvlm_uvlm_create")
+
+    def set_pitch_heave(self, u, h_amp=0.0, h_phase=0.0, a_amp=0.0, a_phase=0.0, omega=0.0, x_pivot=0.25):
+        d = ctypes.c_double
+        H.check(H.lib().tvlm_uvlm_set_pitch_heave(self._plan, d(u), d(h_amp), d(h_phase), d(a_amp), d(a_phase),
+                                                  d(omega), d(x_pivot)), "tvlm_uvlm_set_pitch_heave")
+
+    def set_motion(self, T, dT, fs, ref):
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (T, dT, fs, ref)]
+        H.check(H.lib().tvlm_uvlm_set_motion(self._plan, *[ctypes.c_void_p(a.ctypes.data) for a in arrs]),
+                "tvlm_uvlm_set_motion")
+
+    def run(self, use_graph=True):
+        H.check(H.lib().tvlm_uvlm_run(self._plan, ctypes.c_int32(int(use_graph))), "tvlm_uvlm_run")
+
+    def results(self):
+        n = self.t_steps - 1
+        cl = np.zeros(n)
+        cm = np.zeros((n, 3))
+        g = np.zeros(self.N)
+        info = ctypes.c_int32(0)
+        H.check(H.lib().tvlm_uvlm_results(self._plan, ctypes.c_void_p(cl.ctypes.data), ctypes.c_void_p(cm.ctypes.data),
+                                          ctypes.c_void_p(g.ctypes.data), ctypes.byref(info)), "tvlm_uvlm_results")
+        if info.value != 0:
+            raise H.TvlmError(f"singular influence matrix (getrf info {info.value})")
+        return {"cl": cl, "cm": cm, "gamma": g}
+
+    def time_wake(self):
+        """Instrumented run: (summed wake-kernel ms, launches, receiver x wake-ring pairs)."""
+        ms = ctypes.c_double()
+        n = ctypes.c_int32()
+        pairs = ctypes.c_double()
+        H.check(H.lib().tvlm_uvlm_time_wake(self._plan, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(pairs)),
+                "tvlm_uvlm_time_wake")
+        return ms.value, n.value, pairs.value
+
+    def close(self):
+        if self._plan:
+            H.lib().tvlm_uvlm_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
