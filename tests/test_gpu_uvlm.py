@@ -64,6 +64,9 @@ def test_fast_mode_vs_plain_full_run(wing):
     print(f"[case] uvlm fast vs plain 500 steps: max |dCL| {dcl:.3e} max |dCM| {dcm:.3e}", flush=True)
     assert dcl <= 1e-6
     assert dcm <= 1e-6
+    # the seeded form (hardware estimate + one Newton step) is FP64-accurate, not merely within the
+    # bar: a coarse estimate or a dropped Newton step would show up here
+    assert dcl <= 1e-11 and dcm <= 1e-11
     # a second replay of the captured graph reproduces the first bitwise
     fast.run(use_graph=True)
     rf2 = fast.results()

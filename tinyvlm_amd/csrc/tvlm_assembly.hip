@@ -136,9 +136,7 @@ int wake_influence(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_
         s.row_begin = nwv - 1 - iteration;
         s.row_end = nwv - 1;
         if (sizeof(T) == 4 && iteration > 5000) s.row_end = nwv - 1 - iteration + 5000;  // kernels.ispc:157-162
-        s.sw = (int32_t)(ns < kMaxSpanTile ? ns : kMaxSpanTile);
-        s.rh = kTileRings / s.sw;
-        if (s.rh < 1) s.rh = 1;
+        wake_tile_shape(s.ns, s.row_end - s.row_begin, (total + kRecv - 1) / kRecv, s.sw, s.rh);
         s.tiles_span = (s.ns + s.sw - 1) / s.sw;
         const int tiles_row = (s.row_end - s.row_begin + s.rh - 1) / s.rh;
         s.tile_begin = tiles;
@@ -163,8 +161,7 @@ int wake_influence(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_
             hipLaunchKernelGGL((k_wake_partials<T, T, 0>), grid, dim3(kThreads), 0, st, (T*)ws, rt, wt, sigma);
     }
     TVLM_LAUNCHED();
-    hipLaunchKernelGGL(k_wake_reduce<T>, dim3((total + 255) / 256), dim3(256), 0, st, static_cast<T*>(rhs->ptr),
-                       rhs->stride[0], (const T*)ws, tiles, total);
+    launch_wake_reduce<T>(static_cast<T*>(rhs->ptr), rhs->stride[0], (const T*)ws, tiles, total, st);
     TVLM_LAUNCHED();
     return TVLM_OK;
 }

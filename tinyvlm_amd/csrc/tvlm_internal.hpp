@@ -214,17 +214,17 @@ __device__ __forceinline__ void fast_segment(float& ax, float& ay, float& az, co
 }
 
 // ---- FP64 seeded form ---------------------------------------------------------------------------
-// FP64 arithmetic throughout, but 1/sqrt and 1/x from the single-precision hardware estimates
-// (v_rsq_f32, v_rcp_f32, relative error ~2^-23) refined by one Newton step in FP64 (relative error
-// ~2^-45) instead of the multi-instruction correctly rounded FP64 sqrt and division.  MI355X runs
+// FP64 arithmetic throughout, but 1/sqrt and 1/x from the hardware estimates (v_rsq_f64,
+// v_rcp_f64) refined by one Newton step in FP64 instead of the multi-instruction correctly rounded
+// FP64 sqrt and division (the Newton step squares the estimate's relative error).  MI355X runs
 // FP64 add/mul/fma at the rate of non-packed FP32, so this costs about what the FP32 form costs
 // while keeping FP64 accuracy.  Cutoffs as the FP64 reference: |r| < RCUT <=> |r|^2 < RCUT^2.
 __device__ __forceinline__ double rsqrt_seeded(double x) {
-    const double y = (double)__builtin_amdgcn_rsqf((float)x);
+    const double y = __builtin_amdgcn_rsq(x);  // v_rsq_f64 estimate
     return y * (1.5 - 0.5 * x * y * y);
 }
 __device__ __forceinline__ double rcp_seeded(double x) {
-    const double y = (double)__builtin_amdgcn_rcpf((float)x);
+    const double y = __builtin_amdgcn_rcp(x);  // v_rcp_f64 estimate
     return y * (2.0 - x * y);
 }
 
@@ -254,6 +254,25 @@ __device__ __forceinline__ void seed_segment(double& ax, double& ay, double& az,
     constexpr double cut2 = BS<double>::cut_norm * BS<double>::cut_norm;
     const bool ok = (sq >= BS<double>::cut_cross) & (r1.r2 >= cut2) & (r2.r2 >= cut2);
     const double t = (d1 * r1.rinv - d2 * r2.rinv) * rcp_seeded(BS<double>::fourpi * sq);
+    const double coeff = ok ? t : 0.0;
+    ax += coeff * crx;
+    ay += sy * (coeff * cry);
+    az += coeff * crz;
+}
+
+// Segment a->b of strength g (r1 = c - a, r2 = c - b), accumulated with image sign sy on y.
+__device__ __forceinline__ void seed_segment_w(double& ax, double& ay, double& az, const SeedVertex& r1,
+                                               const SeedVertex& r2, double sy, double g) {
+    const double crx = r1.y * r2.z - r1.z * r2.y;
+    const double cry = r1.z * r2.x - r1.x * r2.z;
+    const double crz = r1.x * r2.y - r1.y * r2.x;
+    const double sq = crx * crx + cry * cry + crz * crz;
+    const double r0x = r1.x - r2.x, r0y = r1.y - r2.y, r0z = r1.z - r2.z;
+    const double d1 = r0x * r1.x + r0y * r1.y + r0z * r1.z;
+    const double d2 = r0x * r2.x + r0y * r2.y + r0z * r2.z;
+    constexpr double cut2 = BS<double>::cut_norm * BS<double>::cut_norm;
+    const bool ok = (sq >= BS<double>::cut_cross) & (r1.r2 >= cut2) & (r2.r2 >= cut2);
+    const double t = (d1 * r1.rinv - d2 * r2.rinv) * rcp_seeded(BS<double>::fourpi * sq) * g;
     const double coeff = ok ? t : 0.0;
     ax += coeff * crx;
     ay += sy * (coeff * cry);

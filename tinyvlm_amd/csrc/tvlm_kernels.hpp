@@ -221,14 +221,198 @@ __global__ __launch_bounds__(kThreads) void k_wake_partials(TS* __restrict__ par
     }
 }
 
+// rhs[p] -= sum over tiles of partial[tile][p]: 64 receivers per work-group, 16 waves each sum
+// every 16th tile, then a fixed-order tree over the waves (bitwise reproducible).
+constexpr int kRedWaves = 16;
+
 template <typename T>
-__global__ void k_wake_reduce(T* __restrict__ rhs, int64_t rhs_stride, const T* __restrict__ partial, int tiles,
-                              int total) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= total) return;
+__global__ __launch_bounds__(64 * kRedWaves) void k_wake_reduce(T* __restrict__ rhs, int64_t rhs_stride,
+                                                                const T* __restrict__ partial, int tiles, int total) {
+    __shared__ T red[kRedWaves][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
     T s = T(0);
-    for (int t = 0; t < tiles; t++) s += partial[(int64_t)t * total + p];
-    rhs[(int64_t)p * rhs_stride] -= s;
+    if (p < total)
+        for (int t = wave; t < tiles; t += kRedWaves) s += partial[(int64_t)t * total + p];
+    red[wave][lane] = s;
+    __syncthreads();
+#pragma unroll
+    for (int h = kRedWaves / 2; h > 0; h >>= 1) {
+        if (wave < h) red[wave][lane] += red[wave + h][lane];
+        __syncthreads();
+    }
+    if (wave == 0 && p < total) rhs[(int64_t)p * rhs_stride] -= red[0][lane];
+}
+
+template <typename T>
+inline void launch_wake_reduce(T* rhs, int64_t rhs_stride, const T* partial, int tiles, int total, hipStream_t st) {
+    hipLaunchKernelGGL(k_wake_reduce<T>, dim3((total + 63) / 64), dim3(64 * kRedWaves), 0, st, rhs, rhs_stride, partial,
+                       tiles, total);
+}
+
+// Tile shape of one wake surface: spans per tile sw <= kMaxSpanTile, rows per tile rh with
+// rh * sw <= kTileRings, and small enough that the launch has about kTargetGroups work-groups
+// (receiver blocks x tiles) so every SIMD holds several waves.
+constexpr int kTargetGroups = 2048;
+
+inline void wake_tile_shape(int ns, int rows, int recv_blocks, int& sw, int& rh) {
+    sw = ns < kMaxSpanTile ? ns : kMaxSpanTile;
+    const int tiles_span = (ns + sw - 1) / sw;
+    int rh_max = kTileRings / sw;
+    if (rh_max < 1) rh_max = 1;
+    int want_row_tiles = kTargetGroups / (recv_blocks * tiles_span);
+    if (want_row_tiles < 1) want_row_tiles = 1;
+    rh = (rows + want_row_tiles - 1) / want_row_tiles;
+    if (rh < 1) rh = 1;
+    if (rh > rh_max) rh = rh_max;
+}
+
+// Wake induction of the seeded FP64 form evaluated on the vortex lattice.  Every segment of the
+// tile is evaluated once with strength = difference of the circulations of the two rings that share
+// it (spanwise segment of vertex row q, span s: G(q,s) - G(q-1,s); chordwise segment of vertex
+// column s: G(q,s-1) - G(q,s); rings outside the work-group tile count 0 -- their own tile adds
+// them).  The work-group tile is RH rows x up to 4*CH spans; wave w walks the ring columns of its
+// span chunk left to right keeping the vertex terms of the previous vertex column in registers, so
+// a ring costs (RH+1)/RH vertex terms and (2 RH + 1)/RH segments (both images share x and z).
+struct LatVertex {
+    double x, y, z, ri;  // c - v, 1/|c - v|
+    double ym, rim;      // image receiver (y -> -y): y component and 1/|.|
+};
+
+__device__ __forceinline__ LatVertex lat_vertex(double cx, double cy, double cz, const double* v) {
+    LatVertex f;
+    f.x = cx - v[0];
+    f.y = cy - v[1];
+    f.z = cz - v[2];
+    f.ym = -cy - v[1];
+    const double xz = f.x * f.x + f.z * f.z;
+    f.ri = rsqrt_seeded(xz + f.y * f.y);
+    f.rim = rsqrt_seeded(xz + f.ym * f.ym);
+    return f;
+}
+
+// segment a->b of strength g for the receiver (image = false) or its image (image = true)
+template <bool IMAGE>
+__device__ __forceinline__ void lat_segment(double& ax, double& ay, double& az, const LatVertex& a,
+                                            const LatVertex& b, double g) {
+    const double r1x = a.x, r1y = IMAGE ? a.ym : a.y, r1z = a.z, ri1 = IMAGE ? a.rim : a.ri;
+    const double r2x = b.x, r2y = IMAGE ? b.ym : b.y, r2z = b.z, ri2 = IMAGE ? b.rim : b.ri;
+    const double crx = r1y * r2z - r1z * r2y;
+    const double cry = r1z * r2x - r1x * r2z;
+    const double crz = r1x * r2y - r1y * r2x;
+    const double sq = crx * crx + cry * cry + crz * crz;
+    const double r0x = r1x - r2x, r0y = r1y - r2y, r0z = r1z - r2z;
+    const double d1 = r0x * r1x + r0y * r1y + r0z * r1z;
+    const double d2 = r0x * r2x + r0y * r2y + r0z * r2z;
+    // |r| < RCUT  <=>  1/|r| > 1/RCUT
+    constexpr double rimax = 1.0 / BS<double>::cut_norm;
+    const bool ok = (sq >= BS<double>::cut_cross) & (ri1 <= rimax) & (ri2 <= rimax);
+    const double t = (d1 * ri1 - d2 * ri2) * rcp_seeded(BS<double>::fourpi * sq) * g;
+    const double coeff = ok ? t : 0.0;
+    ax += coeff * crx;
+    ay += (IMAGE ? -1.0 : 1.0) * (coeff * cry);
+    az += coeff * crz;
+}
+
+__device__ __forceinline__ void lat_pair(double& ax, double& ay, double& az, const LatVertex& a, const LatVertex& b,
+                                         double g) {
+    lat_segment<false>(ax, ay, az, a, b, g);
+    lat_segment<true>(ax, ay, az, a, b, g);
+}
+
+constexpr int kLatChunkMax = 16;  // spans per wave
+
+template <typename TS, int RH>
+__global__ __launch_bounds__(kThreads) void k_wake_lattice(TS* __restrict__ partial, RecvTable<TS> rt,
+                                                           WakeTable<TS> wt) {
+    constexpr int SWG = kWaves * kLatChunkMax;
+    __shared__ double sverts[(RH + 1) * (SWG + 1) * 3];
+    __shared__ double sgam[RH * SWG];
+    __shared__ double sred[kWaves][kRecv];
+    const int tile = blockIdx.y;
+    int w = 0;
+    while (w + 1 < wt.n && tile >= wt.s[w + 1].tile_begin) w++;
+    const WakeSurface<TS>& ws = wt.s[w];
+    const int local = tile - ws.tile_begin;
+    const int ts = local % ws.tiles_span, tr = local / ws.tiles_span;
+    const int s0 = ts * ws.sw, r0 = ws.row_begin + tr * ws.rh;
+    const int swe = min(ws.sw, ws.ns - s0);
+    const int rhe = min(ws.rh, ws.row_end - r0);
+    const int vw = swe + 1;
+    for (int e = threadIdx.x; e < (rhe + 1) * vw; e += kThreads) {
+        const int q = e / vw, sp = e % vw;
+        const int64_t g = (int64_t)(s0 + sp) + (int64_t)(r0 + q) * (ws.ns + 1);
+        sverts[3 * e + 0] = (double)ws.verts[g];
+        sverts[3 * e + 1] = (double)ws.verts[g + ws.ws2];
+        sverts[3 * e + 2] = (double)ws.verts[g + 2 * ws.ws2];
+    }
+    for (int e = threadIdx.x; e < rhe * swe; e += kThreads) {
+        const int q = e / swe, sp = e % swe;
+        sgam[e] = (double)ws.gamma[(int64_t)(r0 + q) * ws.ns + s0 + sp];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kRecv - 1);
+    const int wave = threadIdx.x / kRecv;
+    const int p = blockIdx.x * kRecv + lane;
+    double part = 0.0;
+    const int chunk = (swe + kWaves - 1) / kWaves;
+    const int sa = wave * chunk, sb = min(swe, sa + chunk);
+    if (p < rt.total && sa < sb) {
+        const int si = find_surface(rt, p);
+        const Receivers<TS>& rc = rt.r[si];
+        const int li = p - rt.offset[si];
+        const double cx = rc.colloc[li], cy = rc.colloc[li + rc.cs2], cz = rc.colloc[li + 2 * rc.cs2];
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        LatVertex prev[RH + 1], cur[RH + 1];
+#pragma unroll
+        for (int q = 0; q <= RH; q++)
+            if (q <= rhe) prev[q] = lat_vertex(cx, cy, cz, sverts + 3 * (q * vw + sa));
+        for (int s = sa; s < sb; s++) {
+            // chordwise segments of vertex column s (left edge of ring column s)
+#pragma unroll
+            for (int q = 0; q < RH; q++) {
+                if (q < rhe) {
+                    double g = -sgam[q * swe + s];
+                    if (s > 0) g += sgam[q * swe + s - 1];
+                    lat_pair(ax, ay, az, prev[q], prev[q + 1], g);
+                }
+            }
+            // vertex column s+1 and the spanwise segments of ring column s
+#pragma unroll
+            for (int q = 0; q <= RH; q++) {
+                if (q <= rhe) {
+                    cur[q] = lat_vertex(cx, cy, cz, sverts + 3 * (q * vw + s + 1));
+                    double g = 0.0;
+                    if (q < rhe) g += sgam[q * swe + s];
+                    if (q > 0) g -= sgam[(q - 1) * swe + s];
+                    lat_pair(ax, ay, az, prev[q], cur[q], g);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q <= RH; q++) prev[q] = cur[q];
+        }
+        if (sb == swe) {  // right edge of the work-group tile: chordwise segments of vertex column swe
+#pragma unroll
+            for (int q = 0; q < RH; q++)
+                if (q < rhe) lat_pair(ax, ay, az, prev[q], prev[q + 1], sgam[q * swe + swe - 1]);
+        }
+        part = ax * (double)rc.normals[li] + ay * (double)rc.normals[li + rc.ns2] + az * (double)rc.normals[li + 2 * rc.ns2];
+    }
+    sred[wave][lane] = part;
+    __syncthreads();
+    if (wave == 0 && p < rt.total)
+        partial[(int64_t)tile * rt.total + p] = (TS)((sred[0][lane] + sred[1][lane]) + (sred[2][lane] + sred[3][lane]));
+}
+
+// Tile shape for k_wake_lattice: sw <= 4 * kLatChunkMax spans, rh in {1, 2, 4} rows, chosen so
+// that the launch has about kTargetGroups work-groups.
+inline void lattice_tile_shape(int ns, int rows, int recv_blocks, int& sw, int& rh) {
+    sw = ns < kWaves * kLatChunkMax ? ns : kWaves * kLatChunkMax;
+    const int tiles_span = (ns + sw - 1) / sw;
+    int want_row_tiles = kTargetGroups / (recv_blocks * tiles_span);
+    if (want_row_tiles < 1) want_row_tiles = 1;
+    const int need = (rows + want_row_tiles - 1) / want_row_tiles;
+    rh = need <= 1 ? 1 : need <= 2 ? 2 : 4;
 }
 
 }  // namespace

@@ -9,9 +9,10 @@
 //   k_wake_partials + k_wake_reduce : rhs -= wake induction                   (vlm.cpp:439)
 //   solve    : plain mode: LU forward/backward substitution (getrs);
 //              fast mode : gamma = A^{-1} rhs with the FP64 inverse formed once from the LU
-//   k_post   : gamma post-processing, shed of Gamma_TE into the wake, unsteady forces, CL and CM
-//              of all surfaces (one work-group, fixed-order reductions)       (vlm.cpp:441-487)
-//   k_move   : displacement by T(t+dt) and panel metrics                      (vlm.cpp:489-507)
+//   k_post_advance (one work-group): gamma post-processing, shed of Gamma_TE into the wake,
+//              unsteady forces, CL and CM of all surfaces with fixed-order reductions
+//              (vlm.cpp:441-487); then displacement by T(t+dt), panel metrics (vlm.cpp:489-507)
+//              and the next step's wake shed, velocities and rhs
 // The fast mode evaluates the wake induction in the FP64 seeded form (FP64 arithmetic, rsqrt and
 // reciprocal from the FP32 hardware estimates plus one Newton step) and replays the whole loop as
 // one hipGraph.
@@ -98,27 +99,39 @@ __global__ __launch_bounds__(256) void k_gemv_part(const double* __restrict__ A,
 
 constexpr int kPostThreads = 1024;
 
-template <int NT>
-__device__ __forceinline__ double bsum(double v, double* lds) {
-    const int t = threadIdx.x;
-    lds[t] = v;
-    __syncthreads();
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-    for (int s = NT / 2; s > 0; s >>= 1) {
-        if (t < s) lds[t] += lds[t + s];
-        __syncthreads();
-    }
-    const double r = lds[0];
-    __syncthreads();
-    return r;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
-__global__ __launch_bounds__(kPostThreads) void k_post(UTab tab, const double* __restrict__ gsrc, int cb, int step,
-                                                       double inv_dt, const double* __restrict__ fs_all,
-                                                       const double* __restrict__ ref_all, double rho,
-                                                       double* __restrict__ gamma, double* __restrict__ cl_out,
-                                                       double* __restrict__ cm_out) {
-    __shared__ double lds[kPostThreads];
+// Sums K per-thread values over the work-group: butterfly inside each wave, then thread 0 adds
+// the waves in index order (fixed order: bitwise reproducible).  Result valid in thread 0 only.
+template <int K>
+__device__ __forceinline__ void block_sums(double (&v)[K], double (*lds)[K]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double s = lds[0][k];
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) s += lds[w][k];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void post_step(const UTab& tab, const double* __restrict__ gsrc, int cb, int step,
+                                          double inv_dt, const double* __restrict__ fs_all,
+                                          const double* __restrict__ ref_all, double rho, double* __restrict__ gamma,
+                                          double* __restrict__ cl_out, double* __restrict__ cm_out,
+                                          double (*lds)[6]) {
     const int t = threadIdx.x;
     const int N = tab.N;
     for (int P = t; P < N; P += kPostThreads) {
@@ -185,12 +198,9 @@ __global__ __launch_bounds__(kPostThreads) void k_post(UTab tab, const double* _
                 const double c0 = sqrt(dot(d0, d0)), c1 = sqrt(dot(d1, d1)), w = sqrt(dot(d3, d3));
                 mac_acc += 0.5 * (c0 * c0 + c1 * c1) * w;
             }
-            const double area = bsum<kPostThreads>(a_acc, lds);
-            const double cs = bsum<kPostThreads>(c_acc, lds);
-            const double sx = bsum<kPostThreads>(mx, lds);
-            const double sy = bsum<kPostThreads>(my, lds);
-            const double sz = bsum<kPostThreads>(mz, lds);
-            const double macs = bsum<kPostThreads>(mac_acc, lds);
+            double v[6] = {a_acc, c_acc, mx, my, mz, mac_acc};
+            block_sums<6>(v, lds);
+            const double area = v[0], cs = v[1], sx = v[2], sy = v[3], sz = v[4], macs = v[5];
             // coeff_cl / coeff_cm / mesh_mac / *_multibody (vlm_backend_cpu.cpp:323-362, vlm_backend.cpp:71-139)
             const double mac = macs / area;
             const double cl_m = cs / (q * area);
@@ -209,9 +219,13 @@ __global__ __launch_bounds__(kPostThreads) void k_post(UTab tab, const double* _
     }
 }
 
-// displacement by T(step) (verts = T verts_pos) and panel metrics at alpha = 0
-__global__ void k_move(UTab tab, const double* __restrict__ T_all, int step) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+// Displacement by T(step) (verts = T verts_pos) and panel metrics at alpha = 0 (vlm.cpp:489-507),
+// fused with the start of the next time step when do_pre is set (vlm.cpp:412-438): the thread of a
+// trailing-edge vertex also sheds it into wake row t_steps-1-step, the thread of a panel also
+// forms the kinematic velocity at its t = 0 collocation point and rhs = -(V . n).
+__device__ __forceinline__ void advance_item(const UTab& tab, const double* __restrict__ T_all,
+                                             const double* __restrict__ dT_all, int step, int do_pre,
+                                             double* __restrict__ rhs, int e) {
     if (e < tab.NV) {
         const int m = surf_of(tab, e, 1);
         const USurf& S = tab.s[m];
@@ -219,8 +233,20 @@ __global__ void k_move(UTab tab, const double* __restrict__ T_all, int step) {
         const int64_t nv = (int64_t)(S.ns + 1) * (S.nc + 1);
         const double* M = T_all + ((int64_t)step * tab.n + m) * 16;
         const double x[4] = {S.verts_pos[p], S.verts_pos[p + nv], S.verts_pos[p + 2 * nv], S.verts_pos[p + 3 * nv]};
+        double y[4];
 #pragma unroll
-        for (int r = 0; r < 4; r++) S.verts[p + r * nv] = displace_row(M, 1, 4, r, x);
+        for (int r = 0; r < 4; r++) {
+            y[r] = displace_row(M, 1, 4, r, x);
+            S.verts[p + r * nv] = y[r];
+        }
+        const int vs1 = S.ns + 1;
+        if (do_pre && p / vs1 == S.nc) {
+            const int i = p % vs1;
+            const int64_t ws2 = (int64_t)vs1 * tab.t_steps;
+            const int row = tab.t_steps - 1 - step;
+#pragma unroll
+            for (int k = 0; k < 4; k++) S.wake[i + (int64_t)row * vs1 + k * ws2] = y[k];
+        }
         return;
     }
     const int P = e - tab.NV;
@@ -247,6 +273,36 @@ __global__ void k_move(UTab tab, const double* __restrict__ T_all, int step) {
     S.normals[p + np] = n.y;
     S.normals[p + 2 * np] = n.z;
     S.areas[p] = area;
+    if (do_pre) {
+        const double* d = dT_all + ((int64_t)step * tab.n + m) * 16;
+        const double x = S.colloc_h[p], y = S.colloc_h[p + np], z = S.colloc_h[p + 2 * np];
+        double V[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            V[r] = -(d[r] * x + d[r + 4] * y + d[r + 8] * z + d[r + 12]);
+            S.vel[p + r * np] = V[r];
+        }
+        rhs[P] = -(V[0] * n.x + V[1] * n.y + V[2] * n.z);
+    }
+}
+
+// One work-group per step for everything O(N) after the solve: gamma post-processing, wake Gamma,
+// forces, CL/CM (post_step), then, after a barrier, displacement + metrics + the next step's
+// shed / velocities / rhs (advance_item) -- two launches and their gaps fewer per step.
+__global__ __launch_bounds__(kPostThreads) void k_post_advance(UTab tab, const double* __restrict__ gsrc, int cb,
+                                                               int step, double inv_dt,
+                                                               const double* __restrict__ fs_all,
+                                                               const double* __restrict__ ref_all, double rho,
+                                                               double* __restrict__ gamma, double* __restrict__ cl_out,
+                                                               double* __restrict__ cm_out,
+                                                               const double* __restrict__ T_all,
+                                                               const double* __restrict__ dT_all, int do_pre,
+                                                               double* __restrict__ rhs) {
+    __shared__ double lds[kPostThreads / 64][6];
+    post_step(tab, gsrc, cb, step, inv_dt, fs_all, ref_all, rho, gamma, cl_out, cm_out, lds);
+    __syncthreads();
+    for (int e = threadIdx.x; e < tab.NV + tab.N; e += kPostThreads)
+        advance_item(tab, T_all, dT_all, step + 1, do_pre, rhs, e);
 }
 
 __global__ void k_fill64(double* x, int64_t n, double v) {
@@ -333,8 +389,10 @@ void wake_tables(const tvlm_uvlm_plan* P, int it, RecvTable<double>& rt, WakeTab
         w.nwv = (int32_t)P->t_steps;
         w.row_begin = (int32_t)P->t_steps - 1 - it;
         w.row_end = (int32_t)P->t_steps - 1;
-        w.sw = S.ns < kMaxSpanTile ? S.ns : kMaxSpanTile;
-        w.rh = kTileRings / w.sw < 1 ? 1 : kTileRings / w.sw;
+        if (P->mode == 1)
+            lattice_tile_shape(S.ns, w.row_end - w.row_begin, (P->N + kRecv - 1) / kRecv, w.sw, w.rh);
+        else
+            wake_tile_shape(S.ns, w.row_end - w.row_begin, (P->N + kRecv - 1) / kRecv, w.sw, w.rh);
         w.tiles_span = (S.ns + w.sw - 1) / w.sw;
         w.tile_begin = tiles;
         tiles += w.tiles_span * ((w.row_end - w.row_begin + w.rh - 1) / w.rh);
@@ -386,8 +444,10 @@ int setup(tvlm_uvlm_plan* P) {
 int enqueue_step(tvlm_uvlm_plan* P, int i) {
     hipStream_t st = P->stream;
     UTab& tab = P->tab;
-    hipLaunchKernelGGL(k_pre, dim3(nblk(tab.N + tab.NTE)), dim3(256), 0, st, tab, P->dT_all, i, P->rhs);
-    TVLM_LAUNCHED();
+    if (i == 0) {  // later steps start inside the previous step's k_advance
+        hipLaunchKernelGGL(k_pre, dim3(nblk(tab.N + tab.NTE)), dim3(256), 0, st, tab, P->dT_all, i, P->rhs);
+        TVLM_LAUNCHED();
+    }
     if (i > 0) {
         RecvTable<double> rt;
         WakeTable<double> wt;
@@ -396,14 +456,30 @@ int enqueue_step(tvlm_uvlm_plan* P, int i) {
         TVLM_REQUIRE((size_t)tiles * P->N <= P->wpart_elems, "wake partial buffer too small");
         dim3 grid((P->N + kRecv - 1) / kRecv, tiles);
         if (P->instrument) TVLM_HIP(hipEventRecord(P->wake_events[2 * i], st));
-        if (P->mode == 1)
-            hipLaunchKernelGGL((k_wake_partials<double, double, 3>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt, 0.0);
-        else
+        if (P->mode == 1) {
+            // one rows-per-tile value for every surface of the launch: the largest chosen
+            int rh = 1;
+            for (int m = 0; m < wt.n; m++) rh = wt.s[m].rh > rh ? wt.s[m].rh : rh;
+            for (int m = 0; m < wt.n; m++) wt.s[m].rh = rh;
+            int t2 = 0;
+            for (int m = 0; m < wt.n; m++) {
+                wt.s[m].tile_begin = t2;
+                t2 += wt.s[m].tiles_span * ((wt.s[m].row_end - wt.s[m].row_begin + rh - 1) / rh);
+            }
+            wt.total_tiles = t2;
+            tiles = t2;
+            grid.y = tiles;
+            if (rh == 1)
+                hipLaunchKernelGGL((k_wake_lattice<double, 1>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt);
+            else if (rh == 2)
+                hipLaunchKernelGGL((k_wake_lattice<double, 2>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt);
+            else
+                hipLaunchKernelGGL((k_wake_lattice<double, 4>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt);
+        } else
             hipLaunchKernelGGL((k_wake_partials<double, double, 0>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt, 0.0);
         TVLM_LAUNCHED();
         if (P->instrument) TVLM_HIP(hipEventRecord(P->wake_events[2 * i + 1], st));
-        hipLaunchKernelGGL(k_wake_reduce<double>, dim3(nblk(P->N)), dim3(256), 0, st, P->rhs, (int64_t)1, P->wpart,
-                           tiles, P->N);
+        launch_wake_reduce<double>(P->rhs, 1, P->wpart, tiles, P->N, st);
         TVLM_LAUNCHED();
     }
     const double* gsrc;
@@ -421,10 +497,9 @@ int enqueue_step(tvlm_uvlm_plan* P, int i) {
         gsrc = P->rhs;
         cb = 1;
     }
-    hipLaunchKernelGGL(k_post, dim3(1), dim3(kPostThreads), 0, st, tab, gsrc, cb, i, 1.0 / P->dt, P->fs_all,
-                       P->ref_all, 1.0, P->gamma, P->cl, P->cm);
-    TVLM_LAUNCHED();
-    hipLaunchKernelGGL(k_move, dim3(nblk(tab.NV + tab.N)), dim3(256), 0, st, tab, P->T_all, i + 1);
+    const int do_pre = (i + 1 < (int)P->t_steps - 1) ? 1 : 0;
+    hipLaunchKernelGGL(k_post_advance, dim3(1), dim3(kPostThreads), 0, st, tab, gsrc, cb, i, 1.0 / P->dt, P->fs_all,
+                       P->ref_all, 1.0, P->gamma, P->cl, P->cm, P->T_all, P->dT_all, do_pre, P->rhs);
     TVLM_LAUNCHED();
     return TVLM_OK;
 }
@@ -492,8 +567,7 @@ int tvlm_uvlm_create(tvlm_uvlm_plan** out, int32_t n_surf, const int64_t* ns, co
             return fail(TVLM_ERR_HIP);
         }
         const int sw = ns[m] < kMaxSpanTile ? (int)ns[m] : kMaxSpanTile;
-        const int rh = kTileRings / sw < 1 ? 1 : kTileRings / sw;
-        max_tiles += (size_t)((ns[m] + sw - 1) / sw) * (size_t)((t_steps + rh - 1) / rh);
+        max_tiles += (size_t)((ns[m] + sw - 1) / sw) * (size_t)t_steps;  // worst case: one row per tile
     }
     tab.N = off;
     tab.NV = voff;
