@@ -154,7 +154,7 @@ TVLM_DECLARE_BACKEND(float, f32)
  * loop -- wake shed, kinematic velocities, rhs, wake induction, solve, gamma post-processing,
  * unsteady forces, CL/CM, displacement, panel metrics -- runs without host round trips.
  * mode 0: plain FP64 (exact FP64 Biot-Savart, LU forward/backward substitution every step).
- * mode 1: wake induction in the FP64 seeded form (FP64 arithmetic, 1/sqrt and 1/x from the FP32
+ * mode 1: wake induction in the FP64 seeded form (FP64 arithmetic, 1/sqrt and 1/x from the FP64
  *         hardware estimates plus one Newton step), gamma = A^{-1} rhs with the FP64 inverse formed
  *         once from the LU; use_graph replays the loop as a hipGraph (needs a non-default stream).
  * All surfaces are lifting; LHS assembly uses iteration 0 as the reference does (vlm.cpp:402). */

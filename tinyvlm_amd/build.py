@@ -5,12 +5,22 @@ import subprocess
 from . import PKG_DIR
 
 
-def build_hip(jobs: int = 8) -> str:
-    """hipcc --offload-arch=gfx950 via csrc/Makefile -> tinyvlm_amd/libtinyvlm_hip.so (always rebuilds stale objects)."""
-    csrc = os.path.join(PKG_DIR, "csrc")
-    subprocess.run(["make", "-s", f"-j{jobs}", "-C", csrc], check=True)
+def _make(subdir: str, jobs: int, force: bool):
+    cmd = ["make", "-s", f"-j{jobs}", "-C", os.path.join(PKG_DIR, subdir)]
+    subprocess.run(cmd + (["-B"] if force else []), check=True)
+
+
+def build_hip(jobs: int = 8, force: bool = False) -> str:
+    """hipcc --offload-arch=gfx950 via csrc/Makefile -> tinyvlm_amd/libtinyvlm_hip.so (force: rebuild everything)."""
+    _make("csrc", jobs, force)
     return os.path.join(PKG_DIR, "libtinyvlm_hip.so")
 
 
-def build_all(jobs: int = 8):
-    return [build_hip(jobs)]
+def build_vlm_cpp(jobs: int = 8, force: bool = False) -> str:
+    """g++ via vlm/Makefile -> tinyvlm_amd/libtinyvlm_vlm.so (vlm::Backend over the C ABI) and vlm/bin/*."""
+    _make("vlm", jobs, force)
+    return os.path.join(PKG_DIR, "libtinyvlm_vlm.so")
+
+
+def build_all(jobs: int = 8, force: bool = False):
+    return [build_hip(jobs, force), build_vlm_cpp(jobs, force)]
