@@ -9,14 +9,19 @@ dt = c / (10 u_inf) = last panel chord / u_inf, 500 time steps; the wake grows t
 A bench "step" is one whole simulation (motion evaluation, pose, influence matrix, LU, and 500
 UVLM time steps with wake induction, solve, forces, CL and CM), all inside the HIP library.
 value = UVLM time steps per second = 500 * K / (time of K simulations), inputs resident on the
-device.  The served path evaluates the wake induction with FP64 arithmetic and FP32-seeded
-rsqrt/reciprocal (one Newton step), solves with the FP64 inverse and replays the loop as one
-hipGraph; it is accepted only if |dCL| <= 1e-6 at every step against the plain FP64 path
-(exact FP64 Biot-Savart, LU substitution each step), which is timed too.
+device.  The served path evaluates the wake induction with FP64 arithmetic (1/sqrt and 1/x from
+the FP64 hardware estimates plus one Newton step) over the shared-segment lattice of the wake,
+solves with the FP64 inverse and replays the loop as one hipGraph; it is accepted only if
+|dCL| <= 1e-6 at every step against the plain FP64 path (exact FP64 Biot-Savart ring by ring, LU
+substitution each step), which is timed too.
 
 Also reported: the roofline of the dominant kernel (wake induction, 398 algorithmic flops per
-receiver x wake-ring pair, timed with HIP events on the plan's stream) and the CPU baseline (the
-oracle's FP64 restatement of the reference CPU backend, OpenMP on the host cores).
+receiver x wake-ring pair -- the ring-by-ring count of the reference algorithm -- timed with HIP
+events on the plan's stream, against the FP64 VALU peak since it computes in FP64) and the CPU
+baseline (the oracle's FP64 restatement of the reference CPU backend, OpenMP on the host cores).
+
+--probe runs one served simulation without graph replay and one 1 GiB device copy (the
+calibration kernel for HBM byte counters), for profiler counter passes.
 """
 from __future__ import annotations
 
@@ -33,7 +38,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOPS_PER_PAIR = 398  # algorithmic flops per (receiver, ring) pair, both mirror images
-PEAK_FP32_VALU_TFLOPS = 157.3  # MI355X dense FP32 (packed) nominal
+PEAK_FP64_VALU_TFLOPS = 78.6  # MI355X FP64 vector peak (also non-packed FP32)
+COPY_PROBE_BYTES = 1 << 30
+# HBM bytes per k_wake_lattice launch of this workload (mean over its 499 launches), from the
+# FETCH_SIZE / WRITE_SIZE counter passes over `bench.py --probe`, each scaled by the same counter's
+# reading for a 1 GiB device copy (FETCH x 2.000, WRITE x 1.000); and the executed FP64 flops per
+# pair from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 pass.  Counters cannot be read in-process.
+WAKE_TRAFFIC_BYTES = 3282540
+WAKE_ALGORITHMIC_BYTES = 349384
+WAKE_EXECUTED_FLOPS_PER_PAIR = 286.7
 NS, NC, HALF_SPAN, CHORD = 40, 10, 2.0, 1.0
 T_STEPS = 501  # UVLM::run loops t_steps-1 times -> 500 time steps
 A_AMP = float(np.float32(2) * np.float32(3.141593) / np.float32(180))
@@ -98,6 +111,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3, help="timed plain FP64 simulations")
     ap.add_argument("--cpu-steps", type=int, default=500, help="UVLM steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true", help="one served simulation without graph + a 1 GiB copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -118,6 +132,16 @@ def main():
 
     # served path: seeded FP64 wake induction, FP64 inverse solve, hipGraph replay
     fast = UVLMPlan([verts], T_STEPS, dt, mode=MODE_FAST)
+    if args.probe:
+        run_plan(fast, 1, use_graph=False)
+        src = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "uvlm served simulation (no graph) + device copy",
+                          "copy_bytes_each_way": COPY_PROBE_BYTES, "cl_last": float(fast.results()["cl"][-1])}))
+        fast.close()
+        return
     _, _, _ = run_plan(fast, max(args.warmup, 1), use_graph=True)
     wall, gpu_s, lat = run_plan(fast, args.steps, use_graph=True)
     res_fast = fast.results()
@@ -139,12 +163,16 @@ def main():
     wake_ms_p, launches_p, pairs_p = plain.time_wake()
     roofline = {
         "bound": "valu",
-        "kernel": "k_wake_partials (FP64 seeded rsqrt/rcp form)",
+        "kernel": "k_wake_lattice (FP64, seeded rsqrt/rcp, shared-segment lattice)",
         "achieved": round(achieved, 3),
-        "peak": PEAK_FP32_VALU_TFLOPS,
+        "peak": PEAK_FP64_VALU_TFLOPS,
         "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK_FP32_VALU_TFLOPS, 4),
-        "traffic": None,
+        "frac": round(achieved / PEAK_FP64_VALU_TFLOPS, 4),
+        "traffic": WAKE_TRAFFIC_BYTES,
+        "traffic_unit": "bytes per launch (rocprofv3 counters, copy-calibrated)",
+        "algorithmic_bytes_per_launch": WAKE_ALGORITHMIC_BYTES,
+        "executed_fp64_flops_per_pair": WAKE_EXECUTED_FLOPS_PER_PAIR,
+        "executed_frac": round(achieved * WAKE_EXECUTED_FLOPS_PER_PAIR / FLOPS_PER_PAIR / PEAK_FP64_VALU_TFLOPS, 4),
         "avg_launch_us": round(avg_ms * 1e3, 3),
         "launches": launches,
         "pairs_per_launch": pairs / launches,
@@ -164,7 +192,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (f32-seeded rsqrt/rcp in the wake kernel)",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {
             "workload": "uvlm_pitch_heave_ar4_10c_x_40s_500steps",

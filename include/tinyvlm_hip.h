@@ -185,8 +185,10 @@ int tvlm_uvlm_run(tvlm_uvlm_plan* plan, int32_t use_graph);
  * last gamma[N], getrf info; null pointers are skipped */
 int tvlm_uvlm_results(tvlm_uvlm_plan* plan, double* cl, double* cm, double* gamma, int32_t* info);
 int tvlm_uvlm_destroy(tvlm_uvlm_plan* plan);
-/* measurement: one simulation without capture, HIP events around each wake-induction launch on
- * the plan's stream; returns summed kernel ms, launch count and (receiver, wake ring) pairs */
+/* measurement: one simulation without capture, then every step's wake-induction launch replayed
+ * back to back between two HIP events on the plan's stream; returns their total ms, the launch
+ * count and the (receiver, wake ring) pairs they evaluate.  Leaves the results of that simulation
+ * in the plan. */
 int tvlm_uvlm_time_wake(tvlm_uvlm_plan* plan, double* total_ms, int32_t* launches, double* pairs);
 
 #ifdef __cplusplus

@@ -340,8 +340,7 @@ struct tvlm_uvlm_plan {
     bool motion_set = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::vector<hipEvent_t> wake_events;  // [2 * step]: start/stop around each wake launch
-    bool instrument = false;
+    std::vector<hipEvent_t> wake_events;  // start/stop around the replayed wake launches
 
     int alloc(void** p, size_t bytes) {
         TVLM_HIP(hipMalloc(p, bytes));
@@ -441,6 +440,41 @@ int setup(tvlm_uvlm_plan* P) {
     return TVLM_OK;
 }
 
+// wake-induction partials of step i (> 0) into P->wpart; tiles = partial rows written
+int launch_wake(tvlm_uvlm_plan* P, int i, int& tiles) {
+    hipStream_t st = P->stream;
+    RecvTable<double> rt;
+    WakeTable<double> wt;
+    tiles = 0;
+    wake_tables(P, i, rt, wt, tiles);
+    TVLM_REQUIRE((size_t)tiles * P->N <= P->wpart_elems, "wake partial buffer too small");
+    dim3 grid((P->N + kRecv - 1) / kRecv, tiles);
+    if (P->mode == 1) {
+        // one rows-per-tile value for every surface of the launch: the largest chosen
+        int rh = 1;
+        for (int m = 0; m < wt.n; m++) rh = wt.s[m].rh > rh ? wt.s[m].rh : rh;
+        for (int m = 0; m < wt.n; m++) wt.s[m].rh = rh;
+        int t2 = 0;
+        for (int m = 0; m < wt.n; m++) {
+            wt.s[m].tile_begin = t2;
+            t2 += wt.s[m].tiles_span * ((wt.s[m].row_end - wt.s[m].row_begin + rh - 1) / rh);
+        }
+        wt.total_tiles = t2;
+        tiles = t2;
+        grid.y = tiles;
+        TVLM_REQUIRE((size_t)tiles * P->N <= P->wpart_elems, "wake partial buffer too small");
+        if (rh == 1)
+            hipLaunchKernelGGL((k_wake_lattice<double, 1>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt);
+        else if (rh == 2)
+            hipLaunchKernelGGL((k_wake_lattice<double, 2>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt);
+        else
+            hipLaunchKernelGGL((k_wake_lattice<double, 4>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt);
+    } else
+        hipLaunchKernelGGL((k_wake_partials<double, double, 0>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt, 0.0);
+    TVLM_LAUNCHED();
+    return TVLM_OK;
+}
+
 int enqueue_step(tvlm_uvlm_plan* P, int i) {
     hipStream_t st = P->stream;
     UTab& tab = P->tab;
@@ -449,36 +483,8 @@ int enqueue_step(tvlm_uvlm_plan* P, int i) {
         TVLM_LAUNCHED();
     }
     if (i > 0) {
-        RecvTable<double> rt;
-        WakeTable<double> wt;
         int tiles = 0;
-        wake_tables(P, i, rt, wt, tiles);
-        TVLM_REQUIRE((size_t)tiles * P->N <= P->wpart_elems, "wake partial buffer too small");
-        dim3 grid((P->N + kRecv - 1) / kRecv, tiles);
-        if (P->instrument) TVLM_HIP(hipEventRecord(P->wake_events[2 * i], st));
-        if (P->mode == 1) {
-            // one rows-per-tile value for every surface of the launch: the largest chosen
-            int rh = 1;
-            for (int m = 0; m < wt.n; m++) rh = wt.s[m].rh > rh ? wt.s[m].rh : rh;
-            for (int m = 0; m < wt.n; m++) wt.s[m].rh = rh;
-            int t2 = 0;
-            for (int m = 0; m < wt.n; m++) {
-                wt.s[m].tile_begin = t2;
-                t2 += wt.s[m].tiles_span * ((wt.s[m].row_end - wt.s[m].row_begin + rh - 1) / rh);
-            }
-            wt.total_tiles = t2;
-            tiles = t2;
-            grid.y = tiles;
-            if (rh == 1)
-                hipLaunchKernelGGL((k_wake_lattice<double, 1>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt);
-            else if (rh == 2)
-                hipLaunchKernelGGL((k_wake_lattice<double, 2>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt);
-            else
-                hipLaunchKernelGGL((k_wake_lattice<double, 4>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt);
-        } else
-            hipLaunchKernelGGL((k_wake_partials<double, double, 0>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt, 0.0);
-        TVLM_LAUNCHED();
-        if (P->instrument) TVLM_HIP(hipEventRecord(P->wake_events[2 * i + 1], st));
+        TVLM_TRY(launch_wake(P, i, tiles));
         launch_wake_reduce<double>(P->rhs, 1, P->wpart, tiles, P->N, st);
         TVLM_LAUNCHED();
     }
@@ -651,25 +657,26 @@ int tvlm_uvlm_time_wake(tvlm_uvlm_plan* P, double* total_ms, int32_t* launches, 
     TVLM_REQUIRE(P && P->motion_set && total_ms && launches && pairs, "bad argument");
     const int last = (int)P->t_steps - 1;
     if (P->wake_events.empty()) {
-        P->wake_events.resize(2 * (size_t)last, nullptr);
+        P->wake_events.resize(2, nullptr);
         for (auto& e : P->wake_events) TVLM_HIP(hipEventCreate(&e));
     }
+    // one whole simulation fills the wake; then every step's wake launch (same grid and operand
+    // sizes as in the loop) is replayed back to back between two events
     TVLM_TRY(setup(P));
-    P->instrument = true;
-    int rc = TVLM_OK;
-    for (int i = 0; i < last && rc == TVLM_OK; i++) rc = enqueue_step(P, i);
-    P->instrument = false;
-    TVLM_TRY(rc);
+    for (int i = 0; i < last; i++) TVLM_TRY(enqueue_step(P, i));
+    TVLM_HIP(hipEventRecord(P->wake_events[0], P->stream));
+    for (int i = 1; i < last; i++) {
+        int tiles = 0;
+        TVLM_TRY(launch_wake(P, i, tiles));
+    }
+    TVLM_HIP(hipEventRecord(P->wake_events[1], P->stream));
     TVLM_HIP(hipStreamSynchronize(P->stream));
     int64_t rings_per_row = 0;
     for (int m = 0; m < P->n_surf; m++) rings_per_row += P->tab.s[m].ns;
-    double ms = 0.0, pr = 0.0;
-    for (int i = 1; i < last; i++) {
-        float e = 0.f;
-        TVLM_HIP(hipEventElapsedTime(&e, P->wake_events[2 * i], P->wake_events[2 * i + 1]));
-        ms += e;
-        pr += (double)P->N * (double)rings_per_row * (double)i;
-    }
+    double pr = 0.0;
+    for (int i = 1; i < last; i++) pr += (double)P->N * (double)rings_per_row * (double)i;
+    float ms = 0.f;
+    TVLM_HIP(hipEventElapsedTime(&ms, P->wake_events[0], P->wake_events[1]));
     *total_ms = ms;
     *launches = last - 1;
     *pairs = pr;
