@@ -127,6 +127,11 @@ int tvlm_fill_f64(int32_t location, double* ptr, int64_t stride, double value, i
     /* LU::solve (vlm_backend.hpp:106/:113): x{N,nrhs} <- A^{-1} x ('N') */                         \
     int tvlm_getrs_##S(const tvlm_view* a, const int32_t* ipiv_dev, const tvlm_view* x,            \
                        void* stream);                                                               \
+    /* LSQ::init/solve (vlm_backend.hpp:121-129; LAPACKE_?gels 'N' in vlm_backend_cpu_lapack.cpp:   \
+     * 124-205): min ||A X - B|| for A{M,N}, M >= N, N + NRHS <= 32; A <- Householder QR (?geqrf   \
+     * layout), B{M,NRHS} <- Q^T B with X in rows 0..N-1; info_dev[0] = i if R(i,i) == 0 (X not  \
+     * computed), else 0.  No workspace (LSQ::init has nothing to size). */                        \
+    int tvlm_gels_##S(const tvlm_view* a, const tvlm_view* b, int32_t* info_dev, void* stream);    \
     /* BLAS::gemv (vlm_backend.hpp:84/:91), trans: 0 = No, 1 = Yes */                               \
     int tvlm_gemv_##S(T alpha, const tvlm_view* a, const tvlm_view* x, T beta, const tvlm_view* y, \
                       int32_t trans, void* stream);                                                 \

@@ -51,6 +51,7 @@ def _declare(L):
         getattr(L, f"orc_mesh_mac_{s}").restype = R
         getattr(L, f"orc_coeff_cl_multibody_{s}").restype = R
         getattr(L, f"orc_getrf_{s}").restype = ctypes.c_int
+        getattr(L, f"orc_gels_{s}").restype = ctypes.c_int
         getattr(L, f"orc_vlm_run_{s}").restype = ctypes.c_int
         getattr(L, f"orc_uvlm_run_{s}").restype = ctypes.c_int
 
@@ -292,6 +293,51 @@ def getrs(lu, ipiv, b):
                                                   _I64(lu.shape[0]), _P(_ptr(ipiv)), _P(_ptr(b2)),
                                                   _I64(b2.shape[0]))
     return b2.reshape(b.shape, order="F")
+
+
+def gels(a, b):
+    """LSQ::solve restatement (LAPACKE_?gels 'N'); returns (A overwritten by its QR factors,
+    B overwritten by Q^T B with X in rows 0..N-1, info)."""
+    a = np.asfortranarray(a.copy())
+    b = np.asfortranarray(b.copy())
+    b2 = b.reshape(b.shape[0], -1, order="F")
+    info = getattr(lib(), f"orc_gels_{_sfx(a.dtype)}")(_I64(a.shape[0]), _I64(a.shape[1]), _I64(b2.shape[1]),
+                                                       _P(_ptr(a)), _I64(a.shape[0]), _P(_ptr(b2)),
+                                                       _I64(b2.shape[0]))
+    return a, b2.reshape(b.shape, order="F"), info
+
+
+def anderson_acceleration(x0, f, max_iter=100, tol_res=1e-6, m=3):
+    """anderson_acceleration restatement (vlm/src/vlm_solvers.cpp:6-115): type-II Anderson mixing
+    with a circular history of m columns, least squares by gels.  f(x) returns the fixed-point map's
+    value.  Returns (x, iterations k)."""
+    dt = x0.dtype
+    n = x0.shape[0]
+    X = np.zeros((n, m), dtype=dt, order="F")
+    G = np.zeros((n, m), dtype=dt, order="F")
+    x_curr = x0.copy()
+    x_new = f(x_curr)
+    g_curr = x_new - x_curr
+    X[:, 0] = g_curr
+    x_curr = x_new.copy()
+    x_new = f(x_curr)
+    g_new = x_new - x_curr
+    G[:, 0] = g_new - g_curr
+    g_curr = g_new.copy()
+    k = 1
+    while k < max_iter and np.sqrt(np.dot(g_curr, g_curr)) > tol_res:
+        mk = min(m, k)
+        _, sol, info = gels(G[:, :mk], g_curr.copy())
+        gam = sol[:mk]
+        x_new = x_curr + g_curr - X[:, :mk] @ gam - G[:, :mk] @ gam
+        X[:, k % m] = x_new - x_curr
+        x_curr = x_new.copy()
+        x_new = f(x_curr)
+        g_new = x_new - x_curr
+        G[:, k % m] = g_new - g_curr
+        g_curr = g_new.copy()
+        k += 1
+    return x_curr, k
 
 
 def vlm_run(verts_list, alpha, beta=0.0, u_inf=1.0, rho=1.0, cm_ref=(0.25, 0.0, 0.0)):

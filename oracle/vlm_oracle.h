@@ -80,6 +80,7 @@ extern "C" {
     int orc_getrf_##S(int64_t n, REAL* a, int64_t lda, int32_t* ipiv);                                \
     void orc_getrs_##S(int64_t n, int64_t nrhs, const REAL* a, int64_t lda, const int32_t* ipiv,      \
                        REAL* b, int64_t ldb);                                                         \
+    int orc_gels_##S(int64_t m, int64_t n, int64_t nrhs, REAL* a, int64_t lda, REAL* b, int64_t ldb); \
     void orc_quarterchord_##S(int64_t ns, int64_t nc, REAL* verts);                                   \
     int orc_vlm_run_##S(int32_t nsurf, const int64_t* ns, const int64_t* nc,                          \
                         REAL* const* verts_init, REAL alpha, REAL beta, REAL u_inf, REAL rho,         \

@@ -510,6 +510,65 @@ void FN(orc_getrs)(int64_t n, int64_t nrhs, const REAL* a, int64_t lda, const in
     }
 }
 
+/* LSQ::solve = LAPACKE_?gels 'N' (vlm/backends/cpu/src/vlm_backend_cpu_lapack.cpp:141-205), M >= N:
+ * ?geqr2 Householder QR (reflector ?larfg: beta = -sign(alpha) sqrt(alpha^2 + |x|^2),
+ * tau = (beta - alpha) / beta, v = x / (alpha - beta)), each H(j) applied to the trailing columns
+ * (?larf) and to B in the order H(1), H(2), ... (?orm2r), then R X = (Q^T B)(0:N) by
+ * column-oriented back substitution (?trsm).  Returns info = i if R(i,i) == 0 (then X is not
+ * computed); an all-zero A sets B to zero.  LAPACK's rescaling of extreme norms is not restated. */
+int FN(orc_gels)(int64_t m, int64_t n, int64_t nrhs, REAL* a, int64_t lda, REAL* b, int64_t ldb) {
+    if (m < n) return -1;
+    REAL amax = 0;
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t i = 0; i < m; i++) {
+            const REAL v = a[i + j * lda] < 0 ? -a[i + j * lda] : a[i + j * lda];
+            if (v > amax) amax = v;
+        }
+    if (amax == 0) {
+        for (int64_t r = 0; r < nrhs; r++)
+            for (int64_t i = 0; i < m; i++) b[i + r * ldb] = 0;
+        return 0;
+    }
+    for (int64_t j = 0; j < n; j++) {
+        REAL* cj = a + j * lda;
+        REAL s = 0;
+        for (int64_t i = j + 1; i < m; i++) s += cj[i] * cj[i];
+        const REAL alpha = cj[j];
+        REAL tau = 0, beta = alpha;
+        if (s != 0) {
+            const REAL mag = SQRT(alpha * alpha + s);
+            beta = alpha >= 0 ? -mag : mag;
+            tau = (beta - alpha) / beta;
+            const REAL scal = (REAL)1 / (alpha - beta);
+            for (int64_t i = j + 1; i < m; i++) cj[i] *= scal;
+        }
+        if (tau != 0) {
+            const int64_t na = n - j - 1;
+            for (int64_t k = 0; k < na + nrhs; k++) {
+                REAL* c = k < na ? a + (j + 1 + k) * lda : b + (k - na) * ldb;
+                REAL w = c[j];
+                for (int64_t i = j + 1; i < m; i++) w += cj[i] * c[i];
+                w *= tau;
+                c[j] -= w;
+                for (int64_t i = j + 1; i < m; i++) c[i] -= w * cj[i];
+            }
+        }
+        cj[j] = beta;
+    }
+    for (int64_t i = 0; i < n; i++)
+        if (a[i + i * lda] == 0) return (int)(i + 1);
+    for (int64_t r = 0; r < nrhs; r++) {
+        REAL* x = b + r * ldb;
+        for (int64_t jj = n - 1; jj >= 0; jj--) {
+            const REAL* col = a + jj * lda;
+            const REAL xj = x[jj] / col[jj];
+            x[jj] = xj;
+            for (int64_t i = 0; i < jj; i++) x[i] -= xj * col[i];
+        }
+    }
+    return 0;
+}
+
 /* mesh_quarterchord (vlm/src/vlm_mesh.cpp:15-32): move each chordwise vertex row to its quarter
  * chord, then extrapolate the trailing edge with the float constants 4.f/3.f and 1.f/3.f. */
 void FN(orc_quarterchord)(int64_t ns, int64_t nc, REAL* v) {

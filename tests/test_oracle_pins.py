@@ -447,3 +447,53 @@ def test_uvlm_constant_pitch_tends_to_steady(oracle):
     # the single-pitch steady value is half as large: the double pose is really there
     _, cl_1, _, _ = oracle.vlm_run([oracle.displace_wing(np.asfortranarray(T), v)], 0.0)
     assert abs(cl_t[-1] - cl_1) / abs(cl_1) > 0.5
+
+
+# ---- LSQ (gels) and Anderson acceleration ---------------------------------------------------
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("m,n,nrhs", [(50, 6, 2), (7, 7, 1), (300, 10, 1), (33, 1, 3)])
+def test_oracle_gels_matches_lapack_layout(oracle, dt, m, n, nrhs):
+    """A <- geqrf factors and B <- Q^T B with X on top, as LAPACK ?gels leaves them (scipy's
+    LAPACK binding is the independent implementation); X also equals numpy's lstsq."""
+    import scipy.linalg.lapack as la
+
+    rng = np.random.default_rng(m * 100 + n)
+    A = rng.standard_normal((m, n)).astype(dt)
+    B = rng.standard_normal((m, nrhs)).astype(dt)
+    a, b, info = oracle.gels(A, B)
+    ref = (la.dgels if dt == np.float64 else la.sgels)(np.asfortranarray(A), np.asfortranarray(B))
+    assert info == 0 and ref[2] == 0
+    tol = 1e-12 if dt == np.float64 else 2e-4
+    scale = max(1.0, float(np.max(np.abs(ref[1]))))
+    assert np.max(np.abs(a - ref[0])) <= tol * max(1.0, float(np.max(np.abs(ref[0]))))
+    assert np.max(np.abs(b - ref[1])) <= tol * scale
+    x = np.linalg.lstsq(A.astype(np.float64), B.astype(np.float64), rcond=None)[0]
+    assert np.max(np.abs(b[:n] - x)) <= tol * 10 * max(1.0, float(np.max(np.abs(x))))
+
+
+def test_oracle_gels_rank_deficient_and_zero(oracle):
+    A = np.zeros((8, 3))
+    A[:, 0] = np.arange(8.0)
+    A[:, 2] = 1.0  # column 1 zero -> R(2,2) == 0 exactly after the first reflector
+    _, _, info = oracle.gels(A, np.ones((8, 1)))
+    assert info == 2
+    _, b, info = oracle.gels(np.zeros((5, 2)), np.ones((5, 1)))
+    assert info == 0 and np.all(b == 0)
+
+
+def test_oracle_anderson_linear_fixed_point(oracle):
+    """x = M x + c with spectral radius < 1: Anderson reaches (I - M)^-1 c, and far faster than
+    the plain fixed-point iteration (which needs > 100 steps here)."""
+    n = 64
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    M = 0.9 * np.sin(1.0 + i + 3.0 * j) / n
+    c = np.cos(np.arange(n, dtype=np.float64))
+    x, k = oracle.anderson_acceleration(np.zeros(n), lambda v: M @ v + c, 100, 1e-12, 3)
+    exact = np.linalg.solve(np.eye(n) - M, c)
+    assert k < 40
+    assert np.max(np.abs(x - exact)) < 1e-10
+    # with m = 1 history the method still converges (Anderson(1) = secant mixing)
+    x1, k1 = oracle.anderson_acceleration(np.zeros(n), lambda v: M @ v + c, 100, 1e-12, 1)
+    assert k1 < 100 and np.max(np.abs(x1 - exact)) < 1e-10

@@ -265,6 +265,13 @@ def getrs(a, ipiv, x, stream=None):
                           stream_ptr(stream)), "getrs")
 
 
+def gels(a, b, info, stream=None):
+    """LSQ::solve: a {M,N} <- QR factors, b {M,NRHS} (or {M}) <- Q^T b with X in rows 0..N-1."""
+    b2 = b if b.dim() == 2 else b.unsqueeze(1)
+    check(_fn("gels", a)(ctypes.byref(view(a)), ctypes.byref(view(b2)), ctypes.c_void_p(info.data_ptr()),
+                         stream_ptr(stream)), "gels")
+
+
 def kinematic_velocities(dT_list, colloc, velocities, stream=None):
     """dT_list: one 4x4 numpy time derivative per surface."""
     R = _real(colloc[0])

@@ -183,6 +183,31 @@ class HIPLU final : public LU {
     void solve(const TensorView2dD& A, const TensorView2dD& x) override { solve_(A, x); }
 };
 
+// LAPACK ?gels 'N' semantics on the device (vlm_backend_cpu_lapack.cpp:96-205); no workspace to
+// size, and a rank-deficient A throws like the reference's CHECK_LAPACK exit
+class HIPLSQ final : public LSQ {
+    DeviceScalars m_s;
+
+    template <typename T>
+    void solve_(const TensorView<T, 2, Location::Device>& A, const TensorView<T, 2, Location::Device>& B) {
+        const tvlm_view a = tv(A), b = tv(B);
+        if constexpr (sizeof(T) == 8)
+            check(tvlm_gels_f64(&a, &b, m_s.dev<int32_t>(), nullptr), "gels");
+        else
+            check(tvlm_gels_f32(&a, &b, m_s.dev<int32_t>(), nullptr), "gels");
+        int32_t info = 0;
+        m_s.get(&info, 1);
+        if (info != 0) throw std::runtime_error("LSQ: rank-deficient matrix (gels info " + std::to_string(info) + ")");
+    }
+
+   public:
+    explicit HIPLSQ(std::unique_ptr<Memory> memory) : LSQ(std::move(memory)) {}
+    void init(const TensorView2fD&, const TensorView2fD&) override {}
+    void solve(const TensorView2fD& A, const TensorView2fD& B) override { solve_(A, B); }
+    void init(const TensorView2dD&, const TensorView2dD&) override {}
+    void solve(const TensorView2dD& A, const TensorView2dD& B) override { solve_(A, B); }
+};
+
 class BackendHIP final : public Backend {
     DeviceScalars m_s;
 
@@ -286,9 +311,7 @@ class BackendHIP final : public Backend {
     std::unique_ptr<Memory> create_memory_manager() override { return std::make_unique<HIPMemory>(); }
     std::unique_ptr<LU> create_lu_solver() override { return std::make_unique<HIPLU>(create_memory_manager()); }
     std::unique_ptr<BLAS> create_blas() override { return std::make_unique<HIPBLAS>(); }
-    std::unique_ptr<LSQ> create_lsq_solver() override {
-        throw std::runtime_error("LSQ (least squares for Anderson acceleration) is not part of this build yet");
-    }
+    std::unique_ptr<LSQ> create_lsq_solver() override { return std::make_unique<HIPLSQ>(create_memory_manager()); }
 };
 
 }  // namespace
