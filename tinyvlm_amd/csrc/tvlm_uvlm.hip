@@ -6,7 +6,10 @@
 // needs no host interaction: per step
 //   k_pre    : wake shed of the trailing-edge vertex row, kinematic velocities -(dT [x;1]) at the
 //              t = 0 collocation points, rhs = -(V . n)                        (vlm.cpp:412-438)
-//   k_wake_partials + k_wake_reduce : rhs -= wake induction                   (vlm.cpp:439)
+//   wake induction + k_wake_reduce : rhs -= wake induction                    (vlm.cpp:439)
+//              plain mode: k_wake_partials, exact FP64 Biot-Savart ring by ring;
+//              fast mode : k_wake_lattice, every wake segment once with the difference of the
+//              circulations of the two rings it borders
 //   solve    : plain mode: LU forward/backward substitution (getrs);
 //              fast mode : gamma = A^{-1} rhs with the FP64 inverse formed once from the LU
 //   k_post_advance (one work-group): gamma post-processing, shed of Gamma_TE into the wake,
@@ -14,7 +17,7 @@
 //              (vlm.cpp:441-487); then displacement by T(t+dt), panel metrics (vlm.cpp:489-507)
 //              and the next step's wake shed, velocities and rhs
 // The fast mode evaluates the wake induction in the FP64 seeded form (FP64 arithmetic, rsqrt and
-// reciprocal from the FP32 hardware estimates plus one Newton step) and replays the whole loop as
+// reciprocal from the FP64 hardware estimates plus one Newton step) and replays the whole loop as
 // one hipGraph.
 #include <cstring>
 #include <vector>
