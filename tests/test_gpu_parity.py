@@ -205,7 +205,7 @@ def test_wake_multi_surface_and_tiles(H, torch, oracle, prec):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("n", [1, 5, 64, 65, 130, 700, 2100])
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 130, 700, 1025, 2100, 4100, 9000])
 def test_lu_random(H, torch, prec, n):
     import scipy.linalg as sl
 

@@ -49,10 +49,12 @@ inline bool plane_contiguous(const tvlm_view& v) {
     return v.ptr != nullptr && v.stride[0] == 1 && v.stride[1] == v.shape[0];
 }
 
-// Per-device scratch buffer owned by the library (grow-only).  Used for deterministic two-stage
-// reductions.  Not thread-safe across concurrent callers on one device, like the reference's
-// CUDA scratch scalars (vlm_backend_cuda.hpp:45-54).
-int device_workspace(size_t bytes, void** out);
+// Per-device scratch buffers owned by the library (grow-only), one per slot: wake-induction tile
+// partials, LU row-interchange lists, triangular-solve block inverses.  Not safe for concurrent
+// callers on one device (two streams using the same slot), like the reference's CUDA scratch
+// scalars (vlm_backend_cuda.hpp:45-54).
+enum WorkspaceSlot { kWsWakePartials = 0, kWsLuPerm = 1, kWsSolveInv = 2, kWorkspaceSlots = 3 };
+int device_workspace(size_t bytes, void** out, int slot);
 
 // ---- small vector type ------------------------------------------------------------------------
 template <typename T>

@@ -33,15 +33,18 @@ struct Workspace {
     size_t bytes = 0;
 };
 std::mutex g_ws_mutex;
-std::vector<Workspace> g_ws;
+std::vector<Workspace> g_ws;  // [device * kWorkspaceSlots + slot]
+
 }  // namespace
 
-int device_workspace(size_t bytes, void** out) {
+int device_workspace(size_t bytes, void** out, int slot) {
+    TVLM_REQUIRE(slot >= 0 && slot < kWorkspaceSlots, "workspace slot %d out of range", slot);
     int dev = 0;
     TVLM_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_ws_mutex);
-    if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
-    Workspace& w = g_ws[dev];
+    const size_t idx = (size_t)dev * kWorkspaceSlots + slot;
+    if (g_ws.size() <= idx) g_ws.resize(idx + 1);
+    Workspace& w = g_ws[idx];
     if (w.bytes < bytes) {
         if (w.ptr) {
             // a previous launch may still read the old buffer
