@@ -486,8 +486,9 @@ __global__ __launch_bounds__(64) void k_swap_trsm(T* __restrict__ a, int64_t lda
 
 // C(m x n) -= A(m x kb) B(kb x n) on the matrix cores: 64 x 64 tile per work-group of 4 waves, each
 // wave a 32 x 32 quarter as 2 x 2 blocks of the 16x16x4 MFMA (A operand: lane l holds A(l%16, l/16);
-// B operand: B(l/16, l%16); accumulator register r of lane l: C(4(l/16) + r, l%16)).  A and B
-// are staged k-major in LDS; the accumulators start from C and A enters negated.
+// B operand: B(l/16, l%16); accumulator register r of lane l holds C(row(l/16, r), l%16) with
+// row = 4 (l/16) + r for the FP32 instruction and l/16 + 4 r for the FP64 one, as measured on
+// gfx950).  A and B are staged k-major in LDS; the accumulators start from C and A enters negated.
 template <typename T>
 struct MfmaVec;
 template <>
@@ -496,6 +497,7 @@ struct MfmaVec<double> {
     static __device__ __forceinline__ v4 mfma(double x, double y, v4 c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
 };
 template <>
 struct MfmaVec<float> {
@@ -503,6 +505,7 @@ struct MfmaVec<float> {
     static __device__ __forceinline__ v4 mfma(float x, float y, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
 
 template <typename T>
@@ -532,7 +535,7 @@ __global__ __launch_bounds__(256) void k_update_mfma(T* __restrict__ C, int64_t 
             const int col = j0 + wj + 16 * bj + lr;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = i0 + wi + 16 * bi + 4 * lg + r;
+                const int row = i0 + wi + 16 * bi + MfmaVec<T>::row(lg, r);
                 acc[bi][bj][r] = (row < m && col < n) ? C[row + (int64_t)col * ldc] : T(0);
             }
         }
@@ -557,7 +560,7 @@ __global__ __launch_bounds__(256) void k_update_mfma(T* __restrict__ C, int64_t 
             const int col = j0 + wj + 16 * bj + lr;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = i0 + wi + 16 * bi + 4 * lg + r;
+                const int row = i0 + wi + 16 * bi + MfmaVec<T>::row(lg, r);
                 if (row < m && col < n) C[row + (int64_t)col * ldc] = acc[bi][bj][r];
             }
         }
