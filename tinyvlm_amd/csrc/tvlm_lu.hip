@@ -4,15 +4,17 @@
 // column-major in place L\U, unit lower L, 1-based ipiv, pivot = first row of max |a| in the
 // column, info = first exactly-zero pivot (1-based), factorisation continues past it.
 //
-// getrf is blocked right-looking with panel width NB:
-//   1. k_panel    : one work-group factors the n-k0 x kb panel (pivot search by LDS tree,
-//                   row swap inside the panel, reciprocal scaling, rank-1 update);
-//   2. k_laswp    : the panel's interchanges applied to the columns left and right of it;
-//   3. k_trsm     : U12 = L11^{-1} A12 with L11 staged in LDS, one thread per column;
-//   4. k_update   : A22 -= L21 U12, 64x64 output tiles through LDS, 4x4 register blocking.
-// getrs applies the interchanges, then forward/backward substitution: one work-group per
-// right-hand side for n <= kSmallSolve, otherwise blocked (diagonal block in one work-group,
-// off-diagonal update by all work-groups).
+// getrf is blocked right-looking; each panel (64 columns, 32 while more than 2048 rows remain):
+//   1. k_panel_reg  : one work-group factors the panel (register-resident sub-panels, DPP argmax,
+//                     LDS-only barriers) and lists the panel's composed row interchanges;
+//   2. k_swap_trsm  : the interchanges on the columns left and right of the panel, and
+//                     U12 = L11^{-1} A12 for the columns right of it;
+//   3. k_update_mfma: A22 -= L21 U12 on the matrix cores.
+// Panels taller than 8192 rows use the column-by-column k_panel + k_laswp + k_trsm instead.
+// getrs (n <= 16384): the interchanges are composed per block of 64 pivots and the 64 x 64
+// diagonal blocks of L and U inverted (k_solve_prep), then one work-group per right-hand side
+// applies the interchanges and both blocked substitutions with the column in LDS
+// (k_solve_blocked); larger n uses block-row kernels (k_diag_solve / k_off_update).
 #include "tvlm_internal.hpp"
 
 namespace tvlm {
@@ -186,85 +188,197 @@ __global__ __launch_bounds__(256) void k_update(T* __restrict__ C, int64_t ldc, 
 
 // ---- fast getrf: register-resident panel, fused interchange + TRSM, MFMA trailing update ------
 //
-// k_panel_reg: one work-group of 1024 threads factors the panel rows [k0, n) x columns [k0, k0+kb),
-// kb <= NB.  Thread t owns rows k0 + t + 1024 q (q < Q).  The panel is processed left-looking in
+// k_panel_reg: one work-group of NT threads factors the panel rows [k0, n) x columns [k0, k0+kb),
+// kb <= NB.  Thread t owns rows k0 + t + NT q (q < Q).  The panel is processed left-looking in
 // sub-panels of IB columns held in registers (Q x IB values per thread): a sub-panel first receives
 // the updates of the panel's earlier sub-panels (blocked forward substitution with the inverses of
 // their unit-lower IB x IB diagonal blocks, kept in LDS), then is factored column by column with
 // partial pivoting (argmax by wave shuffles + one LDS exchange, first index on ties; the pivot row
-// and row k exchange through LDS; reciprocal scaling and rank-1 update in registers).  Each
-// interchange is applied at once to the panel's other columns in global memory.  At the end thread
-// 0 composes the panel's interchanges into a list of (destination, source) rows for the rest of the
-// matrix.
-constexpr int kPT = 1024;
-constexpr int kPW = kPT / 64;
+// and row k exchange through LDS; reciprocal scaling and rank-1 update in registers).  Inside the
+// column loop only LDS is touched and the work-group synchronises with an LDS-only barrier, so no
+// step waits for global memory; pivots stay in LDS until the panel ends.  Each sub-panel's
+// interchanges reach the panel's other columns as one gather / scatter, and at the end thread 0
+// composes the panel's interchanges into (destination, source) row pairs for the rest of the matrix.
+
+// barrier for LDS traffic only: waits for this wave's LDS operations, not for global memory
+__device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 template <typename T>
 __device__ __forceinline__ void argmax_merge(T& best, int& bi, T v, int i) {
-    if (v > best || (v == best && i < bi)) {
-        best = v;
-        bi = i;
-    }
+    const bool take = (v > best) | ((v == best) & (i < bi));
+    best = take ? v : best;
+    bi = take ? i : bi;
 }
 
-template <typename T, int Q, int IB>
-__global__ __launch_bounds__(kPT) void k_panel_reg(T* __restrict__ a, int64_t lda, int n, int k0, int kb,
-                                                   int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
-                                                   int32_t* __restrict__ perm) {
-    __shared__ T s_val[kPW];
-    __shared__ int s_idx[kPW];
+// One DPP step of a wave argmax: the value and index of the lane CTRL selects (row_shr / row_bcast
+// within the row and bank masks) are merged in; lanes without a source keep their own pair.
+template <int CTRL, int ROWM>
+__device__ __forceinline__ void dpp_argmax_step(double& v, int& i) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWM, 0xf, false);
+    const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWM, 0xf, false);
+    const int oi = __builtin_amdgcn_update_dpp(i, i, CTRL, ROWM, 0xf, false);
+    argmax_merge(v, i, __hiloint2double(ohi, olo), oi);
+}
+template <int CTRL, int ROWM>
+__device__ __forceinline__ void dpp_argmax_step(float& v, int& i) {
+    const int ov = __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROWM, 0xf, false);
+    const int oi = __builtin_amdgcn_update_dpp(i, i, CTRL, ROWM, 0xf, false);
+    argmax_merge(v, i, __int_as_float(ov), oi);
+}
+__device__ __forceinline__ double readlane63(double v) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+__device__ __forceinline__ float readlane63(float v) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+// wave-wide argmax (larger value, then smaller index), result in every lane: row_shr 1, 2, 4, 8
+// gathers each row's maximum into its lane 15, row_bcast 15 / 31 carry it across rows to lane 63
+template <typename T>
+__device__ __forceinline__ void wave_argmax(T& v, int& i) {
+    dpp_argmax_step<0x111, 0xf>(v, i);
+    dpp_argmax_step<0x112, 0xf>(v, i);
+    dpp_argmax_step<0x114, 0xf>(v, i);
+    dpp_argmax_step<0x118, 0xf>(v, i);
+    dpp_argmax_step<0x142, 0xa>(v, i);
+    dpp_argmax_step<0x143, 0xc>(v, i);
+    v = readlane63(v);
+    i = __builtin_amdgcn_readlane(i, 63);
+}
+
+// Composes the interchanges pos[i] <-> piv[i] (i < cnt <= 64, applied in order, piv[i] >= pos[i],
+// positions [first, first + cnt)) into (dst, src) row pairs, using one wave: lane l holds the source
+// row of position first + l, and the rows outside the block that the interchanges touch are kept as
+// (position, source) entries, one per lane.  Every lane calls it; returns the pair count (<= 2 cnt).
+__device__ int wave_compose_swaps(int first, int cnt, const int* piv, int* pd, int* ps) {
+    const int lane = threadIdx.x & 63;
+    int src = first + lane;  // position first + lane
+    int okey = -1, oval = -1;
+    int nout = 0;
+    for (int i = 0; i < cnt; i++) {
+        const int p = __builtin_amdgcn_readfirstlane(piv[i]);
+        if (p == first + i) continue;
+        const int vi = __builtin_amdgcn_readlane(src, i);
+        if (p < first + cnt) {
+            const int j = p - first;
+            const int vj = __builtin_amdgcn_readlane(src, j);
+            src = lane == i ? vj : (lane == j ? vi : src);
+        } else {
+            const unsigned long long m = __ballot(okey == p);
+            int e;
+            if (m != 0ull) {
+                e = __ffsll((long long)m) - 1;
+            } else {
+                e = nout++;
+                okey = lane == e ? p : okey;
+                oval = lane == e ? p : oval;
+            }
+            const int ve = __builtin_amdgcn_readlane(oval, e);
+            src = lane == i ? ve : src;
+            oval = lane == e ? vi : oval;
+        }
+    }
+    const bool in_pair = lane < cnt && src != first + lane;
+    const bool out_pair = lane < nout && oval != okey;
+    const unsigned long long mi = __ballot(in_pair), mo = __ballot(out_pair);
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int ni = __popcll(mi);
+    if (in_pair) {
+        const int k = __popcll(mi & below);
+        pd[k] = first + lane;
+        ps[k] = src;
+    }
+    if (out_pair) {
+        const int k = ni + __popcll(mo & below);
+        pd[k] = okey;
+        ps[k] = oval;
+    }
+    return ni + __popcll(mo);
+}
+
+template <typename T, int NT, int Q, int IB>
+__global__ __launch_bounds__(NT) void k_panel_reg(T* __restrict__ a, int64_t lda, int n, int k0, int kb,
+                                                  int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
+                                                  int32_t* __restrict__ perm) {
+    constexpr int NW = NT / 64;
+    __shared__ T s_val[NW];
+    __shared__ int s_idx[NW];
     __shared__ T s_rowp[IB], s_rowc[IB];
     __shared__ T s_inv[NB / IB][IB][IB];
     __shared__ T s_blk[IB][IB];
     __shared__ T s_u[IB][IB];
-    __shared__ int s_piv[NB];
-    __shared__ int s_src[NB], s_okey[NB], s_oval[NB];
+    __shared__ int s_arg[NB], s_piv[NB];
+    __shared__ int s_pd[2 * NB], s_ps[2 * NB], s_pcnt, s_zero;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) s_zero = 0;
     T x[Q][IB];
 
     for (int c0 = k0, s = 0; c0 < k0 + kb; c0 += IB, s++) {
         const int ib = min(IB, k0 + kb - c0);
 #pragma unroll
         for (int q = 0; q < Q; q++) {
-            const int r = k0 + t + kPT * q;
+            const int r = k0 + t + NT * q;
 #pragma unroll
             for (int j = 0; j < IB; j++) x[q][j] = (r < n && j < ib) ? a[r + (int64_t)(c0 + j) * lda] : T(0);
         }
-        // updates from the earlier sub-panels sp < s (rows b0 .. b0+IB of block sp hold U after this)
-        for (int sp = 0; sp < s; sp++) {
+        // updates from the earlier sub-panels sp < s (rows b0 .. b0+IB of block sp hold U after this);
+        // when registers allow, the L block of round sp + 1 is loaded while round sp computes (two
+        // register buffers, rounds taken in pairs so that both stay statically indexed)
+        T l0[Q][IB], l1[Q][IB];
+        auto load_l = [&](T (&l)[Q][IB], int b) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const int r = k0 + t + NT * q;
+#pragma unroll
+                for (int p = 0; p < IB; p++) l[q][p] = (r >= b + IB && r < n) ? a[r + (int64_t)(b + p) * lda] : T(0);
+            }
+        };
+        auto round = [&](int sp, T (&lc)[Q][IB], T (&ln)[Q][IB], bool prefetch) {
             const int b0 = k0 + sp * IB;
+            if (prefetch && sp + 1 < s) load_l(ln, b0 + IB);
             const int jj = t - (b0 - k0);  // owners of the block's rows: q = 0
             if (jj >= 0 && jj < IB) {
 #pragma unroll
                 for (int j = 0; j < IB; j++) s_blk[jj][j] = x[0][j];
             }
-            __syncthreads();
+            lds_sync();
             if (t < IB * IB) {
                 const int i = t / IB, j = t % IB;
                 T u = T(0);
                 for (int p = 0; p <= i; p++) u += s_inv[sp][i][p] * s_blk[p][j];
                 s_u[i][j] = u;
             }
-            __syncthreads();
+            lds_sync();
             if (jj >= 0 && jj < IB) {
 #pragma unroll
                 for (int j = 0; j < IB; j++) x[0][j] = s_u[jj][j];
             }
 #pragma unroll
             for (int q = 0; q < Q; q++) {
-                const int r = k0 + t + kPT * q;
+                const int r = k0 + t + NT * q;
                 if (r >= b0 + IB && r < n) {
-                    T l[IB];
-#pragma unroll
-                    for (int p = 0; p < IB; p++) l[p] = a[r + (int64_t)(b0 + p) * lda];
 #pragma unroll
                     for (int j = 0; j < IB; j++) {
                         T acc = x[q][j];
 #pragma unroll
-                        for (int p = 0; p < IB; p++) acc -= l[p] * s_u[p][j];
+                        for (int p = 0; p < IB; p++) acc -= lc[q][p] * s_u[p][j];
                         x[q][j] = acc;
                     }
                 }
+            }
+            lds_sync();  // s_u, s_blk reused by the next block
+        };
+        if constexpr (Q * IB * sizeof(T) <= 256) {
+            if (s > 0) load_l(l0, k0);
+            for (int sp = 0; sp < s; sp += 2) {
+                round(sp, l0, l1, true);
+                if (sp + 1 < s) round(sp + 1, l1, l0, true);
+            }
+        } else {  // registers for one buffer only: each round loads its own block first
+            for (int sp = 0; sp < s; sp++) {
+                load_l(l0, k0 + sp * IB);
+                round(sp, l0, l1, false);
             }
         }
         // factor the sub-panel
@@ -276,27 +390,22 @@ __global__ __launch_bounds__(kPT) void k_panel_reg(T* __restrict__ a, int64_t ld
                 int bi = n;
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
-                    const int r = k0 + t + kPT * q;
+                    const int r = k0 + t + NT * q;
                     if (r >= c && r < n) argmax_merge(best, bi, (T)fabs(x[q][jj]), r);
                 }
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) {
-                    const T ob = __shfl_xor(best, o);
-                    const int oi = __shfl_xor(bi, o);
-                    argmax_merge(best, bi, ob, oi);
-                }
+                wave_argmax(best, bi);
                 if (lane == 0) {
                     s_val[w] = best;
                     s_idx[w] = bi;
                 }
-                __syncthreads();
+                lds_sync();
                 best = s_val[0];
                 bi = s_idx[0];
 #pragma unroll
-                for (int ww = 1; ww < kPW; ww++) argmax_merge(best, bi, s_val[ww], s_idx[ww]);
+                for (int ww = 1; ww < NW; ww++) argmax_merge(best, bi, s_val[ww], s_idx[ww]);
                 const int p = bi;
-                const int tp = (p - k0) % kPT, qp = (p - k0) / kPT;
-                const int tc = (c - k0) % kPT, qc = (c - k0) / kPT;
+                const int tp = (p - k0) % NT, qp = (p - k0) / NT;
+                const int tc = (c - k0) % NT, qc = (c - k0) / NT;
                 if (t == tp) {
 #pragma unroll
                     for (int q = 0; q < Q; q++)
@@ -313,13 +422,13 @@ __global__ __launch_bounds__(kPT) void k_panel_reg(T* __restrict__ a, int64_t ld
                             for (int j = 0; j < IB; j++) s_rowc[j] = x[q][j];
                         }
                 }
-                __syncthreads();
+                lds_sync();
                 const T piv = s_rowp[jj];
                 const bool nz = piv != T(0);
                 if (t == 0) {
-                    ipiv[c] = p + 1;
+                    s_arg[c - k0] = p;
                     s_piv[c - k0] = nz ? p : c;
-                    if (!nz && info[0] == 0) info[0] = c + 1;
+                    if (!nz && s_zero == 0) s_zero = c + 1;
                 }
                 if (nz) {
                     if (p != c) {
@@ -339,20 +448,11 @@ __global__ __launch_bounds__(kPT) void k_panel_reg(T* __restrict__ a, int64_t ld
                                     for (int j = 0; j < IB; j++) x[q][j] = s_rowc[j];
                                 }
                         }
-                        if (t < kb) {
-                            const int col = k0 + t;
-                            if (col < c0 || col >= c0 + ib) {
-                                T* cc = a + (int64_t)col * lda;
-                                const T vc = cc[c], vp = cc[p];
-                                cc[c] = vp;
-                                cc[p] = vc;
-                            }
-                        }
                     }
                     const T rinv = T(1) / piv;
 #pragma unroll
                     for (int q = 0; q < Q; q++) {
-                        const int r = k0 + t + kPT * q;
+                        const int r = k0 + t + NT * q;
                         if (r > c && r < n) {
                             const T l = x[q][jj] * rinv;
                             x[q][jj] = l;
@@ -361,16 +461,39 @@ __global__ __launch_bounds__(kPT) void k_panel_reg(T* __restrict__ a, int64_t ld
                         }
                     }
                 }
+                lds_sync();  // s_rowp / s_rowc / s_val reused by the next column
             }
         }
         // sub-panel back to memory
 #pragma unroll
         for (int q = 0; q < Q; q++) {
-            const int r = k0 + t + kPT * q;
+            const int r = k0 + t + NT * q;
             if (r < n) {
 #pragma unroll
                 for (int j = 0; j < IB; j++)
                     if (j < ib) a[r + (int64_t)(c0 + j) * lda] = x[q][j];
+            }
+        }
+        // the sub-panel's interchanges on the panel's other columns, as one gather / scatter
+        if (t < 64) {
+            const int np = wave_compose_swaps(c0, ib, s_piv + (c0 - k0), s_pd, s_ps);
+            if (t == 0) s_pcnt = np;
+        }
+        __syncthreads();
+        {
+            // whole columns per pass: every source of a column is read before any of its rows is written
+            const int pairs = s_pcnt;
+            if (pairs > 0) {
+                const int cpi = NT / pairs, cl = t / pairs, pr = t % pairs;
+                for (int cb = 0; cb < kb; cb += cpi) {
+                    const int col = k0 + cb + cl;
+                    const bool act = cl < cpi && cb + cl < kb && (col < c0 || col >= c0 + ib);
+                    T v = T(0);
+                    if (act) v = a[s_ps[pr] + (int64_t)col * lda];
+                    __syncthreads();
+                    if (act) a[s_pd[pr] + (int64_t)col * lda] = v;
+                    __syncthreads();
+                }
             }
         }
         // inverse of the unit-lower diagonal block, for the later sub-panels
@@ -380,7 +503,7 @@ __global__ __launch_bounds__(kPT) void k_panel_reg(T* __restrict__ a, int64_t ld
 #pragma unroll
                 for (int j = 0; j < IB; j++) s_blk[jj][j] = x[0][j];
             }
-            __syncthreads();
+            lds_sync();
             if (t < IB) {
                 const int j = t;
                 T col[IB];
@@ -402,86 +525,80 @@ __global__ __launch_bounds__(kPT) void k_panel_reg(T* __restrict__ a, int64_t ld
         }
         __syncthreads();
     }
-    // (destination, source) rows of the composed interchanges, for the columns outside the panel
-    if (t == 0) {
-        int nout = 0;
-        for (int i = 0; i < kb; i++) s_src[i] = k0 + i;
-        for (int i = 0; i < kb; i++) {
-            const int p = s_piv[i];
-            if (p == k0 + i) continue;
-            int* other;
-            if (p < k0 + kb) {
-                other = &s_src[p - k0];
-            } else {
-                int e = 0;
-                while (e < nout && s_okey[e] != p) e++;
-                if (e == nout) {
-                    s_okey[e] = p;
-                    s_oval[e] = p;
-                    nout++;
-                }
-                other = &s_oval[e];
-            }
-            const int tmp = s_src[i];
-            s_src[i] = *other;
-            *other = tmp;
-        }
-        int cnt = 0;
-        for (int i = 0; i < kb; i++)
-            if (s_src[i] != k0 + i) {
-                perm[1 + 2 * cnt] = k0 + i;
-                perm[2 + 2 * cnt] = s_src[i];
-                cnt++;
-            }
-        for (int e = 0; e < nout; e++)
-            if (s_oval[e] != s_okey[e]) {
-                perm[1 + 2 * cnt] = s_okey[e];
-                perm[2 + 2 * cnt] = s_oval[e];
-                cnt++;
-            }
-        perm[0] = cnt;
+    // pivots, first zero pivot, and the panel's composed interchanges for the rest of the matrix
+    if (t < kb) ipiv[k0 + t] = s_arg[t] + 1;
+    if (t == 0 && s_zero != 0 && info[0] == 0) info[0] = s_zero;
+    if (t < 64) {
+        const int np = wave_compose_swaps(k0, kb, s_piv, s_pd, s_ps);
+        if (t == 0) s_pcnt = np;
     }
+    __syncthreads();
+    const int np = s_pcnt;
+    for (int e = t; e < np; e += NT) {
+        perm[1 + 2 * e] = s_pd[e];
+        perm[2 + 2 * e] = s_ps[e];
+    }
+    if (t == 0) perm[0] = np;
 }
 
-// Row interchanges of one panel on the columns outside it (one wave, one column per lane), then
-// for the columns right of the panel U12 = L11^{-1} A12 with L11 (unit lower) staged in LDS.
+// Row interchanges of one panel on the columns outside it, then for the columns right of the panel
+// U12 = L11^{-1} A12 (L11 unit lower).  A work-group of 256 threads takes 64 columns: the swap pairs
+// are gathered for all columns at once, then scattered; U12's 64 x 64 block and L11 sit in LDS and
+// the forward substitution runs one row per step over all 64 columns.
 template <typename T>
-__global__ __launch_bounds__(64) void k_swap_trsm(T* __restrict__ a, int64_t lda, int n, int k0, int kb, int nleft,
-                                                  const int32_t* __restrict__ perm) {
-    __shared__ T s_v[2 * NB][64];
+__global__ __launch_bounds__(256) void k_swap_trsm(T* __restrict__ a, int64_t lda, int n, int k0, int kb, int nleft,
+                                                   const int32_t* __restrict__ perm) {
+    constexpr int kPer = 2 * NB * 64 / 256;  // gathered elements per thread
+    __shared__ int s_pd[2 * NB], s_ps[2 * NB];
+    __shared__ T s_x[NB][64 + 1];
     __shared__ T s_l[NB][NB + 1];
-    const int lane = threadIdx.x;
+    const int t = threadIdx.x;
     const bool left = (int)blockIdx.x < nleft;
-    const int col = left ? blockIdx.x * 64 + lane : k0 + kb + (blockIdx.x - nleft) * 64 + lane;
-    const bool ok = left ? col < k0 : col < n;
+    const int col0 = left ? blockIdx.x * 64 : k0 + kb + (blockIdx.x - nleft) * 64;
+    const int colend = left ? k0 : n;
     const int cnt = perm[0];
-    T* cc = a + (int64_t)col * lda;
-    if (ok) {
-        for (int e = 0; e < cnt; e++) s_v[e][lane] = cc[perm[2 + 2 * e]];
-        for (int e = 0; e < cnt; e++) cc[perm[1 + 2 * e]] = s_v[e][lane];
+    for (int e = t; e < cnt; e += 256) {
+        s_pd[e] = perm[1 + 2 * e];
+        s_ps[e] = perm[2 + 2 * e];
+    }
+    __syncthreads();
+    T v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const int e = t + 256 * u, pr = e >> 6, col = col0 + (e & 63);
+        v[u] = (pr < cnt && col < colend) ? a[s_ps[pr] + (int64_t)col * lda] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const int e = t + 256 * u, pr = e >> 6, col = col0 + (e & 63);
+        if (pr < cnt && col < colend) a[s_pd[pr] + (int64_t)col * lda] = v[u];
     }
     if (left) return;
-    for (int e = lane; e < kb * kb; e += 64) {
+    __syncthreads();
+    for (int e = t; e < NB * 64; e += 256) {
+        const int r = e % NB, c = e / NB;
+        s_x[r][c] = (r < kb && col0 + c < colend) ? a[(k0 + r) + (int64_t)(col0 + c) * lda] : T(0);
+    }
+    for (int e = t; e < kb * kb; e += 256) {
         const int r = e % kb, c = e / kb;
         s_l[r][c] = a[(k0 + r) + (int64_t)(k0 + c) * lda];
     }
     __syncthreads();
-    if (!ok) return;
-    T xr[NB];
+    const int c = t & 63, g = t >> 6;
+    for (int k = 0; k < kb - 1; k++) {
+        const T xk = s_x[k][c];
 #pragma unroll
-    for (int r = 0; r < NB; r++) xr[r] = r < kb ? cc[k0 + r] : T(0);
-#pragma unroll
-    for (int c = 0; c < NB; c++) {
-        if (c < kb) {
-            const T xc = xr[c];
-#pragma unroll
-            for (int r = c + 1; r < NB; r++)
-                if (r < kb) xr[r] -= s_l[r][c] * xc;
+        for (int m = 0; m < NB / 4; m++) {
+            const int r = g + 4 * m;
+            if (r > k && r < kb) s_x[r][c] -= s_l[r][k] * xk;
         }
+        lds_sync();
     }
-#pragma unroll
-    for (int r = 0; r < NB; r++)
-        if (r < kb) cc[k0 + r] = xr[r];
+    for (int e = t; e < NB * 64; e += 256) {
+        const int r = e % NB, cc = e / NB;
+        if (r < kb && col0 + cc < colend) a[(k0 + r) + (int64_t)(col0 + cc) * lda] = s_x[r][cc];
+    }
 }
 
 // C(m x n) -= A(m x kb) B(kb x n) on the matrix cores: 64 x 64 tile per work-group of 4 waves, each
@@ -566,70 +683,36 @@ __global__ __launch_bounds__(256) void k_update_mfma(T* __restrict__ C, int64_t 
         }
 }
 
-template <typename T, int Q, int IB>
+template <typename T, int NT, int Q, int IB>
 int launch_panel_reg(T* a, int64_t lda, int n, int k0, int kb, int32_t* ipiv, int32_t* info, int32_t* perm,
                      hipStream_t st) {
-    hipLaunchKernelGGL((k_panel_reg<T, Q, IB>), dim3(1), dim3(kPT), 0, st, a, lda, n, k0, kb, ipiv, info, perm);
+    hipLaunchKernelGGL((k_panel_reg<T, NT, Q, IB>), dim3(1), dim3(NT), 0, st, a, lda, n, k0, kb, ipiv, info, perm);
     TVLM_LAUNCHED();
     return TVLM_OK;
 }
 
-// sub-panel width per rows-per-thread Q: Q x IB values per thread stay in registers without spills
+// 256 threads (one wave per SIMD: no wave waits for issue slots, up to 512 registers per lane), rows
+// per thread Q and sub-panel width IB per panel height, chosen so that the sub-panel (Q x IB values
+// per thread) and its L-block buffers stay in registers.
 template <typename T>
 int panel_fast(T* a, int64_t lda, int n, int k0, int kb, int32_t* ipiv, int32_t* info, int32_t* perm, hipStream_t st) {
     const int m = n - k0;
-    constexpr bool D = sizeof(T) == 8;
-    if (m <= kPT) return launch_panel_reg<T, 1, 8>(a, lda, n, k0, kb, ipiv, info, perm, st);
-    if (m <= 2 * kPT) return launch_panel_reg<T, 2, D ? 4 : 8>(a, lda, n, k0, kb, ipiv, info, perm, st);
-    if (m <= 4 * kPT) return launch_panel_reg<T, 4, D ? 4 : 8>(a, lda, n, k0, kb, ipiv, info, perm, st);
-    if (m <= 8 * kPT) return launch_panel_reg<T, 8, D ? 2 : 4>(a, lda, n, k0, kb, ipiv, info, perm, st);
-    return launch_panel_reg<T, 16, D ? 1 : 2>(a, lda, n, k0, kb, ipiv, info, perm, st);
+    if (m <= 256) return launch_panel_reg<T, 256, 1, 8>(a, lda, n, k0, kb, ipiv, info, perm, st);
+    if (m <= 512) return launch_panel_reg<T, 256, 2, 8>(a, lda, n, k0, kb, ipiv, info, perm, st);
+    if (m <= 1024) return launch_panel_reg<T, 256, 4, 8>(a, lda, n, k0, kb, ipiv, info, perm, st);
+    if (m <= 2048) return launch_panel_reg<T, 256, 8, 8>(a, lda, n, k0, kb, ipiv, info, perm, st);
+    if constexpr (sizeof(T) == 8) {
+        if (m <= 4096) return launch_panel_reg<T, 256, 16, 4>(a, lda, n, k0, kb, ipiv, info, perm, st);
+        return launch_panel_reg<T, 256, 32, 2>(a, lda, n, k0, kb, ipiv, info, perm, st);
+    } else {
+        if (m <= 4096) return launch_panel_reg<T, 256, 16, 4>(a, lda, n, k0, kb, ipiv, info, perm, st);
+        return launch_panel_reg<T, 256, 32, 4>(a, lda, n, k0, kb, ipiv, info, perm, st);
+    }
 }
 
-constexpr int kFastPanelRows = 16 * kPT;  // larger matrices take the column-by-column panel
+constexpr int kFastPanelRows = 8192;  // taller panels take the column-by-column panel
 
 // ---- getrs ------------------------------------------------------------------------------------
-constexpr int kSmallSolve = 2048;
-constexpr int kSolveThreads = 1024;
-
-// Whole solve of one right-hand side column per work-group (n <= kSmallSolve).
-template <typename T>
-__global__ __launch_bounds__(kSolveThreads) void k_solve_small(const T* __restrict__ a, int64_t lda, int n,
-                                                               const int32_t* __restrict__ ipiv, T* __restrict__ x,
-                                                               int64_t ldx) {
-    __shared__ T xs[kSmallSolve];
-    T* xc = x + (int64_t)blockIdx.x * ldx;
-    const int t = threadIdx.x;
-    for (int i = t; i < n; i += kSolveThreads) xs[i] = xc[i];
-    __syncthreads();
-    if (t == 0) {
-        for (int k = 0; k < n; k++) {
-            const int p = ipiv[k] - 1;
-            if (p != k) {
-                const T tmp = xs[k];
-                xs[k] = xs[p];
-                xs[p] = tmp;
-            }
-        }
-    }
-    __syncthreads();
-    for (int k = 0; k < n; k++) {  // L y = P b (unit diagonal)
-        const T xk = xs[k];
-        const T* col = a + (int64_t)k * lda;
-        for (int i = k + 1 + t; i < n; i += kSolveThreads) xs[i] -= col[i] * xk;
-        __syncthreads();
-    }
-    for (int k = n - 1; k >= 0; k--) {  // U x = y
-        const T* col = a + (int64_t)k * lda;
-        if (t == 0) xs[k] = xs[k] / col[k];
-        __syncthreads();
-        const T xk = xs[k];
-        for (int i = t; i < k; i += kSolveThreads) xs[i] -= col[i] * xk;
-        __syncthreads();
-    }
-    for (int i = t; i < n; i += kSolveThreads) xc[i] = xs[i];
-}
-
 template <typename T>
 __global__ void k_apply_pivots(T* __restrict__ x, int64_t ldx, int n, const int32_t* __restrict__ ipiv) {
     T* xc = x + (int64_t)blockIdx.x * ldx;
@@ -701,50 +784,195 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     T* a = static_cast<T*>(av->ptr);
     hipLaunchKernelGGL(k_zero_info<T>, dim3(1), dim3(64), 0, st, info);
     TVLM_LAUNCHED();
-    if (n <= kFastPanelRows) {
-        void* ws = nullptr;
-        TVLM_TRY(device_workspace(sizeof(int32_t) * (2 * 2 * NB + 1), &ws, kWsLuPerm));
-        int32_t* perm = static_cast<int32_t*>(ws);
-        for (int k0 = 0; k0 < n; k0 += NB) {
-            const int kb = (n - k0 < NB) ? n - k0 : NB;
-            TVLM_TRY(panel_fast<T>(a, lda, n, k0, kb, ipiv, info, perm, st));
-            const int j1 = k0 + kb;
-            const int nleft = (k0 + 63) / 64, nright = (n - j1 + 63) / 64;
+    void* ws = nullptr;
+    TVLM_TRY(device_workspace(sizeof(int32_t) * (2 * 2 * NB + 1), &ws, kWsLuPerm));
+    int32_t* perm = static_cast<int32_t*>(ws);
+    for (int k0 = 0; k0 < n;) {
+        // tall panels are narrower: the in-panel left-looking traffic grows with width squared
+        const int width = n - k0 > 2048 ? NB / 2 : NB;
+        const int kb = (n - k0 < width) ? n - k0 : width;
+        const int j1 = k0 + kb;
+        k0 = j1;
+        const int kp = j1 - kb;  // this panel's first column
+        if (n - kp <= kFastPanelRows) {
+            TVLM_TRY(panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st));
+            const int nleft = (kp + 63) / 64, nright = (n - j1 + 63) / 64;
             if (nleft + nright > 0) {
-                hipLaunchKernelGGL(k_swap_trsm<T>, dim3(nleft + nright), dim3(64), 0, st, a, lda, n, k0, kb, nleft,
+                hipLaunchKernelGGL(k_swap_trsm<T>, dim3(nleft + nright), dim3(256), 0, st, a, lda, n, kp, kb, nleft,
                                    perm);
                 TVLM_LAUNCHED();
             }
-            if (j1 >= n) continue;
-            const int m = n - j1;
-            hipLaunchKernelGGL(k_update_mfma<T>, dim3((m + 63) / 64, (m + 63) / 64), dim3(256), 0, st,
-                               a + j1 + (int64_t)j1 * lda, lda, a + j1 + (int64_t)k0 * lda, lda,
-                               a + k0 + (int64_t)j1 * lda, lda, m, m, kb);
+        } else {
+            hipLaunchKernelGGL(k_panel<T>, dim3(1), dim3(kPanelThreads), 0, st, a, lda, n, kp, kb, ipiv, info);
             TVLM_LAUNCHED();
+            if (kp > 0) {
+                hipLaunchKernelGGL(k_laswp<T>, dim3((kp + 255) / 256), dim3(256), 0, st, a, lda, 0, kp, kp, kb, ipiv);
+                TVLM_LAUNCHED();
+            }
+            if (j1 < n) {
+                hipLaunchKernelGGL(k_laswp<T>, dim3((n - j1 + 255) / 256), dim3(256), 0, st, a, lda, j1, n, kp, kb,
+                                   ipiv);
+                TVLM_LAUNCHED();
+                hipLaunchKernelGGL(k_trsm<T>, dim3((n - j1 + 255) / 256), dim3(256), 0, st, a, lda, n, kp, kb, j1);
+                TVLM_LAUNCHED();
+            }
         }
-        return TVLM_OK;
-    }
-    for (int k0 = 0; k0 < n; k0 += NB) {
-        const int kb = (n - k0 < NB) ? n - k0 : NB;
-        hipLaunchKernelGGL(k_panel<T>, dim3(1), dim3(kPanelThreads), 0, st, a, lda, n, k0, kb, ipiv, info);
-        TVLM_LAUNCHED();
-        if (k0 > 0) {
-            hipLaunchKernelGGL(k_laswp<T>, dim3((k0 + 255) / 256), dim3(256), 0, st, a, lda, 0, k0, k0, kb, ipiv);
-            TVLM_LAUNCHED();
-        }
-        const int j1 = k0 + kb;
         if (j1 >= n) continue;
-        hipLaunchKernelGGL(k_laswp<T>, dim3((n - j1 + 255) / 256), dim3(256), 0, st, a, lda, j1, n, k0, kb, ipiv);
-        TVLM_LAUNCHED();
-        hipLaunchKernelGGL(k_trsm<T>, dim3((n - j1 + 255) / 256), dim3(256), 0, st, a, lda, n, k0, kb, j1);
-        TVLM_LAUNCHED();
         const int m = n - j1;
-        hipLaunchKernelGGL(k_update<T>, dim3((m + 63) / 64, (m + 63) / 64), dim3(256), 0, st,
-                           a + j1 + (int64_t)j1 * lda, lda, a + j1 + (int64_t)k0 * lda, lda,
-                           a + k0 + (int64_t)j1 * lda, lda, m, m, kb);
+        hipLaunchKernelGGL(k_update_mfma<T>, dim3((m + 63) / 64, (m + 63) / 64), dim3(256), 0, st,
+                           a + j1 + (int64_t)j1 * lda, lda, a + j1 + (int64_t)kp * lda, lda,
+                           a + kp + (int64_t)j1 * lda, lda, m, m, kb);
         TVLM_LAUNCHED();
     }
     return TVLM_OK;
+}
+
+// ---- blocked getrs: interchange lists + diagonal-block inverses + one work-group per rhs -------
+//
+// k_solve_prep, y = 0: composes the interchanges of pivot block b (64 positions) into (dst, src)
+//   pairs (one wave); y = 1 / 2: inverts the 64 x 64 diagonal block b of L (unit lower) / U (upper),
+//   one column per lane.
+// k_solve_blocked: the rhs column lives in LDS; the interchange lists are applied block after
+//   block; then x_b = Linv_b (x_b - L_{b,<b} x_{<b}) for b ascending and x_b = Uinv_b (x_b -
+//   U_{b,>b} x_{>b}) for b descending, each off-diagonal product split over 16 thread groups
+//   reading 64 consecutive rows per column, partial sums reduced in LDS.
+constexpr int kSB = 64;            // solve block
+constexpr int kSolveMaxN = 16384;  // rhs column in LDS
+constexpr int kSolvePairs = 1 + 2 * 2 * kSB;
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_solve_prep(const T* __restrict__ a, int64_t lda, int n,
+                                                   const int32_t* __restrict__ ipiv, int32_t* __restrict__ pairs,
+                                                   T* __restrict__ inv) {
+    __shared__ int s_p[kSB];
+    __shared__ int s_pd[2 * kSB], s_ps[2 * kSB];
+    __shared__ T s_a[kSB][kSB + 1];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const int r0 = b * kSB, rb = min(kSB, n - r0);
+    if (blockIdx.y == 0) {
+        if (lane < rb) s_p[lane] = ipiv[r0 + lane] - 1;
+        __syncthreads();
+        const int np = wave_compose_swaps(r0, rb, s_p, s_pd, s_ps);
+        int32_t* out = pairs + (int64_t)b * kSolvePairs;
+        for (int e = lane; e < np; e += 64) {
+            out[1 + 2 * e] = s_pd[e];
+            out[2 + 2 * e] = s_ps[e];
+        }
+        if (lane == 0) out[0] = np;
+        return;
+    }
+    const bool lower = blockIdx.y == 1;
+    for (int e = lane; e < kSB * kSB; e += 64) {
+        const int r = e % kSB, c = e / kSB;
+        s_a[r][c] = (r < rb && c < rb) ? a[(r0 + r) + (int64_t)(r0 + c) * lda] : (r == c ? T(1) : T(0));
+    }
+    __syncthreads();
+    const int j = lane;
+    T col[kSB];
+    if (lower) {
+#pragma unroll
+        for (int i = 0; i < kSB; i++) {
+            T v = i == j ? T(1) : T(0);
+#pragma unroll
+            for (int p = 0; p < i; p++) v -= (p >= j ? s_a[i][p] * col[p] : T(0));
+            col[i] = i < j ? T(0) : v;
+        }
+    } else {
+#pragma unroll
+        for (int i = kSB - 1; i >= 0; i--) {
+            T v = i == j ? T(1) : T(0);
+#pragma unroll
+            for (int p = i + 1; p < kSB; p++) v -= (p <= j ? s_a[i][p] * col[p] : T(0));
+            col[i] = i > j ? T(0) : v / s_a[i][i];
+        }
+    }
+    T* out = inv + ((int64_t)(lower ? 0 : 1) * ((n + kSB - 1) / kSB) + b) * kSB * kSB;
+#pragma unroll
+    for (int i = 0; i < kSB; i++) out[i + (int64_t)j * kSB] = col[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_solve_blocked(const T* __restrict__ a, int64_t lda, int n,
+                                                        const int32_t* __restrict__ pairs, const T* __restrict__ inv,
+                                                        T* __restrict__ x, int64_t ldx) {
+    __shared__ T xs[kSolveMaxN];
+    __shared__ T red[16][kSB];
+    __shared__ T yb[kSB];
+    const int t = threadIdx.x, r = t & 63, g = t >> 6;
+    const int nblk = (n + kSB - 1) / kSB;
+    T* xc = x + (int64_t)blockIdx.x * ldx;
+    for (int i = t; i < n; i += 1024) xs[i] = xc[i];
+    __syncthreads();
+    for (int b = 0; b < nblk; b++) {  // P b: interchange lists in pivot order
+        const int32_t* pl = pairs + (int64_t)b * kSolvePairs;
+        const int np = pl[0];
+        T v = T(0);
+        if (t < np) v = xs[pl[2 + 2 * t]];
+        __syncthreads();
+        if (t < np) xs[pl[1 + 2 * t]] = v;
+        __syncthreads();
+    }
+    const T* linv = inv;
+    const T* uinv = inv + (int64_t)nblk * kSB * kSB;
+    // L y = P b
+    for (int b = 0; b < nblk; b++) {
+        const int r0 = b * kSB, rb = min(kSB, n - r0);
+        T acc = T(0);
+        if (r < rb)
+            for (int j = g; j < r0; j += 16) acc += a[(r0 + r) + (int64_t)j * lda] * xs[j];
+        red[g][r] = acc;
+        __syncthreads();
+        if (t < kSB) {
+            T y = t < rb ? xs[r0 + t] : T(0);
+#pragma unroll
+            for (int q = 0; q < 16; q++) y -= red[q][t];
+            yb[t] = y;
+        }
+        __syncthreads();
+        T s = T(0);
+        const T* lb = linv + (int64_t)b * kSB * kSB;
+#pragma unroll
+        for (int p = 4 * g; p < 4 * g + 4; p++) s += lb[r + p * kSB] * yb[p];
+        red[g][r] = s;
+        __syncthreads();
+        if (t < rb) {
+            T v = T(0);
+#pragma unroll
+            for (int q = 0; q < 16; q++) v += red[q][t];
+            xs[r0 + t] = v;
+        }
+        __syncthreads();
+    }
+    // U x = y
+    for (int b = nblk - 1; b >= 0; b--) {
+        const int r0 = b * kSB, rb = min(kSB, n - r0);
+        T acc = T(0);
+        if (r < rb)
+            for (int j = r0 + kSB + g; j < n; j += 16) acc += a[(r0 + r) + (int64_t)j * lda] * xs[j];
+        red[g][r] = acc;
+        __syncthreads();
+        if (t < kSB) {
+            T y = t < rb ? xs[r0 + t] : T(0);
+#pragma unroll
+            for (int q = 0; q < 16; q++) y -= red[q][t];
+            yb[t] = y;
+        }
+        __syncthreads();
+        T s = T(0);
+        const T* ub = uinv + (int64_t)b * kSB * kSB;
+#pragma unroll
+        for (int p = 4 * g; p < 4 * g + 4; p++) s += ub[r + p * kSB] * yb[p];
+        red[g][r] = s;
+        __syncthreads();
+        if (t < rb) {
+            T v = T(0);
+#pragma unroll
+            for (int q = 0; q < 16; q++) v += red[q][t];
+            xs[r0 + t] = v;
+        }
+        __syncthreads();
+    }
+    for (int i = t; i < n; i += 1024) xc[i] = xs[i];
 }
 
 template <typename T>
@@ -759,8 +987,16 @@ int getrs(const tvlm_view* av, const int32_t* ipiv, const tvlm_view* xv, void* s
     hipStream_t st = as_stream(stream);
     const T* a = static_cast<const T*>(av->ptr);
     T* x = static_cast<T*>(xv->ptr);
-    if (n <= kSmallSolve) {
-        hipLaunchKernelGGL(k_solve_small<T>, dim3(nrhs), dim3(kSolveThreads), 0, st, a, lda, n, ipiv, x, ldx);
+    if (n <= kSolveMaxN && (int64_t)n * sizeof(T) <= 128 * 1024) {
+        const int nblk = (n + kSB - 1) / kSB;
+        void* ws = nullptr;
+        const size_t inv_bytes = sizeof(T) * 2 * (size_t)nblk * kSB * kSB;
+        TVLM_TRY(device_workspace(inv_bytes + sizeof(int32_t) * (size_t)nblk * kSolvePairs, &ws, kWsSolveInv));
+        T* inv = static_cast<T*>(ws);
+        int32_t* pairs = reinterpret_cast<int32_t*>(static_cast<char*>(ws) + inv_bytes);
+        hipLaunchKernelGGL(k_solve_prep<T>, dim3(nblk, 3), dim3(64), 0, st, a, lda, n, ipiv, pairs, inv);
+        TVLM_LAUNCHED();
+        hipLaunchKernelGGL(k_solve_blocked<T>, dim3(nrhs), dim3(1024), 0, st, a, lda, n, pairs, inv, x, ldx);
         TVLM_LAUNCHED();
         return TVLM_OK;
     }
