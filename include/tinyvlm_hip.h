@@ -122,9 +122,12 @@ int tvlm_fill_f64(int32_t location, double* ptr, int64_t stride, double value, i
                                const tvlm_view* verts_wing, const tvlm_view* verts_wing_init,       \
                                void* stream);                                                       \
     /* LU::factorize (vlm_backend.hpp:105/:112): in-place L\U with row pivots; ipiv_dev{N} is      \
-     * 1-based like LAPACK; info_dev[0] = 0 or k (1-based) of the first exactly zero pivot */       \
+     * 1-based like LAPACK; info_dev[0] = 0 or k (1-based) of the first exactly zero pivot.         \
+     * a->shape[2] = B > 1 factors B independent matrices in one pass (matrix b at                  \
+     * ptr + b*stride[2], pivots at ipiv_dev + b*N, status info_dev[b]). */                         \
     int tvlm_getrf_##S(const tvlm_view* a, int32_t* ipiv_dev, int32_t* info_dev, void* stream);    \
-    /* LU::solve (vlm_backend.hpp:106/:113): x{N,nrhs} <- A^{-1} x ('N') */                         \
+    /* LU::solve (vlm_backend.hpp:106/:113): x{N,nrhs} <- A^{-1} x ('N'); with a batch, x is       \
+     * {N, nrhs, B} and set b is solved with matrix b */                                            \
     int tvlm_getrs_##S(const tvlm_view* a, const int32_t* ipiv_dev, const tvlm_view* x,            \
                        void* stream);                                                               \
     /* LSQ::init/solve (vlm_backend.hpp:121-129; LAPACKE_?gels 'N' in vlm_backend_cpu_lapack.cpp:   \

@@ -432,3 +432,42 @@ def test_error_status_no_launch(H, torch):
     lhs = capi.colmajor((60, 60), torch.float64)
     with pytest.raises(capi.TvlmError, match="status 1"):
         capi.lhs_assemble(lhs, [c], [c], [c], [c], 0)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("n,batch", [(130, 5), (1500, 3)])
+def test_lu_batched(H, torch, prec, n, batch):
+    """A batch of independent matrices in one getrf/getrs pass equals factoring each alone
+    (LAPACK pivots for FP64, per-matrix info, solutions to the single-matrix bar)."""
+    import scipy.linalg as sl
+
+    print(f"[case] lu batched n={n} batch={batch} {prec}", flush=True)
+    dt = DT(torch, prec)
+    rng = np.random.default_rng(1000 + n)
+    As = [rng.standard_normal((n, n)) for _ in range(batch)]
+    As[1][:, 7] = 0.0  # exactly singular member: info = 8 for it only
+    Bs = [rng.standard_normal((n, 2)) for _ in range(batch)]
+    a = H.colmajor((n, n, batch), dt)
+    x = H.colmajor((n, 2, batch), dt)
+    for b in range(batch):
+        a[:, :, b] = H.to_device(As[b], dt)
+        x[:, :, b] = H.to_device(Bs[b], dt)
+    ipiv = torch.zeros(n * batch, dtype=torch.int32, device="cuda")
+    info = torch.full((batch,), -1, dtype=torch.int32, device="cuda")
+    H.getrf(a, ipiv, info)
+    H.getrs(a, ipiv, x)
+    inf = info.cpu().numpy()
+    assert inf[1] == 8 and all(inf[b] == 0 for b in range(batch) if b != 1)
+    piv = ipiv.cpu().numpy().reshape(batch, n)
+    for b in range(batch):
+        if b == 1:
+            continue
+        X = H.to_host(x[:, :, b])
+        res = np.linalg.norm(As[b] @ X - Bs[b]) / (np.linalg.norm(As[b]) * np.linalg.norm(X))
+        if prec == "f64":
+            lu_ref, piv_ref = sl.lu_factor(As[b])
+            assert np.array_equal(piv[b] - 1, piv_ref)
+            assert rel(H.to_host(a[:, :, b]), lu_ref) < 1e-11
+            assert res < 4 * n * np.finfo(np.float64).eps
+        else:
+            assert res < 4 * n * np.finfo(np.float32).eps

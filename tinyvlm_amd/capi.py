@@ -260,7 +260,8 @@ def getrf(a, ipiv, info, stream=None):
 
 
 def getrs(a, ipiv, x, stream=None):
-    x2 = x if x.dim() == 2 else x.unsqueeze(1)
+    """x {N}, {N, nrhs} or, for a batched a {N, N, B}, {N, nrhs, B}."""
+    x2 = x.unsqueeze(1) if x.dim() == 1 else x
     check(_fn("getrs", a)(ctypes.byref(view(a)), ctypes.c_void_p(ipiv.data_ptr()), ctypes.byref(view(x2)),
                           stream_ptr(stream)), "getrs")
 

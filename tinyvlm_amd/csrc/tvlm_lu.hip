@@ -23,10 +23,6 @@ namespace {
 constexpr int kPanelThreads = 1024;
 constexpr int NB = 64;
 
-template <typename T>
-__global__ void k_zero_info(int32_t* info) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) info[0] = 0;
-}
 
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void k_panel(T* __restrict__ a, int64_t lda, int n, int k0, int kb,
@@ -200,6 +196,8 @@ __global__ __launch_bounds__(256) void k_update(T* __restrict__ C, int64_t ldc, 
 // interchanges reach the panel's other columns as one gather / scatter, and at the end thread 0
 // composes the panel's interchanges into (destination, source) row pairs for the rest of the matrix.
 
+constexpr int kPermInts = 1 + 2 * 2 * NB;  // composed interchanges of one panel
+
 // barrier for LDS traffic only: waits for this wave's LDS operations, not for global memory
 __device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
@@ -300,8 +298,12 @@ __device__ int wave_compose_swaps(int first, int cnt, const int* piv, int* pd, i
 template <typename T, int NT, int Q, int IB>
 __global__ __launch_bounds__(NT) void k_panel_reg(T* __restrict__ a, int64_t lda, int n, int k0, int kb,
                                                   int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
-                                                  int32_t* __restrict__ perm) {
+                                                  int32_t* __restrict__ perm, int64_t sb) {
     constexpr int NW = NT / 64;
+    a += (int64_t)blockIdx.z * sb;  // batch: one matrix per blockIdx.z
+    ipiv += (int64_t)blockIdx.z * n;
+    info += blockIdx.z;
+    perm += (int64_t)blockIdx.z * kPermInts;
     __shared__ T s_val[NW];
     __shared__ int s_idx[NW];
     __shared__ T s_rowp[IB], s_rowc[IB];
@@ -547,7 +549,9 @@ __global__ __launch_bounds__(NT) void k_panel_reg(T* __restrict__ a, int64_t lda
 // the forward substitution runs one row per step over all 64 columns.
 template <typename T>
 __global__ __launch_bounds__(256) void k_swap_trsm(T* __restrict__ a, int64_t lda, int n, int k0, int kb, int nleft,
-                                                   const int32_t* __restrict__ perm) {
+                                                   const int32_t* __restrict__ perm, int64_t sb) {
+    a += (int64_t)blockIdx.z * sb;
+    perm += (int64_t)blockIdx.z * kPermInts;
     constexpr int kPer = 2 * NB * 64 / 256;  // gathered elements per thread
     __shared__ int s_pd[2 * NB], s_ps[2 * NB];
     __shared__ T s_x[NB][64 + 1];
@@ -628,8 +632,11 @@ struct MfmaVec<float> {
 template <typename T>
 __global__ __launch_bounds__(256) void k_update_mfma(T* __restrict__ C, int64_t ldc, const T* __restrict__ A,
                                                      int64_t lda, const T* __restrict__ B, int64_t ldb, int m, int n,
-                                                     int kb) {
+                                                     int kb, int64_t sbatch) {
     using V = typename MfmaVec<T>::v4;
+    C += (int64_t)blockIdx.z * sbatch;  // batch: one matrix per blockIdx.z
+    A += (int64_t)blockIdx.z * sbatch;
+    B += (int64_t)blockIdx.z * sbatch;
     __shared__ T sa[NB][64 + 1];
     __shared__ T sb[NB][64 + 1];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -685,8 +692,9 @@ __global__ __launch_bounds__(256) void k_update_mfma(T* __restrict__ C, int64_t 
 
 template <typename T, int NT, int Q, int IB>
 int launch_panel_reg(T* a, int64_t lda, int n, int k0, int kb, int32_t* ipiv, int32_t* info, int32_t* perm,
-                     hipStream_t st) {
-    hipLaunchKernelGGL((k_panel_reg<T, NT, Q, IB>), dim3(1), dim3(NT), 0, st, a, lda, n, k0, kb, ipiv, info, perm);
+                     hipStream_t st, int batch, int64_t sb) {
+    hipLaunchKernelGGL((k_panel_reg<T, NT, Q, IB>), dim3(1, 1, batch), dim3(NT), 0, st, a, lda, n, k0, kb, ipiv, info,
+                       perm, sb);
     TVLM_LAUNCHED();
     return TVLM_OK;
 }
@@ -695,18 +703,19 @@ int launch_panel_reg(T* a, int64_t lda, int n, int k0, int kb, int32_t* ipiv, in
 // per thread Q and sub-panel width IB per panel height, chosen so that the sub-panel (Q x IB values
 // per thread) and its L-block buffers stay in registers.
 template <typename T>
-int panel_fast(T* a, int64_t lda, int n, int k0, int kb, int32_t* ipiv, int32_t* info, int32_t* perm, hipStream_t st) {
+int panel_fast(T* a, int64_t lda, int n, int k0, int kb, int32_t* ipiv, int32_t* info, int32_t* perm, hipStream_t st,
+               int batch, int64_t sb) {
     const int m = n - k0;
-    if (m <= 256) return launch_panel_reg<T, 256, 1, 8>(a, lda, n, k0, kb, ipiv, info, perm, st);
-    if (m <= 512) return launch_panel_reg<T, 256, 2, 8>(a, lda, n, k0, kb, ipiv, info, perm, st);
-    if (m <= 1024) return launch_panel_reg<T, 256, 4, 8>(a, lda, n, k0, kb, ipiv, info, perm, st);
-    if (m <= 2048) return launch_panel_reg<T, 256, 8, 8>(a, lda, n, k0, kb, ipiv, info, perm, st);
+    if (m <= 256) return launch_panel_reg<T, 256, 1, 8>(a, lda, n, k0, kb, ipiv, info, perm, st, batch, sb);
+    if (m <= 512) return launch_panel_reg<T, 256, 2, 8>(a, lda, n, k0, kb, ipiv, info, perm, st, batch, sb);
+    if (m <= 1024) return launch_panel_reg<T, 256, 4, 8>(a, lda, n, k0, kb, ipiv, info, perm, st, batch, sb);
+    if (m <= 2048) return launch_panel_reg<T, 256, 8, 8>(a, lda, n, k0, kb, ipiv, info, perm, st, batch, sb);
     if constexpr (sizeof(T) == 8) {
-        if (m <= 4096) return launch_panel_reg<T, 256, 16, 4>(a, lda, n, k0, kb, ipiv, info, perm, st);
-        return launch_panel_reg<T, 256, 32, 2>(a, lda, n, k0, kb, ipiv, info, perm, st);
+        if (m <= 4096) return launch_panel_reg<T, 256, 16, 4>(a, lda, n, k0, kb, ipiv, info, perm, st, batch, sb);
+        return launch_panel_reg<T, 256, 32, 2>(a, lda, n, k0, kb, ipiv, info, perm, st, batch, sb);
     } else {
-        if (m <= 4096) return launch_panel_reg<T, 256, 16, 4>(a, lda, n, k0, kb, ipiv, info, perm, st);
-        return launch_panel_reg<T, 256, 32, 4>(a, lda, n, k0, kb, ipiv, info, perm, st);
+        if (m <= 4096) return launch_panel_reg<T, 256, 16, 4>(a, lda, n, k0, kb, ipiv, info, perm, st, batch, sb);
+        return launch_panel_reg<T, 256, 32, 4>(a, lda, n, k0, kb, ipiv, info, perm, st, batch, sb);
     }
 }
 
@@ -772,56 +781,71 @@ __global__ void k_off_update(const T* __restrict__ a, int64_t lda, int q0, int q
 }
 
 template <typename T>
+__global__ void k_zero_info_batch(int32_t* info, int batch) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < batch) info[i] = 0;
+}
+
+// shape[2] > 1 factors a batch of matrices, matrix b at ptr + b * stride[2]; ipiv holds n entries
+// per matrix (matrix b at ipiv + b n), info one per matrix.
+template <typename T>
 int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     TVLM_REQUIRE(av && av->ptr && ipiv && info, "null argument");
     TVLM_REQUIRE(av->stride[0] == 1, "A must be column-major");
     TVLM_REQUIRE(av->shape[0] == av->shape[1], "A must be square (got %lld x %lld)", (long long)av->shape[0],
                  (long long)av->shape[1]);
     const int n = (int)av->shape[0];
-    const int64_t lda = av->stride[1];
+    const int batch = (int)av->shape[2];
+    const int64_t lda = av->stride[1], sb = av->stride[2];
     TVLM_REQUIRE(lda >= n || n == 0, "lda < n");
+    TVLM_REQUIRE(batch >= 1 && batch <= 65535, "batch %d out of range", batch);
+    TVLM_REQUIRE(batch == 1 || sb >= lda * n, "matrices of the batch overlap (stride[2] < stride[1] * n)");
     hipStream_t st = as_stream(stream);
     T* a = static_cast<T*>(av->ptr);
-    hipLaunchKernelGGL(k_zero_info<T>, dim3(1), dim3(64), 0, st, info);
+    hipLaunchKernelGGL(k_zero_info_batch<T>, dim3((batch + 255) / 256), dim3(256), 0, st, info, batch);
     TVLM_LAUNCHED();
     void* ws = nullptr;
-    TVLM_TRY(device_workspace(sizeof(int32_t) * (2 * 2 * NB + 1), &ws, kWsLuPerm));
+    TVLM_TRY(device_workspace(sizeof(int32_t) * kPermInts * (size_t)batch, &ws, kWsLuPerm));
     int32_t* perm = static_cast<int32_t*>(ws);
     for (int k0 = 0; k0 < n;) {
         // tall panels are narrower: the in-panel left-looking traffic grows with width squared
         const int width = n - k0 > 2048 ? NB / 2 : NB;
         const int kb = (n - k0 < width) ? n - k0 : width;
         const int j1 = k0 + kb;
+        const int kp = k0;  // this panel's first column
         k0 = j1;
-        const int kp = j1 - kb;  // this panel's first column
         if (n - kp <= kFastPanelRows) {
-            TVLM_TRY(panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st));
+            TVLM_TRY(panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb));
             const int nleft = (kp + 63) / 64, nright = (n - j1 + 63) / 64;
             if (nleft + nright > 0) {
-                hipLaunchKernelGGL(k_swap_trsm<T>, dim3(nleft + nright), dim3(256), 0, st, a, lda, n, kp, kb, nleft,
-                                   perm);
+                hipLaunchKernelGGL(k_swap_trsm<T>, dim3(nleft + nright, 1, batch), dim3(256), 0, st, a, lda, n, kp, kb,
+                                   nleft, perm, sb);
                 TVLM_LAUNCHED();
             }
         } else {
-            hipLaunchKernelGGL(k_panel<T>, dim3(1), dim3(kPanelThreads), 0, st, a, lda, n, kp, kb, ipiv, info);
-            TVLM_LAUNCHED();
-            if (kp > 0) {
-                hipLaunchKernelGGL(k_laswp<T>, dim3((kp + 255) / 256), dim3(256), 0, st, a, lda, 0, kp, kp, kb, ipiv);
+            for (int z = 0; z < batch; z++) {
+                T* az = a + (int64_t)z * sb;
+                int32_t* pz = ipiv + (int64_t)z * n;
+                hipLaunchKernelGGL(k_panel<T>, dim3(1), dim3(kPanelThreads), 0, st, az, lda, n, kp, kb, pz, info + z);
                 TVLM_LAUNCHED();
-            }
-            if (j1 < n) {
-                hipLaunchKernelGGL(k_laswp<T>, dim3((n - j1 + 255) / 256), dim3(256), 0, st, a, lda, j1, n, kp, kb,
-                                   ipiv);
-                TVLM_LAUNCHED();
-                hipLaunchKernelGGL(k_trsm<T>, dim3((n - j1 + 255) / 256), dim3(256), 0, st, a, lda, n, kp, kb, j1);
-                TVLM_LAUNCHED();
+                if (kp > 0) {
+                    hipLaunchKernelGGL(k_laswp<T>, dim3((kp + 255) / 256), dim3(256), 0, st, az, lda, 0, kp, kp, kb, pz);
+                    TVLM_LAUNCHED();
+                }
+                if (j1 < n) {
+                    hipLaunchKernelGGL(k_laswp<T>, dim3((n - j1 + 255) / 256), dim3(256), 0, st, az, lda, j1, n, kp, kb,
+                                       pz);
+                    TVLM_LAUNCHED();
+                    hipLaunchKernelGGL(k_trsm<T>, dim3((n - j1 + 255) / 256), dim3(256), 0, st, az, lda, n, kp, kb, j1);
+                    TVLM_LAUNCHED();
+                }
             }
         }
         if (j1 >= n) continue;
         const int m = n - j1;
-        hipLaunchKernelGGL(k_update_mfma<T>, dim3((m + 63) / 64, (m + 63) / 64), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_update_mfma<T>, dim3((m + 63) / 64, (m + 63) / 64, batch), dim3(256), 0, st,
                            a + j1 + (int64_t)j1 * lda, lda, a + j1 + (int64_t)kp * lda, lda,
-                           a + kp + (int64_t)j1 * lda, lda, m, m, kb);
+                           a + kp + (int64_t)j1 * lda, lda, m, m, kb, sb);
         TVLM_LAUNCHED();
     }
     return TVLM_OK;
@@ -843,8 +867,13 @@ constexpr int kSolvePairs = 1 + 2 * 2 * kSB;
 template <typename T>
 __global__ __launch_bounds__(64) void k_solve_prep(const T* __restrict__ a, int64_t lda, int n,
                                                    const int32_t* __restrict__ ipiv, int32_t* __restrict__ pairs,
-                                                   T* __restrict__ inv) {
+                                                   T* __restrict__ inv, int64_t sa) {
     __shared__ int s_p[kSB];
+    const int nb_ = (n + kSB - 1) / kSB;  // batch: one matrix per blockIdx.z
+    a += (int64_t)blockIdx.z * sa;
+    ipiv += (int64_t)blockIdx.z * n;
+    pairs += (int64_t)blockIdx.z * nb_ * kSolvePairs;
+    inv += (int64_t)blockIdx.z * 2 * nb_ * kSB * kSB;
     __shared__ int s_pd[2 * kSB], s_ps[2 * kSB];
     __shared__ T s_a[kSB][kSB + 1];
     const int b = blockIdx.x, lane = threadIdx.x;
@@ -894,12 +923,16 @@ __global__ __launch_bounds__(64) void k_solve_prep(const T* __restrict__ a, int6
 template <typename T>
 __global__ __launch_bounds__(1024) void k_solve_blocked(const T* __restrict__ a, int64_t lda, int n,
                                                         const int32_t* __restrict__ pairs, const T* __restrict__ inv,
-                                                        T* __restrict__ x, int64_t ldx) {
+                                                        T* __restrict__ x, int64_t ldx, int64_t sa, int64_t sx) {
     __shared__ T xs[kSolveMaxN];
     __shared__ T red[16][kSB];
     __shared__ T yb[kSB];
     const int t = threadIdx.x, r = t & 63, g = t >> 6;
     const int nblk = (n + kSB - 1) / kSB;
+    a += (int64_t)blockIdx.z * sa;  // batch: one matrix per blockIdx.z
+    pairs += (int64_t)blockIdx.z * nblk * kSolvePairs;
+    inv += (int64_t)blockIdx.z * 2 * nblk * kSB * kSB;
+    x += (int64_t)blockIdx.z * sx;
     T* xc = x + (int64_t)blockIdx.x * ldx;
     for (int i = t; i < n; i += 1024) xs[i] = xc[i];
     __syncthreads();
@@ -975,53 +1008,66 @@ __global__ __launch_bounds__(1024) void k_solve_blocked(const T* __restrict__ a,
     for (int i = t; i < n; i += 1024) xc[i] = xs[i];
 }
 
+// shape[2] > 1 solves a batch: matrix b at A.ptr + b A.stride[2] with pivots ipiv + b n, its
+// right-hand sides at X.ptr + b X.stride[2].
 template <typename T>
 int getrs(const tvlm_view* av, const int32_t* ipiv, const tvlm_view* xv, void* stream) {
     TVLM_REQUIRE(av && xv && av->ptr && xv->ptr && ipiv, "null argument");
     TVLM_REQUIRE(av->stride[0] == 1 && xv->stride[0] == 1, "column-major operands required");
     const int n = (int)av->shape[0];
     TVLM_REQUIRE(av->shape[1] == n && xv->shape[0] == n, "shape mismatch");
+    const int batch = (int)av->shape[2];
+    TVLM_REQUIRE(xv->shape[2] == batch, "A has %d matrices, X %lld right-hand-side sets", batch,
+                 (long long)xv->shape[2]);
+    TVLM_REQUIRE(batch >= 1 && batch <= 65535, "batch %d out of range", batch);
     const int nrhs = (int)xv->shape[1];
     if (n == 0 || nrhs == 0) return TVLM_OK;
     const int64_t lda = av->stride[1], ldx = nrhs > 1 ? xv->stride[1] : n;
+    const int64_t sa = av->stride[2], sx = xv->stride[2];
     hipStream_t st = as_stream(stream);
     const T* a = static_cast<const T*>(av->ptr);
     T* x = static_cast<T*>(xv->ptr);
     if (n <= kSolveMaxN && (int64_t)n * sizeof(T) <= 128 * 1024) {
         const int nblk = (n + kSB - 1) / kSB;
         void* ws = nullptr;
-        const size_t inv_bytes = sizeof(T) * 2 * (size_t)nblk * kSB * kSB;
-        TVLM_TRY(device_workspace(inv_bytes + sizeof(int32_t) * (size_t)nblk * kSolvePairs, &ws, kWsSolveInv));
+        const size_t inv_bytes = sizeof(T) * 2 * (size_t)nblk * kSB * kSB * batch;
+        TVLM_TRY(device_workspace(inv_bytes + sizeof(int32_t) * (size_t)nblk * kSolvePairs * batch, &ws, kWsSolveInv));
         T* inv = static_cast<T*>(ws);
         int32_t* pairs = reinterpret_cast<int32_t*>(static_cast<char*>(ws) + inv_bytes);
-        hipLaunchKernelGGL(k_solve_prep<T>, dim3(nblk, 3), dim3(64), 0, st, a, lda, n, ipiv, pairs, inv);
+        hipLaunchKernelGGL(k_solve_prep<T>, dim3(nblk, 3, batch), dim3(64), 0, st, a, lda, n, ipiv, pairs, inv, sa);
         TVLM_LAUNCHED();
-        hipLaunchKernelGGL(k_solve_blocked<T>, dim3(nrhs), dim3(1024), 0, st, a, lda, n, pairs, inv, x, ldx);
+        hipLaunchKernelGGL(k_solve_blocked<T>, dim3(nrhs, 1, batch), dim3(1024), 0, st, a, lda, n, pairs, inv, x, ldx,
+                           sa, sx);
         TVLM_LAUNCHED();
         return TVLM_OK;
     }
-    hipLaunchKernelGGL(k_apply_pivots<T>, dim3(nrhs), dim3(64), 0, st, x, ldx, n, ipiv);
-    TVLM_LAUNCHED();
-    for (int r0 = 0; r0 < n; r0 += SB) {  // forward: L
-        const int nb = (n - r0 < SB) ? n - r0 : SB;
-        hipLaunchKernelGGL((k_diag_solve<T, true>), dim3(nrhs), dim3(SB), 0, st, a, lda, r0, nb, x, ldx);
+    for (int z = 0; z < batch; z++) {
+        const T* az = a + (int64_t)z * sa;
+        const int32_t* pz = ipiv + (int64_t)z * n;
+        T* xz = x + (int64_t)z * sx;
+        hipLaunchKernelGGL(k_apply_pivots<T>, dim3(nrhs), dim3(64), 0, st, xz, ldx, n, pz);
         TVLM_LAUNCHED();
-        if (r0 + nb < n) {
-            hipLaunchKernelGGL(k_off_update<T>, dim3((n - r0 - nb + 255) / 256, nrhs), dim3(256), 0, st, a, lda,
-                               r0 + nb, n, r0, nb, x, ldx);
+        for (int r0 = 0; r0 < n; r0 += SB) {  // forward: L
+            const int nb = (n - r0 < SB) ? n - r0 : SB;
+            hipLaunchKernelGGL((k_diag_solve<T, true>), dim3(nrhs), dim3(SB), 0, st, az, lda, r0, nb, xz, ldx);
             TVLM_LAUNCHED();
+            if (r0 + nb < n) {
+                hipLaunchKernelGGL(k_off_update<T>, dim3((n - r0 - nb + 255) / 256, nrhs), dim3(256), 0, st, az, lda,
+                                   r0 + nb, n, r0, nb, xz, ldx);
+                TVLM_LAUNCHED();
+            }
         }
-    }
-    const int nblk = (n + SB - 1) / SB;
-    for (int b = nblk - 1; b >= 0; b--) {  // backward: U
-        const int r0 = b * SB;
-        const int nb = (n - r0 < SB) ? n - r0 : SB;
-        hipLaunchKernelGGL((k_diag_solve<T, false>), dim3(nrhs), dim3(SB), 0, st, a, lda, r0, nb, x, ldx);
-        TVLM_LAUNCHED();
-        if (r0 > 0) {
-            hipLaunchKernelGGL(k_off_update<T>, dim3((r0 + 255) / 256, nrhs), dim3(256), 0, st, a, lda, 0, r0, r0, nb,
-                               x, ldx);
+        const int nblk = (n + SB - 1) / SB;
+        for (int b = nblk - 1; b >= 0; b--) {  // backward: U
+            const int r0 = b * SB;
+            const int nb = (n - r0 < SB) ? n - r0 : SB;
+            hipLaunchKernelGGL((k_diag_solve<T, false>), dim3(nrhs), dim3(SB), 0, st, az, lda, r0, nb, xz, ldx);
             TVLM_LAUNCHED();
+            if (r0 > 0) {
+                hipLaunchKernelGGL(k_off_update<T>, dim3((r0 + 255) / 256, nrhs), dim3(256), 0, st, az, lda, 0, r0, r0,
+                                   nb, xz, ldx);
+                TVLM_LAUNCHED();
+            }
         }
     }
     return TVLM_OK;
