@@ -199,6 +199,26 @@ int tvlm_uvlm_destroy(tvlm_uvlm_plan* plan);
  * in the plan. */
 int tvlm_uvlm_time_wake(tvlm_uvlm_plan* plan, double* total_ms, int32_t* launches, double* pairs);
 
+/* ---- steady solves of a batch of requests ---------------------------------------------------
+ * The caller of the hot path VLM::run (vlm/src/vlm.cpp:97-151) for `batch` angles of attack on one
+ * mesh, every step on the device.  mode 0: plain FP64, requests one after another (exact FP64
+ * assembly, FP64 LU + substitution).  mode 1: served -- the FP64 influence matrices of all requests
+ * in the seeded form, one batched FP32 LU, FP64 iterative refinement against the FP64 matrix
+ * (at most 8192 panels).  CM is taken about cm_ref moved with the wing (the reference returns 0). */
+typedef struct tvlm_vlm_plan tvlm_vlm_plan;
+/* verts_init[m]: host {ns+1, nc+1, 4} column-major double (w = 1), e.g. a plot3d mesh after the
+ * quarter-chord shift */
+int tvlm_vlm_create(tvlm_vlm_plan** plan, int32_t n_surf, const int64_t* ns, const int64_t* nc,
+                    const double* const* verts_init, int32_t batch, int32_t mode, void* stream);
+/* alpha, beta: host [batch] (radians; beta may be null); cm_ref: host [3] or null = (0.25, 0, 0);
+ * asynchronous on the plan's stream */
+int tvlm_vlm_solve(tvlm_vlm_plan* plan, const double* alpha, const double* beta, double u_inf,
+                   double rho, const double* cm_ref);
+/* synchronises; cl[batch], cm[batch][3], gamma[batch][N], info[batch] (getrf status per request);
+ * null pointers are skipped */
+int tvlm_vlm_results(tvlm_vlm_plan* plan, double* cl, double* cm, double* gamma, int32_t* info);
+int tvlm_vlm_destroy(tvlm_vlm_plan* plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,9 +24,13 @@ int validate_surface3(const tvlm_view& v, int64_t a, int64_t b, const char* what
     return TVLM_OK;
 }
 
+// seeded = true (FP64 only): the FP64 seeded form (hardware 1/sqrt and 1/x estimates + one Newton
+// step) instead of the exact form; used by the served steady solve, which refines against the exact
+// FP64 residual anyway.
 template <typename T>
 int lhs_assemble(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, const tvlm_view* normals,
-                 const tvlm_view* verts_wing, const tvlm_view* verts_wake, int32_t iteration, T sigma, void* stream) {
+                 const tvlm_view* verts_wing, const tvlm_view* verts_wake, int32_t iteration, T sigma, void* stream,
+                 bool seeded = false) {
     TVLM_REQUIRE(lhs && colloc && normals && verts_wing && verts_wake, "null argument");
     TVLM_REQUIRE(n_surf >= 1 && n_surf <= TVLM_MAX_SURFACES, "n_surf %d out of range", n_surf);
     TVLM_REQUIRE(iteration >= 0, "negative iteration");
@@ -81,6 +85,8 @@ int lhs_assemble(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, 
             } else {
                 if (sigma != T(0)) {
                     hipLaunchKernelGGL((k_lhs_block<T, TC, 1>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
+                } else if (seeded) {
+                    hipLaunchKernelGGL((k_lhs_block<T, TC, 3>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
                 } else {
                     hipLaunchKernelGGL((k_lhs_block<T, TC, 0>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
                 }
@@ -167,6 +173,12 @@ int wake_influence(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_
 }
 
 }  // namespace
+
+int lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, const tvlm_view* normals,
+                            const tvlm_view* verts_wing, const tvlm_view* verts_wake, int32_t iteration, void* stream) {
+    return lhs_assemble<double>(lhs, n_surf, colloc, normals, verts_wing, verts_wake, iteration, 0.0, stream, true);
+}
+
 }  // namespace tvlm
 
 extern "C" {

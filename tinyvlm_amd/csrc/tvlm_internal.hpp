@@ -56,6 +56,10 @@ inline bool plane_contiguous(const tvlm_view& v) {
 enum WorkspaceSlot { kWsWakePartials = 0, kWsLuPerm = 1, kWsSolveInv = 2, kWorkspaceSlots = 3 };
 int device_workspace(size_t bytes, void** out, int slot);
 
+// lhs_assemble in the FP64 seeded form (not part of the C ABI; the served steady solve uses it)
+int lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, const tvlm_view* normals,
+                            const tvlm_view* verts_wing, const tvlm_view* verts_wake, int32_t iteration, void* stream);
+
 // ---- small vector type ------------------------------------------------------------------------
 template <typename T>
 struct v3 {

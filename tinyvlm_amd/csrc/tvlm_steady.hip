@@ -1,0 +1,370 @@
+// tvlm_steady.hip -- steady VLM solves of a batch of requests (the caller of the hot path,
+// VLM::run, vlm/src/vlm.cpp:97-151, for many angles of attack at once).
+//
+// A plan owns the device buffers of `batch` requests on one mesh (any number of surfaces).  For
+// request b, solve() repeats VLM::run's sequence: wing reset to the input mesh, wake row shed,
+// displacement 100 chords along -freestream, second shed, panel metrics at alpha_b, influence matrix
+// (iteration 1: the trailing-edge columns also see the one wake row), rhs = -(V_inf . n), solve,
+// circulation differences, steady forces and per-surface CL/CM.
+//   mode 0 (plain FP64): exact FP64 assembly, FP64 LU and substitution, one request after another;
+//                        this is the reference's arithmetic, operation for operation.
+//   mode 1 (served)    : every request's FP64 influence matrix in the seeded form, copied to FP32,
+//                        one batched FP32 LU for all requests, then FP64 iterative refinement
+//                        x += A32^{-1} (b - A64 x) (kRefine steps, residual in FP64 against the FP64
+//                        matrix), so the answer carries the FP64 matrix's accuracy.
+// Per-surface coefficients are combined on the host (area-weighted CL, area*MAC-weighted CM,
+// vlm/src/vlm_backend.cpp:71-139).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "tvlm_internal.hpp"
+
+namespace {
+constexpr int kMaxS = TVLM_MAX_SURFACES;
+constexpr int kRefine = 3;
+constexpr int kResidRows = 256;
+constexpr int kResidMaxN = 8192;  // x in LDS
+}  // namespace
+
+struct tvlm_vlm_plan {
+    int32_t mode = 0, n_surf = 0, N = 0, batch = 0;
+    hipStream_t stream = nullptr;
+    int64_t ns[kMaxS] = {}, nc[kMaxS] = {}, off[kMaxS] = {};  // off: first panel of surface m
+    int64_t vsz[kMaxS] = {}, wsz[kMaxS] = {}, psz[kMaxS] = {};  // elements per request: verts, wake, panels
+    std::vector<void*> allocs;
+    double* init[kMaxS] = {};                                    // {ns+1, nc+1, 4}
+    double *verts[kMaxS] = {}, *wake[kMaxS] = {};                // per request b: + b * vsz / wsz
+    double *colloc[kMaxS] = {}, *normals[kMaxS] = {}, *areas[kMaxS] = {};
+    double *vel[kMaxS] = {}, *forces[kMaxS] = {}, *gdelta[kMaxS] = {};
+    double *lhs = nullptr, *rhs = nullptr, *x = nullptr, *res = nullptr, *tr = nullptr, *coef = nullptr;
+    float *lhs32 = nullptr, *r32 = nullptr;
+    int32_t *ipiv = nullptr, *info = nullptr;
+    double* tr_host = nullptr;  // pinned: transforms of all requests and surfaces
+    std::vector<double> alpha, beta, ref;  // per request; ref: CM reference point in the moved frame
+    double u_inf = 1.0, rho = 1.0;
+    bool solved = false;
+
+    int alloc(void** p, size_t bytes) {
+        TVLM_HIP(hipMalloc(p, bytes));
+        allocs.push_back(*p);
+        return TVLM_OK;
+    }
+};
+
+namespace tvlm {
+namespace {
+
+// r = rhs - A x per request (FP64), one row per thread, x staged in LDS
+__global__ __launch_bounds__(kResidRows) void k_residual(const double* __restrict__ A, int n, const double* __restrict__ rhs,
+                                                         const double* __restrict__ x, double* __restrict__ r) {
+    __shared__ double xs[kResidMaxN];
+    const int64_t b = blockIdx.y;
+    A += b * (int64_t)n * n;
+    rhs += b * n;
+    x += b * n;
+    r += b * n;
+    for (int i = threadIdx.x; i < n; i += kResidRows) xs[i] = x[i];
+    __syncthreads();
+    const int i = blockIdx.x * kResidRows + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int j = 0; j < n; j++) acc += A[i + (int64_t)j * n] * xs[j];
+    r[i] = rhs[i] - acc;
+}
+
+__global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst, int64_t count) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
+__global__ void k_add_f32(double* __restrict__ x, const float* __restrict__ d, int64_t count) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] += (double)d[i];
+}
+
+// gamma_delta(i, j) = gamma(i, j) - gamma(i, j-1), gamma_delta(i, 0) = gamma(i, 0) (vlm.cpp:126-139)
+__global__ void k_gamma_delta(const double* __restrict__ g, double* __restrict__ gd, int ns, int nc) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ns * nc) return;
+    gd[p] = p >= ns ? g[p] - g[p - ns] : g[p];
+}
+
+int grid_for(int64_t n) { return (int)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096); }
+
+void views(tvlm_vlm_plan* P, int b, int m, tvlm_view& v, tvlm_view& w, tvlm_view& c, tvlm_view& nn, tvlm_view& a,
+           tvlm_view& vel, tvlm_view& f, tvlm_view& gd) {
+    const int64_t ns = P->ns[m], nc = P->nc[m];
+    v = {P->verts[m] + b * P->vsz[m], {ns + 1, nc + 1, 4}, {1, ns + 1, (ns + 1) * (nc + 1)}};
+    w = {P->wake[m] + b * P->wsz[m], {ns + 1, 2, 4}, {1, ns + 1, (ns + 1) * 2}};
+    c = {P->colloc[m] + b * 3 * P->psz[m], {ns, nc, 3}, {1, ns, ns * nc}};
+    nn = {P->normals[m] + b * 3 * P->psz[m], {ns, nc, 3}, {1, ns, ns * nc}};
+    a = {P->areas[m] + b * P->psz[m], {ns, nc, 1}, {1, ns, ns * nc}};
+    vel = {P->vel[m] + b * 3 * P->psz[m], {ns, nc, 3}, {1, ns, ns * nc}};
+    f = {P->forces[m] + b * 3 * P->psz[m], {ns, nc, 3}, {1, ns, ns * nc}};
+    gd = {P->gdelta[m] + b * P->psz[m], {ns, nc, 1}, {1, ns, ns * nc}};
+}
+
+// geometry, influence matrix and rhs of request b (vlm.cpp:97-124)
+int assemble(tvlm_vlm_plan* P, int b, bool seeded) {
+    hipStream_t st = P->stream;
+    const int n = P->n_surf;
+    std::vector<tvlm_view> V(n), W(n), C(n), Nn(n), A(n), VEL(n), F(n), GD(n), TR(n), INIT(n);
+    for (int m = 0; m < n; m++) {
+        views(P, b, m, V[m], W[m], C[m], Nn[m], A[m], VEL[m], F[m], GD[m]);
+        const int64_t ns = P->ns[m], nc = P->nc[m];
+        INIT[m] = {P->init[m], {ns + 1, nc + 1, 4}, {1, ns + 1, (ns + 1) * (nc + 1)}};
+        TR[m] = {P->tr + ((int64_t)b * n + m) * 16, {4, 4, 1}, {1, 4, 16}};
+        TVLM_HIP(hipMemcpyAsync(V[m].ptr, P->init[m], sizeof(double) * P->vsz[m], hipMemcpyDeviceToDevice, st));
+    }
+    TVLM_TRY(tvlm_wake_shed_f64(n, V.data(), W.data(), 0, st));
+    TVLM_TRY(tvlm_displace_wing_f64(n, TR.data(), V.data(), INIT.data(), st));
+    TVLM_TRY(tvlm_wake_shed_f64(n, V.data(), W.data(), 1, st));
+    TVLM_TRY(tvlm_mesh_metrics_f64(P->alpha[b], n, V.data(), C.data(), Nn.data(), A.data(), st));
+    const int64_t NN = (int64_t)P->N * P->N;
+    double* lhs = P->lhs + (P->mode == 1 ? b * NN : 0);
+    double* rhs = P->rhs + (int64_t)b * P->N;
+    TVLM_TRY(tvlm_fill_f64(0, lhs, 1, 0.0, NN, st));
+    TVLM_TRY(tvlm_fill_f64(0, rhs, 1, 0.0, P->N, st));
+    tvlm_view L = {lhs, {P->N, P->N, 1}, {1, P->N, NN}};
+    if (seeded)
+        TVLM_TRY(lhs_assemble_seeded_f64(&L, n, C.data(), Nn.data(), V.data(), W.data(), 1, st));
+    else
+        TVLM_TRY(tvlm_lhs_assemble_f64(&L, n, C.data(), Nn.data(), V.data(), W.data(), 1, 0.0, st));
+    const double a = P->alpha[b], be = P->beta[b], u = P->u_inf;
+    const double fs[3] = {u * std::cos(a) * std::cos(be), -u * std::cos(a) * std::sin(be), u * std::sin(a)};
+    for (int m = 0; m < n; m++)
+        for (int k = 0; k < 3; k++)
+            TVLM_TRY(tvlm_fill_f64(0, static_cast<double*>(VEL[m].ptr) + k * P->psz[m], 1, fs[k], P->psz[m], st));
+    tvlm_view R = {rhs, {P->N, 1, 1}, {1, P->N, P->N}};
+    TVLM_TRY(tvlm_rhs_assemble_velocities_f64(&R, n, Nn.data(), VEL.data(), st));
+    return TVLM_OK;
+}
+
+// circulation differences, forces and per-surface coefficients of request b (vlm.cpp:126-151);
+// coef[(b n_surf + m) * 8 + {0: area, 1: mac, 2: cl, 3..5: cm}]
+int post(tvlm_vlm_plan* P, int b) {
+    hipStream_t st = P->stream;
+    const int n = P->n_surf;
+    const double a = P->alpha[b], be = P->beta[b], u = P->u_inf;
+    const double fs[3] = {u * std::cos(a) * std::cos(be), -u * std::cos(a) * std::sin(be), u * std::sin(a)};
+    // lift axis normalize(fs x y) (vlm/src/vlm_utils.cpp:13-15)
+    double la[3] = {-fs[2], 0.0, fs[0]};
+    const double ln = std::sqrt(la[0] * la[0] + la[2] * la[2]);
+    la[0] /= ln;
+    la[2] /= ln;
+    const double* ref = P->ref.data() + 3 * b;
+    for (int m = 0; m < n; m++) {
+        tvlm_view V, W, C, Nn, A, VEL, F, GD;
+        views(P, b, m, V, W, C, Nn, A, VEL, F, GD);
+        const double* g = P->x + (int64_t)b * P->N + P->off[m];
+        hipLaunchKernelGGL(k_gamma_delta, dim3(grid_for(P->psz[m])), dim3(256), 0, st, g,
+                           static_cast<double*>(GD.ptr), (int)P->ns[m], (int)P->nc[m]);
+        TVLM_LAUNCHED();
+        TVLM_TRY(tvlm_forces_steady_f64(&V, &GD, &VEL, &F, st));
+        double* cf = P->coef + ((int64_t)b * n + m) * 8;
+        tvlm_view A2 = {A.ptr, {P->ns[m], P->nc[m], 1}, {1, P->ns[m], P->psz[m]}};
+        TVLM_TRY(tvlm_sum_f64(&A2, 2, cf + 0, st));
+        TVLM_TRY(tvlm_mesh_mac_f64(&V, &A2, cf + 1, st));
+        // area and MAC enter only the normalisation: CL and CM are formed on the host from these sums
+        TVLM_TRY(tvlm_coeff_cl_f64(&F, la, fs, P->rho, 1.0, cf + 2, st));
+        TVLM_TRY(tvlm_coeff_cm_f64(&F, &V, ref, fs, P->rho, 1.0, 1.0, cf + 3, st));
+    }
+    return TVLM_OK;
+}
+
+}  // namespace
+}  // namespace tvlm
+
+using namespace tvlm;
+
+extern "C" {
+
+int tvlm_vlm_create(tvlm_vlm_plan** plan, int32_t n_surf, const int64_t* ns, const int64_t* nc,
+                    const double* const* verts_init, int32_t batch, int32_t mode, void* stream) {
+    TVLM_REQUIRE(plan && ns && nc && verts_init, "null argument");
+    TVLM_REQUIRE(n_surf >= 1 && n_surf <= kMaxS, "n_surf %d out of range", n_surf);
+    TVLM_REQUIRE(batch >= 1 && batch <= 4096, "batch %d out of range", batch);
+    TVLM_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (plain FP64) or 1 (served)");
+    auto* P = new tvlm_vlm_plan();
+    P->mode = mode;
+    P->n_surf = n_surf;
+    P->batch = batch;
+    P->stream = as_stream(stream);
+    int64_t N = 0;
+    for (int m = 0; m < n_surf; m++) {
+        if (ns[m] < 1 || nc[m] < 1) {
+            delete P;
+            set_error("surface %d: empty mesh", m);
+            return TVLM_ERR_INVALID;
+        }
+        P->ns[m] = ns[m];
+        P->nc[m] = nc[m];
+        P->off[m] = N;
+        P->vsz[m] = (ns[m] + 1) * (nc[m] + 1) * 4;
+        P->wsz[m] = (ns[m] + 1) * 2 * 4;
+        P->psz[m] = ns[m] * nc[m];
+        N += ns[m] * nc[m];
+    }
+    if (mode == 1 && N > kResidMaxN) {
+        delete P;
+        set_error("served steady solve supports at most %d panels (got %lld)", kResidMaxN, (long long)N);
+        return TVLM_ERR_UNSUPPORTED;
+    }
+    P->N = (int32_t)N;
+    const int64_t NN = N * N;
+    const int64_t nb = mode == 1 ? batch : 1;  // plain mode reuses one matrix for every request
+    auto fail = [&](int rc) {
+        for (void* p : P->allocs) (void)hipFree(p);
+        if (P->tr_host) (void)hipHostFree(P->tr_host);
+        delete P;
+        return rc;
+    };
+    int rc = TVLM_OK;
+    for (int m = 0; m < n_surf && rc == TVLM_OK; m++) {
+        const size_t v = sizeof(double) * P->vsz[m], p3 = sizeof(double) * 3 * P->psz[m], p1 = sizeof(double) * P->psz[m];
+        rc = P->alloc((void**)&P->init[m], v);
+        if (rc == TVLM_OK) rc = P->alloc((void**)&P->verts[m], v * batch);
+        if (rc == TVLM_OK) rc = P->alloc((void**)&P->wake[m], sizeof(double) * P->wsz[m] * batch);
+        if (rc == TVLM_OK) rc = P->alloc((void**)&P->colloc[m], p3 * batch);
+        if (rc == TVLM_OK) rc = P->alloc((void**)&P->normals[m], p3 * batch);
+        if (rc == TVLM_OK) rc = P->alloc((void**)&P->areas[m], p1 * batch);
+        if (rc == TVLM_OK) rc = P->alloc((void**)&P->vel[m], p3 * batch);
+        if (rc == TVLM_OK) rc = P->alloc((void**)&P->forces[m], p3 * batch);
+        if (rc == TVLM_OK) rc = P->alloc((void**)&P->gdelta[m], p1 * batch);
+        if (rc == TVLM_OK) rc = tvlm_copy(0, P->init[m], 1, 1, verts_init[m], 1, sizeof(double), P->vsz[m], stream);
+    }
+    if (rc == TVLM_OK) rc = P->alloc((void**)&P->lhs, sizeof(double) * NN * nb);
+    if (rc == TVLM_OK && mode == 1) rc = P->alloc((void**)&P->lhs32, sizeof(float) * NN * nb);
+    if (rc == TVLM_OK && mode == 1) rc = P->alloc((void**)&P->r32, sizeof(float) * N * batch);
+    if (rc == TVLM_OK && mode == 1) rc = P->alloc((void**)&P->res, sizeof(double) * N * batch);
+    if (rc == TVLM_OK) rc = P->alloc((void**)&P->rhs, sizeof(double) * N * batch);
+    if (rc == TVLM_OK) rc = P->alloc((void**)&P->x, sizeof(double) * N * batch);
+    if (rc == TVLM_OK) rc = P->alloc((void**)&P->tr, sizeof(double) * 16 * n_surf * batch);
+    if (rc == TVLM_OK) rc = P->alloc((void**)&P->coef, sizeof(double) * 8 * n_surf * batch);
+    if (rc == TVLM_OK) rc = P->alloc((void**)&P->ipiv, sizeof(int32_t) * N * nb);
+    if (rc == TVLM_OK) rc = P->alloc((void**)&P->info, sizeof(int32_t) * batch);
+    if (rc == TVLM_OK) {
+        hipError_t e = hipHostMalloc((void**)&P->tr_host, sizeof(double) * 16 * n_surf * batch, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            set_error("hipHostMalloc failed: %s", hipGetErrorString(e));
+            rc = TVLM_ERR_NOMEM;
+        }
+    }
+    if (rc == TVLM_OK) {
+        // the wake rows' w coordinate is 1 and stays so (vlm.cpp:94)
+        for (int m = 0; m < n_surf && rc == TVLM_OK; m++)
+            for (int b = 0; b < batch && rc == TVLM_OK; b++)
+                rc = tvlm_fill_f64(0, P->wake[m] + b * P->wsz[m] + 3 * (P->ns[m] + 1) * 2, 1, 1.0, (P->ns[m] + 1) * 2,
+                                   stream);
+    }
+    if (rc != TVLM_OK) return fail(rc);
+    P->alpha.assign(batch, 0.0);
+    P->beta.assign(batch, 0.0);
+    P->ref.assign(3 * batch, 0.0);
+    *plan = P;
+    return TVLM_OK;
+}
+
+int tvlm_vlm_solve(tvlm_vlm_plan* P, const double* alpha, const double* beta, double u_inf, double rho,
+                   const double* cm_ref) {
+    TVLM_REQUIRE(P && alpha, "null argument");
+    const int n = P->n_surf, B = P->batch;
+    P->u_inf = u_inf;
+    P->rho = rho;
+    const double c0[3] = {cm_ref ? cm_ref[0] : 0.25, cm_ref ? cm_ref[1] : 0.0, cm_ref ? cm_ref[2] : 0.0};
+    TVLM_HIP(hipStreamSynchronize(P->stream));  // the pinned transform buffer may still be read
+    for (int b = 0; b < B; b++) {
+        P->alpha[b] = alpha[b];
+        P->beta[b] = beta ? beta[b] : 0.0;
+        // translation by -100 u (cos a, 0, sin a): 100-chord wake row (vlm.cpp:104-108)
+        const double tx = -100.0 * u_inf * std::cos(alpha[b]), tz = -100.0 * u_inf * std::sin(alpha[b]);
+        for (int m = 0; m < n; m++) {
+            double* T = P->tr_host + ((int64_t)b * n + m) * 16;
+            for (int k = 0; k < 16; k++) T[k] = (k % 5 == 0) ? 1.0 : 0.0;
+            T[12] = tx;
+            T[14] = tz;
+        }
+        P->ref[3 * b + 0] = c0[0] + tx;
+        P->ref[3 * b + 1] = c0[1];
+        P->ref[3 * b + 2] = c0[2] + tz;
+    }
+    hipStream_t st = P->stream;
+    TVLM_HIP(hipMemcpyAsync(P->tr, P->tr_host, sizeof(double) * 16 * n * B, hipMemcpyHostToDevice, st));
+    const int64_t N = P->N, NN = N * N;
+    if (P->mode == 0) {
+        tvlm_view L = {P->lhs, {N, N, 1}, {1, N, NN}};
+        for (int b = 0; b < B; b++) {
+            TVLM_TRY(assemble(P, b, false));
+            TVLM_TRY(tvlm_getrf_f64(&L, P->ipiv, P->info + b, st));
+            TVLM_HIP(hipMemcpyAsync(P->x + b * N, P->rhs + b * N, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            tvlm_view X = {P->x + b * N, {N, 1, 1}, {1, N, N}};
+            TVLM_TRY(tvlm_getrs_f64(&L, P->ipiv, &X, st));
+            TVLM_TRY(post(P, b));
+        }
+    } else {
+        for (int b = 0; b < B; b++) TVLM_TRY(assemble(P, b, true));
+        hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, st, P->lhs, P->lhs32, NN * B);
+        TVLM_LAUNCHED();
+        tvlm_view L32 = {P->lhs32, {N, N, B}, {1, N, NN}};
+        TVLM_TRY(tvlm_getrf_f32(&L32, P->ipiv, P->info, st));
+        tvlm_view R32 = {P->r32, {N, 1, B}, {1, N, N}};
+        // x = 0, r = b; then kRefine + 1 corrections x += A32^{-1} r, r = b - A64 x
+        TVLM_TRY(tvlm_fill_f64(0, P->x, 1, 0.0, N * B, st));
+        for (int it = 0; it <= kRefine; it++) {
+            if (it == 0) {
+                hipLaunchKernelGGL(k_to_f32, dim3(grid_for(N * B)), dim3(256), 0, st, P->rhs, P->r32, N * B);
+            } else {
+                hipLaunchKernelGGL(k_residual, dim3((unsigned)((N + kResidRows - 1) / kResidRows), B), dim3(kResidRows),
+                                   0, st, P->lhs, (int)N, P->rhs, P->x, P->res);
+                TVLM_LAUNCHED();
+                hipLaunchKernelGGL(k_to_f32, dim3(grid_for(N * B)), dim3(256), 0, st, P->res, P->r32, N * B);
+            }
+            TVLM_LAUNCHED();
+            TVLM_TRY(tvlm_getrs_f32(&L32, P->ipiv, &R32, st));
+            hipLaunchKernelGGL(k_add_f32, dim3(grid_for(N * B)), dim3(256), 0, st, P->x, P->r32, N * B);
+            TVLM_LAUNCHED();
+        }
+        for (int b = 0; b < B; b++) TVLM_TRY(post(P, b));
+    }
+    P->solved = true;
+    return TVLM_OK;
+}
+
+// Synchronises.  cl[batch], cm[batch][3], gamma[batch][N], info[batch]; null pointers are skipped.
+int tvlm_vlm_results(tvlm_vlm_plan* P, double* cl, double* cm, double* gamma, int32_t* info) {
+    TVLM_REQUIRE(P && P->solved, "no solve has run on this plan");
+    TVLM_HIP(hipStreamSynchronize(P->stream));
+    const int n = P->n_surf, B = P->batch;
+    std::vector<double> coef((size_t)8 * n * B);
+    TVLM_HIP(hipMemcpy(coef.data(), P->coef, sizeof(double) * coef.size(), hipMemcpyDeviceToHost));
+    for (int b = 0; b < B; b++) {
+        // per surface: cf[2] = sum(F . l) / q and cf[3..5] = sum(r x F) / q (q = rho |V|^2 / 2,
+        // vlm_backend_cpu.cpp:323-362); CL area-weighted, CM area*MAC-weighted over the surfaces
+        double cl_sum = 0.0, area_sum = 0.0, mac_sum = 0.0, cm_sum[3] = {0.0, 0.0, 0.0};
+        for (int m = 0; m < n; m++) {
+            const double* cf = coef.data() + ((size_t)b * n + m) * 8;
+            cl_sum += cf[2];
+            for (int k = 0; k < 3; k++) cm_sum[k] += cf[3 + k];
+            area_sum += cf[0];
+            mac_sum += cf[1];
+        }
+        if (cl) cl[b] = cl_sum / area_sum;
+        if (cm)
+            for (int k = 0; k < 3; k++) cm[3 * b + k] = cm_sum[k] / (area_sum * mac_sum);
+    }
+    if (gamma) TVLM_HIP(hipMemcpy(gamma, P->x, sizeof(double) * P->N * B, hipMemcpyDeviceToHost));
+    if (info) TVLM_HIP(hipMemcpy(info, P->info, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    return TVLM_OK;
+}
+
+int tvlm_vlm_destroy(tvlm_vlm_plan* P) {
+    if (!P) return TVLM_OK;
+    (void)hipStreamSynchronize(P->stream);
+    for (void* p : P->allocs) (void)hipFree(p);
+    if (P->tr_host) (void)hipHostFree(P->tr_host);
+    delete P;
+    return TVLM_OK;
+}
+
+}  // extern "C"
