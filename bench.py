@@ -20,6 +20,12 @@ receiver x wake-ring pair -- the ring-by-ring count of the reference algorithm -
 events on the plan's stream, against the FP64 VALU peak since it computes in FP64) and the CPU
 baseline (the oracle's FP64 restatement of the reference CPU backend, OpenMP on the host cores).
 
+Also reported under "steady": the steady configuration -- elliptic wing mesh/elliptic_64x64.x
+(N = 4096), u_inf = 1, rho = 1 -- as served batches of 64 requests (alpha = 5 deg and 63 more drawn
+from U[-5, 10] deg, seed 0: seeded FP64 influence matrices, one batched FP32 LU, FP64 refinement)
+against the plain FP64 path (one request at a time, exact FP64 assembly, FP64 LU), with |dCL| between
+them on the same angles, per-batch latency, and one CPU solve of the oracle.
+
 --probe runs one served simulation without graph replay and one 1 GiB device copy (the
 calibration kernel for HBM byte counters), for profiler counter passes.
 """
@@ -103,6 +109,72 @@ def cpu_baseline(verts, budget_steps):
     return budget_steps / el, el, C.lib().orc_num_threads(), cl
 
 
+STEADY_MESH = os.path.join(ROOT, "mesh", "elliptic_64x64.x")
+
+
+def steady_alphas(batch):
+    """alpha = 5 deg first (the steady configuration), then U[-5, 10] deg, PCG64 seed 0; float32
+    degrees -> radians as the reference's to_radians (vlm/include/vlm_utils.hpp:68-70)."""
+    deg = np.random.default_rng(0).uniform(-5.0, 10.0, batch).astype(np.float32)
+    deg[0] = 5.0
+    return [float(np.float32(d) * np.float32(3.141593) / np.float32(180)) for d in deg]
+
+
+def steady_bench(batch, reps, plain_n, with_cpu):
+    from tinyvlm_amd.mesh_io import read_plot3d
+    from tinyvlm_amd.steady_plan import MODE_PLAIN_FP64, MODE_SERVED, SteadyPlan
+
+    verts = read_plot3d(STEADY_MESH)
+    alphas = steady_alphas(batch)
+    served = SteadyPlan([verts], batch, mode=MODE_SERVED)
+    served.solve(alphas)
+    served.results(gamma=False)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        t1 = time.perf_counter()
+        served.solve(alphas)
+        res_s = served.results(gamma=True)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    wall_s = time.perf_counter() - t0
+    served.close()
+    plain = SteadyPlan([verts], plain_n, mode=MODE_PLAIN_FP64)
+    plain.solve(alphas[:plain_n])
+    plain.results(gamma=False)
+    t0 = time.perf_counter()
+    plain.solve(alphas[:plain_n])
+    res_p = plain.results(gamma=True)
+    wall_p = time.perf_counter() - t0
+    plain.close()
+    dcl = float(np.max(np.abs(res_s["cl"][:plain_n] - res_p["cl"])))
+    dg = float(np.max(np.abs(res_s["gamma"][:plain_n] - res_p["gamma"])) / np.max(np.abs(res_p["gamma"])))
+    out = {
+        "workload": "steady_elliptic_64x64_N4096",
+        "mesh": "mesh/elliptic_64x64.x", "panels": int(verts.shape[0] - 1) * int(verts.shape[1] - 1),
+        "served": {"value": round(batch * reps / wall_s, 2), "unit": "steady solves/s", "batch": batch,
+                   "batches": reps, "latency_ms_per_batch": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                                             "p99": round(float(np.percentile(lat, 99)), 3)}},
+        "plain_fp64": {"value": round(plain_n / wall_p, 2), "unit": "steady solves/s", "solves": plain_n,
+                       "latency_ms_per_solve": round(wall_p / plain_n * 1e3, 3)},
+        "speedup_of_served_path": round((batch * reps / wall_s) / (plain_n / wall_p), 3),
+        "accuracy": {"max_abs_dCL_vs_plain_fp64": dcl, "max_rel_dgamma_vs_plain_fp64": dg, "bar": 1e-6,
+                     "accepted": bool(dcl <= 1e-6)},
+        "cl_alpha5": float(res_s["cl"][0]),
+    }
+    if with_cpu:
+        from oracle import cpuref as C
+
+        v = C.read_mesh(STEADY_MESH)
+        C.vlm_run([C.read_mesh(os.path.join(ROOT, "mesh", "rectangular_4x4.x"))], alphas[0])  # warm-up
+        t0 = time.perf_counter()
+        _, cl_cpu, _, _ = C.vlm_run([v], alphas[0])
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(1.0 / el, 4), "unit": "steady solves/s", "cores": C.lib().orc_num_threads(),
+                               "kind": "port", "sample": f"one oracle FP64 VLM::run at alpha 5 deg ({el:.2f} s)",
+                               "abs_dCL_plain_gpu_vs_cpu": abs(float(res_p["cl"][0]) - cl_cpu)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +184,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=500, help="UVLM steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true", help="one served simulation without graph + a 1 GiB copy")
+    ap.add_argument("--steady-batch", type=int, default=64, help="requests per served steady batch")
+    ap.add_argument("--steady-reps", type=int, default=3, help="timed served steady batches")
+    ap.add_argument("--steady-plain", type=int, default=4, help="timed plain FP64 steady solves")
+    ap.add_argument("--no-steady", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -223,6 +299,8 @@ def main():
         out["cpu_baseline"] = None
     fast.close()
     plain.close()
+    if not args.no_steady:
+        out["steady"] = steady_bench(args.steady_batch, args.steady_reps, args.steady_plain, not args.no_cpu)
     print(json.dumps(out))
 
 

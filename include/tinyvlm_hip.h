@@ -199,6 +199,12 @@ int tvlm_uvlm_destroy(tvlm_uvlm_plan* plan);
  * in the plan. */
 int tvlm_uvlm_time_wake(tvlm_uvlm_plan* plan, double* total_ms, int32_t* launches, double* pairs);
 
+/* ---- plot3d meshes (MeshIO, vlm/src/vlm_mesh.cpp:34-124); host only, no device needed ----------
+ * dims: ns spanwise and nc chordwise panels.  read: verts {ns+1, nc+1, 4} column-major double,
+ * parsed and (quarter_chord != 0) quarter-chord shifted in FP32 as the reference drivers do, w = 1 */
+int tvlm_plot3d_dims(const char* path, int64_t* ns, int64_t* nc);
+int tvlm_plot3d_read(const char* path, int32_t quarter_chord, double* verts);
+
 /* ---- steady solves of a batch of requests ---------------------------------------------------
  * The caller of the hot path VLM::run (vlm/src/vlm.cpp:97-151) for `batch` angles of attack on one
  * mesh, every step on the device.  mode 0: plain FP64, requests one after another (exact FP64
