@@ -18,7 +18,7 @@
  *    norms) are written to DEVICE memory (`*_dev` arguments) so that callers can keep whole
  *    solver loops on the device; the C++ adapter (tinyvlm_amd/vlm) turns them back into the
  *    reference's by-value returns.
- *  - Outputs the reference accumulates into (lhs, rhs, forces_unsteady) are accumulated here too;
+ *  - Outputs the reference accumulates into (lhs, rhs) are accumulated here too;
  *    the caller zero-fills them, as the reference drivers do (vlm/src/vlm.cpp:99-100, :437).
  *  - Return value: TVLM_OK or an error code; tvlm_last_error() gives the message.
  */
@@ -200,10 +200,11 @@ int tvlm_uvlm_destroy(tvlm_uvlm_plan* plan);
 int tvlm_uvlm_time_wake(tvlm_uvlm_plan* plan, double* total_ms, int32_t* launches, double* pairs);
 
 /* ---- plot3d meshes (MeshIO, vlm/src/vlm_mesh.cpp:34-124); host only, no device needed ----------
- * dims: ns spanwise and nc chordwise panels.  read: verts {ns+1, nc+1, 4} column-major double,
- * parsed and (quarter_chord != 0) quarter-chord shifted in FP32 as the reference drivers do, w = 1 */
+ * dims: ns spanwise and nc chordwise panels.  read: verts {ns+1, nc+1, 4} column-major double, w = 1,
+ * parsed in FP32 (parse_f64 = 0: the VLM/UVLM drivers' buffers) or FP64 (the HB driver's) and, if
+ * quarter_chord != 0, quarter-chord shifted in that precision */
 int tvlm_plot3d_dims(const char* path, int64_t* ns, int64_t* nc);
-int tvlm_plot3d_read(const char* path, int32_t quarter_chord, double* verts);
+int tvlm_plot3d_read(const char* path, int32_t quarter_chord, int32_t parse_f64, double* verts);
 
 /* ---- steady solves of a batch of requests ---------------------------------------------------
  * The caller of the hot path VLM::run (vlm/src/vlm.cpp:97-151) for `batch` angles of attack on one
