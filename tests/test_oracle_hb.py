@@ -1,0 +1,195 @@
+"""CPU pins of the harmonic-balance oracle (oracle/hb.py) and of the host-side logic of
+tinyvlm_amd.hb that needs no GPU.
+
+What fixes the oracle beyond retyping lib/libhbvlm3.cpp:
+- the closed-form kinematics against central finite differences of themselves (derivative part)
+  and against the product's dual-number tree (two independent evaluations of one chain);
+- the DFT matrix is orthogonal and its derivative matrix is the time derivative of the basis;
+- zero motion gives zero circulation and zero loads;
+- the Anderson result equals the fixed point of the (affine) map found by a direct dense solve;
+- physics on the quasi-2D wing/flap meshes (span 10000 chords): the first-harmonic lift of pure
+  heave and pure pitch about the quarter chord agrees with Theodorsen's closed form within 10 %
+  (a 12-panel chord, finite wake), and a constant pitch gives the steady VLM lift of the merged
+  mesh within 5 %, linear in the angle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+WING, FLAP = "mesh/3dof_wing_9x5.x", "mesh/3dof_flap_3x5.x"
+SCALE = 1.0 / 0.127
+
+
+@pytest.fixture(scope="module")
+def ohb(oracle, root):
+    import os
+
+    from oracle import hb
+
+    os.chdir(root)
+    return hb
+
+
+def _motion(K):
+    X = np.zeros((3, K))
+    X[0, 1] = 0.01 / 0.127
+    X[1, 2] = 0.02
+    X[2, 1] = 0.05
+    if K > 3:
+        X[1, 3] = 0.005
+        X[2, 4] = -0.01
+    return X
+
+
+def test_kinematics_derivative_fd(ohb):
+    X = _motion(5)
+    for t in (0.0, 0.7, 3.1):
+        pose, fs = ohb.transforms(X, 0.8, 6.0, t, 2)
+        h = 1e-6
+        pp, _ = ohb.transforms(X, 0.8, 6.0, t + h, 2)
+        pm, _ = ohb.transforms(X, 0.8, 6.0, t - h, 2)
+        for m in range(2):
+            fd = (pp[m][0] - pm[m][0]) / (2 * h)
+            assert np.max(np.abs(fd - pose[m][1])) < 1e-8
+        # rigid transforms: rotation part orthonormal
+        for T, _ in pose:
+            assert np.max(np.abs(T[:3, :3] @ T[:3, :3].T - np.eye(3))) < 1e-14
+        assert fs[0] == pytest.approx(6.0 / (ohb.B_SEMI * ohb.OMEGA_ALPHA), rel=1e-15)
+
+
+def test_product_dual_kinematics_match_closed_form(ohb):
+    from tinyvlm_amd.hb import Vars, build_kinematics
+
+    X = _motion(5)
+    fs_node, nodes = build_kinematics(X, 0.8, Vars(6.0), 2)
+    ts = np.array([0.0, 0.3, 2.2, 11.7])
+    T_vec = [n.transform(ts) for n in nodes]
+    for j, t in enumerate(ts):
+        pose, fs = ohb.transforms(X, 0.8, 6.0, float(t), 2)
+        for m in range(2):
+            d = nodes[m].transform_dual(float(t))
+            assert np.max(np.abs(d.value() - pose[m][0])) < 1e-14
+            assert np.max(np.abs(d.grad() - pose[m][1])) < 1e-13
+            assert np.max(np.abs(T_vec[m][:, :, j] - pose[m][0])) < 1e-14
+        assert np.allclose(-fs_node.linear_velocity(float(t), (0.0, 0.0, 0.0)), fs, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("K", [1, 3, 11])
+def test_dft_orthogonal_and_derivative(ohb, K):
+    omega = 0.7
+    period = 2 * math.pi / omega
+    D, dD = ohb.dft_matrices(K, omega, period)
+    assert np.max(np.abs(D.T @ D - np.eye(K))) < 1e-13
+    # row i of D is the basis at t_i; d/dt by central differences of the basis
+    h = 1e-6
+    for i in range(K):
+        t = i / K * period
+        Dp, _ = ohb.dft_matrices(K, omega, period * (t + h) / t) if t > 0 else (None, None)
+        if Dp is None:
+            continue
+        Dm, _ = ohb.dft_matrices(K, omega, period * (t - h) / t)
+        assert np.max(np.abs((Dp[i] - Dm[i]) / (2 * h) - dD[i])) < 1e-6
+
+
+def test_zero_motion_gives_zero_loads(ohb):
+    o = ohb.HBVLMOracle([WING, FLAP])
+    o.init(1, SCALE)
+    F = np.ones((3, 3))
+    o.run(0.8, 6.0, np.zeros((3, 3)), F)
+    assert np.max(np.abs(o.gamma_coeffs)) < 1e-14
+    assert np.max(np.abs(F)) < 1e-14
+
+
+def test_anderson_reaches_direct_fixed_point(ohb, oracle):
+    """The map g -> A^-1 (R_vel + W(g)) D is affine in g; solve (I - M) g = f(0) densely and compare
+    with the Anderson result.  Pins the Anderson/LSQ restatement and its convergence."""
+    o = ohb.HBVLMOracle([WING, FLAP])
+    o.init(1, SCALE)
+    X = _motion(3)
+    F = np.zeros((3, 3))
+    captured = {}
+    real_anderson = oracle.anderson_acceleration
+
+    def spy(x0, f, max_iter, tol, m):
+        captured["f"] = f
+        return real_anderson(x0, f, max_iter, tol, m)
+
+    oracle.anderson_acceleration = spy
+    try:
+        o.run(0.8, 6.0, X, F)
+    finally:
+        oracle.anderson_acceleration = real_anderson
+    f = captured["f"]
+    n = o.N * o.K
+    f0 = f(np.zeros(n))
+    M = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        M[:, j] = f(e) - f0
+    g_direct = np.linalg.solve(np.eye(n) - M, f0)
+    g = o.gamma_coeffs.reshape(-1, order="F")
+    assert np.max(np.abs(g - g_direct)) <= 1e-10 * np.max(np.abs(g_direct))
+    assert np.max(np.abs(f(g) - g)) <= 1e-11 * np.max(np.abs(g))
+
+
+def _theodorsen_C(k):
+    from scipy.special import hankel2
+
+    return hankel2(1, k) / (hankel2(1, k) + 1j * hankel2(0, k))
+
+
+@pytest.mark.parametrize("omega", [0.2, 0.5, 1.0])
+@pytest.mark.parametrize("dof", ["heave", "pitch"])
+def test_first_harmonic_lift_vs_theodorsen(ohb, omega, dof):
+    U = 6.0
+    u = U / (ohb.B_SEMI * ohb.OMEGA_ALPHA)
+    k = omega / u  # semi-chord 1 after scaling by 1/b
+    amp = 0.01
+    o = ohb.HBVLMOracle([WING, FLAP])
+    o.init(1, SCALE)
+    X = np.zeros((3, 3))
+    X[0 if dof == "heave" else 1, 1] = amp  # amp * cos(omega t)
+    F = np.zeros((3, 3))
+    o.run(omega, U, X, F)
+    ph = np.arange(3) / 3 * 2 * math.pi
+    first = 2 / 3 * np.sum(F[0] * np.cos(ph)) - 2j / 3 * np.sum(F[0] * np.sin(ph))
+    Ck = _theodorsen_C(k)
+    a = -0.5  # elastic axis at the quarter chord
+    if dof == "heave":  # h positive down: CL = pi b h''/U^2 + 2 pi C(k) h'/U
+        theo = amp * (-math.pi * k * k + 2j * math.pi * k * Ck)
+    else:  # CL = pi b (a'/U - b a a''/U^2) + 2 pi C(k) (a + b (1/2 - a) a'/U)
+        theo = amp * (math.pi * (1j * k + k * k * a) + 2 * math.pi * Ck * (1 + (0.5 - a) * 1j * k))
+    assert abs(first - theo) <= 0.1 * abs(theo), (first, theo)
+
+
+def test_constant_pitch_matches_steady_vlm(ohb, oracle, root):
+    import os
+
+    w = oracle.plot3d_read(os.path.join(root, WING))
+    f = oracle.plot3d_read(os.path.join(root, FLAP))
+    assert np.max(np.abs(w[:, -1, :3] - f[:, 0, :3])) == 0.0  # flap continues the wing
+    merged = np.asfortranarray(np.concatenate([w[:, :-1], f], axis=1))
+    merged[:, :, :3] *= SCALE
+    cls = []
+    for deg in (1.0, 3.0):
+        a = math.radians(deg)
+        o = ohb.HBVLMOracle([WING, FLAP])
+        o.init(0, SCALE)
+        X = np.zeros((3, 1))
+        X[1, 0] = a
+        F = np.zeros((3, 1))
+        o.run(0.3, 6.0, X, F)
+        _, cl_steady, _, info = oracle.vlm_run([merged], a)
+        assert info == 0
+        assert abs(F[0, 0] - cl_steady) <= 0.05 * cl_steady
+        cls.append(F[0, 0] / a)
+    assert cls[1] == pytest.approx(cls[0], rel=2e-3)
+
+
+def test_product_rejects_unknown_backend():
+    from tinyvlm_amd.hb import HBVLM
+
+    with pytest.raises(RuntimeError, match="Unsupported backend"):
+        HBVLM("cpu", [WING, FLAP])

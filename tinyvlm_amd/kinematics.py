@@ -14,14 +14,19 @@ from typing import Callable
 import numpy as np
 
 
+def _num(x):
+    return x if isinstance(x, np.ndarray) else float(x)
+
+
 class Dual:
-    """value + derivative; only the operations the transforms need."""
+    """value + derivative; only the operations the transforms need.  v and d are floats or, to
+    evaluate many times at once, numpy arrays of times (elementwise the same arithmetic)."""
 
     __slots__ = ("v", "d")
 
     def __init__(self, v, d=0.0):
-        self.v = float(v)
-        self.d = float(d)
+        self.v = _num(v)
+        self.d = _num(d)
 
     def __add__(self, o):
         o = o if isinstance(o, Dual) else Dual(o)
@@ -46,12 +51,20 @@ class Dual:
         return Dual(-self.v, -self.d)
 
 
+def _sin(x):
+    return np.sin(x) if isinstance(x, np.ndarray) else math.sin(x)
+
+
+def _cos(x):
+    return np.cos(x) if isinstance(x, np.ndarray) else math.cos(x)
+
+
 def dsin(x: Dual) -> Dual:
-    return Dual(math.sin(x.v), math.cos(x.v) * x.d)
+    return Dual(_sin(x.v), _cos(x.v) * x.d)
 
 
 def dcos(x: Dual) -> Dual:
-    return Dual(math.cos(x.v), -math.sin(x.v) * x.d)
+    return Dual(_cos(x.v), -_sin(x.v) * x.d)
 
 
 class DMat4:
@@ -71,11 +84,23 @@ class DMat4:
                 r[i][j] = s
         return DMat4(r)
 
+    def _stack(self, part) -> np.ndarray:
+        ent = [[getattr(x, part) for x in row] for row in self.m]
+        n = [np.shape(e) for row in ent for e in row if isinstance(e, np.ndarray)]
+        if not n:
+            return np.array(ent)
+        out = np.empty((4, 4) + n[0])
+        for i in range(4):
+            for j in range(4):
+                out[i, j] = ent[i][j]
+        return out
+
     def value(self) -> np.ndarray:
-        return np.array([[x.v for x in row] for row in self.m])
+        """4x4 values, or (4, 4, len(t)) when evaluated at an array of times."""
+        return self._stack("v")
 
     def grad(self) -> np.ndarray:
-        return np.array([[x.d for x in row] for row in self.m])
+        return self._stack("d")
 
 
 def translation(x, y, z) -> DMat4:
@@ -117,11 +142,11 @@ class KinematicNode:
         self.parent = parent
         return self
 
-    def transform_dual(self, t: float) -> DMat4:
-        r = self.fn(Dual(t, 1.0))
+    def transform_dual(self, t) -> DMat4:
+        r = self.fn(Dual(t, np.ones_like(t) if isinstance(t, np.ndarray) else 1.0))
         return self.parent.transform_dual(t) @ r if self.parent else r
 
-    def transform(self, t: float) -> np.ndarray:
+    def transform(self, t) -> np.ndarray:
         return self.transform_dual(t).value()
 
     def linear_velocity(self, t: float, vertex) -> np.ndarray:
