@@ -18,7 +18,10 @@
 //              and the next step's wake shed, velocities and rhs
 // The fast mode evaluates the wake induction in the FP64 seeded form (FP64 arithmetic, rsqrt and
 // reciprocal from the FP64 hardware estimates plus one Newton step) and replays the whole loop as
-// one hipGraph.
+// one hipGraph.  Its geometry does not depend on the solution (prescribed motion), so k_geom_all
+// computes every step's pose, panel metrics, kinematic velocities, velocity rhs and wake vertex row
+// once at setup; a step is then the wake lattice, its reduction into rhs = rhs_vel - wake, the GEMV
+// partials of A^{-1} rhs and k_post (gamma post-processing, wake Gamma, forces, CL/CM).
 #include <cstring>
 #include <vector>
 
@@ -34,6 +37,8 @@ constexpr int kMaxS = TVLM_MAX_SURFACES;
 struct USurf {
     double *verts_pos, *verts, *colloc, *colloc_h, *normals, *areas, *vel, *forces;
     double *gprev, *gdelta, *gdt, *wake, *gamma_wake;
+    // fast mode: geometry of every step (step s at + s * {nv*3, np*3, np*3, np, np*3})
+    double *g_verts, *g_colloc, *g_normals, *g_areas, *g_vel;
     int32_t ns, nc, off, v_off, te_off;
 };
 
@@ -130,6 +135,10 @@ __device__ __forceinline__ void block_sums(double (&v)[K], double (*lds)[K]) {
     __syncthreads();
 }
 
+// FROM_PARTIALS: a panel's circulation and its upstream chordwise neighbour's are summed from the
+// solve's partials inside the panel loop (same order, so the same values) instead of a first pass
+// through gamma[] and a barrier; the panel's partial and geometry loads are then issued together.
+template <int NT, bool FROM_PARTIALS = false>
 __device__ __forceinline__ void post_step(const UTab& tab, const double* __restrict__ gsrc, int cb, int step,
                                           double inv_dt, const double* __restrict__ fs_all,
                                           const double* __restrict__ ref_all, double rho, double* __restrict__ gamma,
@@ -137,12 +146,15 @@ __device__ __forceinline__ void post_step(const UTab& tab, const double* __restr
                                           double (*lds)[6]) {
     const int t = threadIdx.x;
     const int N = tab.N;
-    for (int P = t; P < N; P += kPostThreads) {
+    auto gsum = [&](int P) {
         double g = 0.0;
         for (int c = 0; c < cb; c++) g += gsrc[(int64_t)c * N + P];
-        gamma[P] = g;
+        return g;
+    };
+    if constexpr (!FROM_PARTIALS) {
+        for (int P = t; P < N; P += NT) gamma[P] = gsum(P);
+        __syncthreads();
     }
-    __syncthreads();
     const v3<double> fs = {fs_all[3 * step], fs_all[3 * step + 1], fs_all[3 * step + 2]};
     const v3<double> ref = {ref_all[3 * step], ref_all[3 * step + 1], ref_all[3 * step + 2]};
     // lift axis = normalize(freestream x (0,1,0))  (vlm/src/vlm_backend.cpp:86)
@@ -158,10 +170,11 @@ __device__ __forceinline__ void post_step(const UTab& tab, const double* __restr
         const int ns = S.ns, nc = S.nc, np = ns * nc;
         const int64_t vs1 = ns + 1, vs2 = vs1 * (nc + 1);
         double a_acc = 0.0, c_acc = 0.0, mx = 0.0, my = 0.0, mz = 0.0, mac_acc = 0.0;
-        for (int p = t; p < np; p += kPostThreads) {
+        for (int p = t; p < np; p += NT) {
             const int i = p % ns, j = p / ns;
-            const double g = gamma[S.off + p];
-            const double gd = j > 0 ? g + (-1.0) * gamma[S.off + p - ns] : g;
+            const double g = FROM_PARTIALS ? gsum(S.off + p) : gamma[S.off + p];
+            const double gd = j > 0 ? g + (-1.0) * (FROM_PARTIALS ? gsum(S.off + p - ns) : gamma[S.off + p - ns]) : g;
+            if constexpr (FROM_PARTIALS) gamma[S.off + p] = g;
             const double gt = (g + (-1.0) * S.gprev[p]) * inv_dt;
             S.gdelta[p] = gd;
             S.gdt[p] = gt;
@@ -191,7 +204,7 @@ __device__ __forceinline__ void post_step(const UTab& tab, const double* __restr
             }
         }
         if (forces) {
-            for (int i = t; i < ns; i += kPostThreads) {
+            for (int i = t; i < ns; i += NT) {
                 const int64_t te = (int64_t)nc * vs1;
                 const double* V = S.verts;
                 const v3<double> d0 = {V[i] - V[i + te], V[i + vs2] - V[i + te + vs2], V[i + 2 * vs2] - V[i + te + 2 * vs2]};
@@ -302,10 +315,111 @@ __global__ __launch_bounds__(kPostThreads) void k_post_advance(UTab tab, const d
                                                                const double* __restrict__ dT_all, int do_pre,
                                                                double* __restrict__ rhs) {
     __shared__ double lds[kPostThreads / 64][6];
-    post_step(tab, gsrc, cb, step, inv_dt, fs_all, ref_all, rho, gamma, cl_out, cm_out, lds);
+    post_step<kPostThreads>(tab, gsrc, cb, step, inv_dt, fs_all, ref_all, rho, gamma, cl_out, cm_out, lds);
     __syncthreads();
     for (int e = threadIdx.x; e < tab.NV + tab.N; e += kPostThreads)
         advance_item(tab, T_all, dT_all, step + 1, do_pre, rhs, e);
+}
+
+// Fast mode: the geometry of steps 0 .. nsteps-1 at once.  Step s poses the mesh at T_all[s]
+// applied to the t = 0 pose (step 0: the t = 0 pose itself, vlm.cpp:366-373, 489-507), stores the
+// vertex coordinates, the panel metrics at alpha = 0, the kinematic velocity -(dT_all[s] [x_h; 1])
+// at the t = 0 collocation point and rhs_vel = -(V . n) (vlm.cpp:419-438), and sheds the
+// trailing-edge vertex row into wake row t_steps-1-s (vlm.cpp:412-418).  Same arithmetic as
+// k_pre / advance_item.
+__global__ void k_geom_all(UTab tab, const double* __restrict__ T_all, const double* __restrict__ dT_all,
+                           int nsteps, double* __restrict__ g_rhs) {
+    const int per = tab.NV + tab.N;
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= (int64_t)per * nsteps) return;
+    const int s = (int)(e / per), r = (int)(e % per);
+    if (r < tab.NV) {
+        const int m = surf_of(tab, r, 1);
+        const USurf& S = tab.s[m];
+        const int p = r - S.v_off;
+        const int64_t nv = (int64_t)(S.ns + 1) * (S.nc + 1);
+        const double* M = T_all + ((int64_t)s * tab.n + m) * 16;
+        const double x[4] = {S.verts_pos[p], S.verts_pos[p + nv], S.verts_pos[p + 2 * nv], S.verts_pos[p + 3 * nv]};
+        double y[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) y[k] = s == 0 ? x[k] : displace_row(M, 1, 4, k, x);
+        double* gv = S.g_verts + (int64_t)s * nv * 3;
+#pragma unroll
+        for (int k = 0; k < 3; k++) gv[p + k * nv] = y[k];
+        const int vs1 = S.ns + 1;
+        if (p / vs1 == S.nc) {
+            const int i = p % vs1;
+            const int64_t ws2 = (int64_t)vs1 * tab.t_steps;
+            const int row = tab.t_steps - 1 - s;
+#pragma unroll
+            for (int k = 0; k < 4; k++) S.wake[i + (int64_t)row * vs1 + k * ws2] = y[k];
+        }
+        return;
+    }
+    const int P = r - tab.NV;
+    const int m = surf_of(tab, P, 0);
+    const USurf& S = tab.s[m];
+    const int p = P - S.off;
+    const int ns = S.ns, nc = S.nc, np = ns * nc;
+    const int i = p % ns, j = p / ns;
+    const int64_t vs1 = ns + 1, nv = vs1 * (nc + 1);
+    const double* M = T_all + ((int64_t)s * tab.n + m) * 16;
+    auto vtx = [&](int ii, int jj) -> v3<double> {
+        const int64_t o = ii + jj * vs1;
+        const double x[4] = {S.verts_pos[o], S.verts_pos[o + nv], S.verts_pos[o + 2 * nv], S.verts_pos[o + 3 * nv]};
+        if (s == 0) return {x[0], x[1], x[2]};
+        return {displace_row(M, 1, 4, 0, x), displace_row(M, 1, 4, 1, x), displace_row(M, 1, 4, 2, x)};
+    };
+    v3<double> c, n;
+    double area;
+    panel_metrics(0.0, vtx(i, j), vtx(i + 1, j), vtx(i + 1, j + 1), vtx(i, j + 1), c, n, area);
+    const int64_t o3 = (int64_t)s * np * 3;
+    S.g_colloc[o3 + p] = c.x;
+    S.g_colloc[o3 + p + np] = c.y;
+    S.g_colloc[o3 + p + 2 * np] = c.z;
+    S.g_normals[o3 + p] = n.x;
+    S.g_normals[o3 + p + np] = n.y;
+    S.g_normals[o3 + p + 2 * np] = n.z;
+    S.g_areas[(int64_t)s * np + p] = area;
+    const double* d = dT_all + ((int64_t)s * tab.n + m) * 16;
+    const double x = S.colloc_h[p], y = S.colloc_h[p + np], z = S.colloc_h[p + 2 * np];
+    double V[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        V[k] = -(d[k] * x + d[k + 4] * y + d[k + 8] * z + d[k + 12]);
+        S.g_vel[o3 + p + k * np] = V[k];
+    }
+    g_rhs[(int64_t)s * tab.N + P] = -(V[0] * n.x + V[1] * n.y + V[2] * n.z);
+}
+
+__global__ __launch_bounds__(kPostThreads) void k_post(UTab tab, const double* __restrict__ gsrc, int cb, int step,
+                                                       double inv_dt, const double* __restrict__ fs_all,
+                                                       const double* __restrict__ ref_all, double rho,
+                                                       double* __restrict__ gamma, double* __restrict__ cl_out,
+                                                       double* __restrict__ cm_out) {
+    __shared__ double lds[kPostThreads / 64][6];
+    post_step<kPostThreads, true>(tab, gsrc, cb, step, inv_dt, fs_all, ref_all, rho, gamma, cl_out, cm_out, lds);
+}
+
+// rhs[p] = base[p] - sum over tiles of partial[tile][p] (fixed order, as k_wake_reduce)
+__global__ __launch_bounds__(64 * kRedWaves) void k_wake_reduce_base(double* __restrict__ rhs,
+                                                                     const double* __restrict__ base,
+                                                                     const double* __restrict__ partial, int tiles,
+                                                                     int total) {
+    __shared__ double red[kRedWaves][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
+    double s = 0.0;
+    if (p < total)
+        for (int t = wave; t < tiles; t += kRedWaves) s += partial[(int64_t)t * total + p];
+    red[wave][lane] = s;
+    __syncthreads();
+#pragma unroll
+    for (int h = kRedWaves / 2; h > 0; h >>= 1) {
+        if (wave < h) red[wave][lane] += red[wave + h][lane];
+        __syncthreads();
+    }
+    if (wave == 0 && p < total) rhs[p] = base[p] - red[0][lane];
 }
 
 __global__ void k_fill64(double* x, int64_t n, double v) {
@@ -338,6 +452,7 @@ struct tvlm_uvlm_plan {
     double *lhs = nullptr, *ainv = nullptr, *rhs = nullptr, *gamma = nullptr, *gpart = nullptr, *wpart = nullptr;
     double *cl = nullptr, *cm = nullptr, *T_all = nullptr, *dT_all = nullptr, *fs_all = nullptr, *ref_all = nullptr;
     int32_t *ipiv = nullptr, *info = nullptr;
+    double* g_rhs = nullptr;  // fast mode: rhs_vel of every step, [t_steps-1][N]
     int cb = 1, cbs = 0;
     size_t wpart_elems = 0;
     bool motion_set = false;
@@ -376,7 +491,10 @@ void wake_tables(const tvlm_uvlm_plan* P, int it, RecvTable<double>& rt, WakeTab
         const USurf& S = P->tab.s[m];
         const int np = S.ns * S.nc;
         rt.offset[m] = S.off;
-        rt.r[m] = {S.colloc, np, S.normals, np, np};
+        if (P->mode == 1)  // the step's precomputed geometry
+            rt.r[m] = {S.g_colloc + (int64_t)it * np * 3, np, S.g_normals + (int64_t)it * np * 3, np, np};
+        else
+            rt.r[m] = {S.colloc, np, S.normals, np, np};
     }
     rt.offset[P->n_surf] = P->N;
     rt.total = P->N;
@@ -439,6 +557,10 @@ int setup(tvlm_uvlm_plan* P) {
         TVLM_LAUNCHED();
         tvlm_view x = {P->ainv, {P->N, P->N, 1}, {1, P->N, NN}};
         TVLM_TRY(tvlm_getrs_f64(&lhs, P->ipiv, &x, st));
+        const int nsteps = (int)P->t_steps - 1;
+        hipLaunchKernelGGL(k_geom_all, dim3(nblk((int64_t)(P->tab.NV + P->N) * nsteps)), dim3(256), 0, st, P->tab,
+                           P->T_all, P->dT_all, nsteps, P->g_rhs);
+        TVLM_LAUNCHED();
     }
     return TVLM_OK;
 }
@@ -478,7 +600,39 @@ int launch_wake(tvlm_uvlm_plan* P, int i, int& tiles) {
     return TVLM_OK;
 }
 
+// Fast mode step i: wake lattice + reduction into rhs = rhs_vel - wake (i > 0), gamma partials of
+// A^{-1} rhs, post-processing on the step's precomputed geometry.
+int enqueue_step_fast(tvlm_uvlm_plan* P, int i) {
+    hipStream_t st = P->stream;
+    const double* x = P->g_rhs;  // step 0 has no wake: rhs = rhs_vel
+    if (i > 0) {
+        int tiles = 0;
+        TVLM_TRY(launch_wake(P, i, tiles));
+        hipLaunchKernelGGL(k_wake_reduce_base, dim3((P->N + 63) / 64), dim3(64 * kRedWaves), 0, st, P->rhs,
+                           P->g_rhs + (int64_t)i * P->N, P->wpart, tiles, P->N);
+        TVLM_LAUNCHED();
+        x = P->rhs;
+    }
+    UTab t = P->tab;
+    for (int m = 0; m < t.n; m++) {
+        USurf& S = t.s[m];
+        const int64_t np = (int64_t)S.ns * S.nc, nv = (int64_t)(S.ns + 1) * (S.nc + 1);
+        S.verts = S.g_verts + i * nv * 3;
+        S.vel = S.g_vel + i * np * 3;
+        S.normals = S.g_normals + i * np * 3;
+        S.areas = S.g_areas + i * np;
+    }
+    hipLaunchKernelGGL(k_gemv_part, dim3((P->N + 63) / 64, P->cb), dim3(256), 0, st, P->ainv, (int64_t)P->N, P->N,
+                       P->cbs, x, P->gpart);
+    TVLM_LAUNCHED();
+    hipLaunchKernelGGL(k_post, dim3(1), dim3(kPostThreads), 0, st, t, P->gpart, P->cb, i, 1.0 / P->dt, P->fs_all,
+                       P->ref_all, 1.0, P->gamma, P->cl, P->cm);
+    TVLM_LAUNCHED();
+    return TVLM_OK;
+}
+
 int enqueue_step(tvlm_uvlm_plan* P, int i) {
+    if (P->mode == 1) return enqueue_step_fast(P, i);
     hipStream_t st = P->stream;
     UTab& tab = P->tab;
     if (i == 0) {  // later steps start inside the previous step's k_advance
@@ -570,6 +724,14 @@ int tvlm_uvlm_create(tvlm_uvlm_plan** out, int32_t n_surf, const int64_t* ns, co
             return fail(rc);
         if ((rc = P->alloc((void**)&S.gamma_wake, sizeof(double) * (size_t)ns[m] * (t_steps - 1))) != TVLM_OK)
             return fail(rc);
+        if (mode == 1) {
+            const size_t st1 = (size_t)(t_steps - 1);
+            if ((rc = P->alloc((void**)&S.g_verts, sizeof(double) * nv * 3 * st1)) != TVLM_OK) return fail(rc);
+            double** gb3[] = {&S.g_colloc, &S.g_normals, &S.g_vel};
+            for (double** b : gb3)
+                if ((rc = P->alloc((void**)b, sizeof(double) * np * 3 * st1)) != TVLM_OK) return fail(rc);
+            if ((rc = P->alloc((void**)&S.g_areas, sizeof(double) * np * st1)) != TVLM_OK) return fail(rc);
+        }
         if (hipMemcpyAsync(P->init[m], verts_init[m], sizeof(double) * nv * 4, hipMemcpyHostToDevice, P->stream) !=
             hipSuccess) {
             set_error("upload of surface %d failed", m);
@@ -601,6 +763,8 @@ int tvlm_uvlm_create(tvlm_uvlm_plan** out, int32_t n_surf, const int64_t* ns, co
     if ((rc = P->alloc((void**)&P->ref_all, sizeof(double) * 3 * t_steps)) != TVLM_OK) return fail(rc);
     if ((rc = P->alloc((void**)&P->ipiv, sizeof(int32_t) * N)) != TVLM_OK) return fail(rc);
     if ((rc = P->alloc((void**)&P->info, sizeof(int32_t))) != TVLM_OK) return fail(rc);
+    if (mode == 1 && (rc = P->alloc((void**)&P->g_rhs, sizeof(double) * N * (size_t)(t_steps - 1))) != TVLM_OK)
+        return fail(rc);
     if (hipStreamSynchronize(P->stream) != hipSuccess) {
         set_error("stream synchronize failed");
         return fail(TVLM_ERR_HIP);
