@@ -9,12 +9,17 @@
 //                     LDS-only barriers) and lists the panel's composed row interchanges;
 //   2. k_swap_trsm  : the interchanges on the columns left and right of the panel, and
 //                     U12 = L11^{-1} A12 for the columns right of it;
-//   3. k_update_mfma: A22 -= L21 U12 on the matrix cores.
+//   3. A22 -= L21 U12 on the matrix cores: rocBLAS ?gemm_strided_batched for trailing blocks of at
+//                     least 512 rows, k_update_mfma below that.
 // Panels taller than 8192 rows use the column-by-column k_panel + k_laswp + k_trsm instead.
 // getrs (n <= 16384): the interchanges are composed per block of 64 pivots and the 64 x 64
 // diagonal blocks of L and U inverted (k_solve_prep), then one work-group per right-hand side
 // applies the interchanges and both blocked substitutions with the column in LDS
 // (k_solve_blocked); larger n uses block-row kernels (k_diag_solve / k_off_update).
+#include <mutex>
+
+#include <rocblas/rocblas.h>
+
 #include "tvlm_internal.hpp"
 
 namespace tvlm {
@@ -690,6 +695,51 @@ __global__ __launch_bounds__(256) void k_update_mfma(T* __restrict__ C, int64_t 
         }
 }
 
+// The trailing update is a plain strided-batched GEMM, C -= A B with k = panel width; for
+// m >= kBlasUpdateRows it goes to rocBLAS (?gemm_strided_batched on the matrix cores, which moves
+// the trailing matrix at about 4 TB/s here), smaller updates stay in k_update_mfma.  One rocBLAS
+// handle per device, created on first use; calls are serialised because the handle's stream is
+// set per call.
+constexpr int kBlasUpdateRows = 512;
+
+rocblas_handle blas_handle() {
+    static std::mutex mu;
+    static rocblas_handle handles[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!handles[dev] && rocblas_create_handle(&handles[dev]) != rocblas_status_success) handles[dev] = nullptr;
+    return handles[dev];
+}
+
+template <typename T>
+bool update_blas(T* C, int64_t ldc, const T* A, int64_t lda, const T* B, int64_t ldb, int m, int n, int kb, int64_t sb,
+                 int batch, hipStream_t st) {
+    static std::mutex mu;
+    rocblas_handle h = blas_handle();
+    if (!h) return false;
+    std::lock_guard<std::mutex> lock(mu);
+    if (rocblas_set_stream(h, st) != rocblas_status_success) return false;
+    const T alpha = T(-1), beta = T(1);
+    const rocblas_stride str = batch > 1 ? (rocblas_stride)sb : 0;
+    rocblas_status rs;
+    if constexpr (sizeof(T) == 4)
+        rs = rocblas_sgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, kb, &alpha, A, lda,
+                                           str, B, ldb, str, &beta, C, ldc, str, batch);
+    else
+        rs = rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, kb, &alpha, A, lda,
+                                           str, B, ldb, str, &beta, C, ldc, str, batch);
+    return rs == rocblas_status_success;
+}
+
+template <typename T>
+void launch_update(T* C, int64_t ldc, const T* A, int64_t lda, const T* B, int64_t ldb, int m, int n, int kb,
+                   int64_t sb, int batch, hipStream_t st) {
+    if (m >= kBlasUpdateRows && update_blas<T>(C, ldc, A, lda, B, ldb, m, n, kb, sb, batch, st)) return;
+    hipLaunchKernelGGL(k_update_mfma<T>, dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, st, C, ldc, A, lda,
+                       B, ldb, m, n, kb, sb);
+}
+
 template <typename T, int NT, int Q, int IB>
 int launch_panel_reg(T* a, int64_t lda, int n, int k0, int kb, int32_t* ipiv, int32_t* info, int32_t* perm,
                      hipStream_t st, int batch, int64_t sb) {
@@ -720,6 +770,7 @@ int panel_fast(T* a, int64_t lda, int n, int k0, int kb, int32_t* ipiv, int32_t*
 }
 
 constexpr int kFastPanelRows = 8192;  // taller panels take the column-by-column panel
+constexpr int kWidePanelBatch = 8;     // batches of at least this many matrices use 64-column panels
 
 // ---- getrs ------------------------------------------------------------------------------------
 template <typename T>
@@ -808,8 +859,10 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     TVLM_TRY(device_workspace(sizeof(int32_t) * kPermInts * (size_t)batch, &ws, kWsLuPerm));
     int32_t* perm = static_cast<int32_t*>(ws);
     for (int k0 = 0; k0 < n;) {
-        // tall panels are narrower: the in-panel left-looking traffic grows with width squared
-        const int width = n - k0 > 2048 ? NB / 2 : NB;
+        // tall panels of one matrix are narrower: the in-panel left-looking traffic grows with width
+        // squared.  A batch is bound by the trailing updates instead, whose traffic (the whole
+        // trailing matrix read and written per panel) halves with the full width.
+        const int width = (n - k0 > 2048 && batch < kWidePanelBatch) ? NB / 2 : NB;
         const int kb = (n - k0 < width) ? n - k0 : width;
         const int j1 = k0 + kb;
         const int kp = k0;  // this panel's first column
@@ -843,9 +896,8 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
         }
         if (j1 >= n) continue;
         const int m = n - j1;
-        hipLaunchKernelGGL(k_update_mfma<T>, dim3((m + 63) / 64, (m + 63) / 64, batch), dim3(256), 0, st,
-                           a + j1 + (int64_t)j1 * lda, lda, a + j1 + (int64_t)kp * lda, lda,
-                           a + kp + (int64_t)j1 * lda, lda, m, m, kb, sb);
+        launch_update<T>(a + j1 + (int64_t)j1 * lda, lda, a + j1 + (int64_t)kp * lda, lda, a + kp + (int64_t)j1 * lda,
+                         lda, m, m, kb, sb, batch, st);
         TVLM_LAUNCHED();
     }
     return TVLM_OK;
