@@ -1,0 +1,238 @@
+"""CPU restatement of the coupled pitch/heave aeroelastic driver UVLM_2DOF (tests/uvlm_2dof.cpp:48-865,
+COUPLED path) with the Newmark-beta integrator (vlm/src/vlm_integrator.cpp:1-68).
+
+TEST INFRASTRUCTURE ONLY.  Imported by ``tests/`` as the checker of ``tinyvlm_amd.two_dof``; the HIP
+product path never imports it.  The aerodynamic calls go through the oracle's C restatement
+(``oracle.cpuref``) in the chosen precision (the reference runs in FP32); the 2 x 2 structural
+algebra is numpy in the same precision.
+
+The kinematics are written in closed form: wing = F(t) H P with the freestream F(t) = T(-t, 0, 0),
+the heave H = T(0, 0, -(h + h' tau)) and the pitch P = rotation about (1 + a_h, 0, 0), axis +y, by
+alpha + alpha' tau, all evaluated at tau = 0 with the time derivative by the product rule
+(tests/uvlm_2dof.cpp:645-680).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import cpuref as C
+
+PI_F = np.float32(3.141593)
+EPS_F = np.float32(1.19209e-07)
+
+
+@dataclass
+class Vars:
+    """UVLM_2DOF_Vars (tests/uvlm_2dof.cpp:23-39) with the values of the reference's main
+    (tests/uvlm_2dof.cpp:884-898)."""
+    b: float = 0.5
+    k_a: float = 1000.0
+    rho: float = 1.0
+    a_h: float = -0.5
+    omega: float = 0.2
+    zeta_a: float = 0.0
+    zeta_h: float = 0.0
+    x_a: float = 0.25
+    mu: float = 100.0
+    r_a: float = 0.5
+    U_a: float = 7.0
+
+
+def torsional_func(alpha, beta0=0.0, beta1=0.1, beta2=0.0, beta3=40.0):
+    """Cubic pitch stiffness (tests/uvlm_2dof.cpp:285-287)."""
+    return beta0 + beta1 * alpha + beta2 * alpha * alpha + beta3 * alpha * alpha * alpha
+
+
+def _rot_y(px, ang, dang):
+    c, s = math.cos(ang), math.sin(ang)
+    R = np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+    dR = np.array([[-s, 0.0, c], [0.0, 0.0, 0.0], [-c, 0.0, -s]]) * dang
+    p = np.array([px, 0.0, 0.0])
+    M = np.eye(4)
+    M[:3, :3] = R
+    M[:3, 3] = p - R @ p
+    dM = np.zeros((4, 4))
+    dM[:3, :3] = dR
+    dM[:3, 3] = -dR @ p
+    return M, dM
+
+
+def wing_transform(t, h, hd, al, ald, a_h):
+    """(T, dT/dt) of F(t) H P at the structural state (h, h', alpha, alpha')."""
+    F = np.eye(4)
+    F[0, 3] = -t
+    dF = np.zeros((4, 4))
+    dF[0, 3] = -1.0
+    Hm = np.eye(4)
+    Hm[2, 3] = -h
+    dH = np.zeros((4, 4))
+    dH[2, 3] = -hd
+    P, dP = _rot_y(1.0 + a_h, al, ald)
+    T = F @ Hm @ P
+    dT = dF @ Hm @ P + F @ dH @ P + F @ Hm @ dP
+    return T, dT
+
+
+class NewmarkBeta:
+    """vlm/src/vlm_integrator.cpp:1-68 (average acceleration: beta 1/4, gamma 1/2)."""
+
+    def __init__(self, M, Cm, K, dt, beta=0.25, gamma=0.5, dtype=np.float32):
+        self.dt, self.beta, self.gamma, self.dt_ = dt, beta, gamma, dtype
+        x1 = dtype(gamma / (beta * dt))
+        x0 = dtype(1.0 / (beta * dt * dt))
+        self.K_eff = (K + x0 * M + x1 * Cm).astype(dtype)
+
+    def step(self, M, Cm, v_i, a_i, dF):
+        b, g, dt, T = self.beta, self.gamma, self.dt, self.dt_
+        xd0, xd1 = T(1.0 / (b * dt)), T(g / b)
+        xdd0, xdd1 = T(1.0 / (2 * b)), T(-dt * (1 - g / (2 * b)))
+        x1, x0 = T(g / (b * dt)), T(1.0 / (b * dt * dt))
+        rhs = dF + M @ (xd0 * v_i + xdd0 * a_i) + Cm @ (xd1 * v_i + xdd1 * a_i)
+        du = np.linalg.solve(self.K_eff.astype(np.float64), rhs.astype(np.float64)).astype(T)
+        dv = x1 * du - xd1 * v_i - xdd1 * a_i
+        da = x0 * du - xd0 * v_i - xdd0 * a_i
+        return du.astype(T), dv.astype(T), da.astype(T)
+
+
+def run(meshes, vars: Vars, t_final_nd: float, dtype=np.float32, max_steps=None):
+    """UVLM_2DOF::run (tests/uvlm_2dof.cpp:428-865, COUPLED).  meshes: [(path, lifting), ...].
+    Returns dict t, u (2, n), v, a, F, iters (per step) with n = steps + 1 states."""
+    T = dtype
+    b = T(vars.b)
+    surfs = []
+    for path, lifting in meshes:
+        v = C.read_mesh(path, dtype)  # FP32 parse + quarter chord (Simulation, vlm.cpp:17-45)
+        v[:, :, :3] = (v[:, :, :3] / b).astype(T)
+        surfs.append(C.Surface(verts=np.asfortranarray(v), lifting=lifting))
+    verts_init = [s.verts.copy(order="F") for s in surfs]
+    a_h = float(T(vars.a_h))
+    init_angle = float(T(3.0) * PI_F / T(180.0))
+    T0, _ = _rot_y(1.0 + a_h, init_angle, 0.0)
+    for s, vi in zip(surfs, verts_init):
+        s.verts = C.displace_wing(T0.astype(T), vi)
+    C.mesh_metrics(0.0, surfs)
+    colloc_h = [s.colloc.copy(order="F") for s in surfs]
+    last = verts_init[-1]
+    d = last[0, -1, :3].astype(T) - last[0, -2, :3].astype(T)
+    dt_nd = T(np.sqrt(T(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])))
+    t_steps = int(T(t_final_nd) / dt_nd)
+    n_loop = t_steps - 2 if max_steps is None else min(t_steps - 2, max_steps)
+    for s in surfs:
+        s.verts_wake = np.zeros((s.ns + 1, t_steps, 4), dtype=T, order="F")
+        s.gamma_wake = np.zeros((s.ns, t_steps - 1), dtype=T, order="F")
+    # structure (tests/uvlm_2dof.cpp:334-412); linear stiffness is on the force side
+    M = np.array([[1.0, vars.x_a], [vars.x_a / vars.r_a ** 2, 1.0]], dtype=T)
+    Cm = np.array([[2.0 * vars.zeta_h * (vars.omega / vars.U_a), 0.0], [0.0, 2.0 * vars.zeta_a / vars.U_a]], dtype=T)
+    K = np.zeros((2, 2), dtype=T)
+    integ = NewmarkBeta(M, Cm, K, float(dt_nd), dtype=T)
+    n = n_loop + 1
+    u = np.zeros((2, n + 1), dtype=T)
+    v = np.zeros((2, n + 1), dtype=T)
+    a = np.zeros((2, n + 1), dtype=T)
+    F = np.zeros((2, n + 1), dtype=T)
+    u[1, 0] = T(init_angle)
+    kh = T((vars.omega / vars.U_a) ** 2)
+    ka = T(1.0 / (vars.U_a * vars.U_a))
+    a0 = np.array([-kh * u[0, 0], -ka * T(torsional_func(u[1, 0]))], dtype=T)
+    a0 = a0 - Cm @ v[:, 0] - K @ u[:, 0]
+    a[:, 0] = np.linalg.solve(M.astype(np.float64), a0.astype(np.float64)).astype(T)
+    # aerodynamics at t = 0 (tests/uvlm_2dof.cpp:547-559)
+    lhs = C.lhs_assemble(surfs, 0)
+    lu, ipiv, info = C.getrf(lhs)
+    if info != 0:
+        raise RuntimeError(f"singular LHS (getrf info {info})")
+    for s in surfs:
+        C.wake_shed(s.verts, s.verts_wake, 0)
+    _, dT0 = wing_transform(0.0, 0.0, 0.0, 0.0, 0.0, a_h)  # freestream only at t = 0
+    Fm = np.eye(4)
+    dFm = np.zeros((4, 4))
+    dFm[0, 3] = -1.0
+
+    def velocities(dT):
+        return [np.asfortranarray(-(np.einsum("ij,abj->abi", dT[:3, :3], c0) + dT[:3, 3]).astype(T))
+                for c0 in colloc_h]
+
+    def solve_gamma(vel, it):
+        rhs = np.zeros(sum(s.ns * s.nc for s in surfs), dtype=T)
+        C.rhs_assemble_velocities(rhs, surfs, vel)
+        C.rhs_assemble_wake_influence(rhs, surfs, it)
+        return C.getrs(lu, ipiv, rhs)
+
+    def update_gamma(g, it, gamma_prev=None):
+        out_delta, out_dt = [], []
+        begin = 0
+        gw = []
+        for s in surfs:
+            gm = np.asfortranarray(g[begin: begin + s.ns * s.nc].reshape(s.ns, s.nc, order="F"))
+            gw.append(gm)
+            begin += s.ns * s.nc
+        for m, s in enumerate(surfs):
+            gm = gw[m]
+            s.gamma_wake[:, s.gamma_wake.shape[1] - 1 - it] = gm[:, -1]
+            gd = gm.copy(order="F")
+            if m == 1:  # the flap's first row continues the wing (tests/uvlm_2dof.cpp:320-327)
+                gd[:, 0] = gd[:, 0] - gw[0][:, -1]
+            gd[:, 1:] = gd[:, 1:] - gm[:, :-1]
+            out_delta.append(np.asfortranarray(gd))
+            if gamma_prev is not None:
+                out_dt.append(np.asfortranarray(((gm - gamma_prev[m]) * T(1.0 / float(dt_nd))).astype(T)))
+        return gw, out_delta, out_dt
+
+    g0 = solve_gamma(velocities(dFm), 0)
+    gamma, _, _ = update_gamma(g0, 0)
+    iters = []
+    fs = np.array([1.0, 0.0, 0.0], dtype=T)
+    for i in range(n_loop):
+        t_nd = float(T(i) * dt_nd)
+        gamma_prev = [g.copy(order="F") for g in gamma]
+        dF = np.zeros(2, dtype=T)
+        du = np.zeros(2, dtype=T)
+        du_k = np.ones(2, dtype=T)
+        it = 0
+        while float(np.sqrt(np.dot(du_k.astype(np.float64), du_k.astype(np.float64)))) > float(EPS_F):
+            du_k = du.copy()
+            du, dv, da = integ.step(M, Cm, v[:, i], a[:, i], dF)
+            u[:, i + 1] = u[:, i] + du
+            v[:, i + 1] = v[:, i] + dv
+            a[:, i + 1] = a[:, i] + da
+            tt = float(T(t_nd) + dt_nd)
+            Tw, dTw = wing_transform(tt, float(u[0, i + 1]), float(v[0, i + 1]), float(u[1, i + 1]),
+                                     float(v[1, i + 1]), a_h)
+            ref = (Tw @ np.array([1.0 + a_h, 0.0, 0.0, 1.0]))[:3]
+            for s, vi in zip(surfs, verts_init):
+                s.verts = C.displace_wing(Tw.astype(T), vi)
+            C.mesh_metrics(0.0, surfs)
+            for s in surfs:
+                C.wake_shed(s.verts, s.verts_wake, i + 1)
+            vel = velocities(dTw)
+            g = solve_gamma(vel, i + 1)
+            gamma, gdelta, gdt = update_gamma(g, i + 1, gamma_prev)
+            forces = [C.forces_unsteady(s.verts, gd, gt, vv, s.areas, s.normals)
+                      for s, gd, gt, vv in zip(surfs, gdelta, gdt, vel)]
+            # coeff_cl_multibody / coeff_cm_multibody (vlm/src/vlm_backend.cpp:71-135)
+            la = np.array([0.0, 0.0, 1.0], dtype=T)
+            cl = cmy = ta = tm = 0.0
+            for s, f in zip(surfs, forces):
+                area = C.sum_(s.areas)
+                mac = C.mesh_mac(s.verts, s.areas)
+                cl += C.coeff_cl(f, la, fs, vars.rho, area) * area
+                cmy += C.coeff_cm(f, s.verts, ref, fs, vars.rho, area, mac)[1] * area * mac
+                ta += area
+                tm += mac
+            cl, cmy = cl / ta, cmy / (ta * tm)
+            F[0, i + 1] = T(-cl / (float(PI_F) * vars.mu))
+            F[1, i + 1] = T(2.0 * cmy / (float(PI_F) * vars.mu * vars.r_a ** 2))
+            dF = np.array([F[0, i + 1] - F[0, i] - kh * du[0],
+                           F[1, i + 1] - F[1, i] - ka * T(torsional_func(u[1, i + 1]) - torsional_func(u[1, i]))],
+                          dtype=T)
+            du_k = du_k - du
+            it += 1
+            if it > 50:
+                break
+        iters.append(it)
+    steps = np.arange(n_loop + 1) * float(dt_nd)
+    return {"t": steps, "u": u[:, : n_loop + 1], "v": v[:, : n_loop + 1], "a": a[:, : n_loop + 1],
+            "F": F[:, : n_loop + 1], "iters": np.array(iters), "dt": float(dt_nd), "t_steps": t_steps}
