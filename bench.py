@@ -26,6 +26,15 @@ from U[-5, 10] deg, seed 0: seeded FP64 influence matrices, one batched FP32 LU,
 against the plain FP64 path (one request at a time, exact FP64 assembly, FP64 LU), with |dCL| between
 them on the same angles, per-batch latency, and one CPU solve of the oracle.
 
+Also reported under "hb": the harmonic-balance driver (the libhbvlm3 interface) on a rectangular
+wing/flap section of chord 2b = 0.254 and span 8 chords (wing 20 + flap 10 chordwise x 20 spanwise =
+600 panels, H = 5: 11 time instances), heave 0.05 c and pitch 1 deg, omega = 0.8, as a continuation
+over airspeeds U = linspace(0.5, 1.5) * 6 (each solve warm-started from the previous one), with one
+cold-start CPU solve of the oracle and the load difference against it.  Under "two_dof": the
+coupled pitch/heave driver (tests/uvlm_2dof.cpp) on the reference's wing/flap section at U_a = 7 in
+FP32, time steps per second (each step runs its Newton load iterations), against the oracle's; under
+"two_dof_600" the same on a 600-panel wing/flap (20 + 10 chordwise x 20 spanwise, half span 4).
+
 --probe runs one served simulation without graph replay and one 1 GiB device copy (the
 calibration kernel for HBM byte counters), for profiler counter passes.
 """
@@ -175,6 +184,130 @@ def steady_bench(batch, reps, plain_n, with_cpu):
     return out
 
 
+def write_plot3d(path, xs, ys):
+    """Single-block 2-D plot3d file of the flat grid xs (chordwise) x ys (spanwise), z = 0."""
+    ni, nj = len(xs), len(ys)
+    with open(path, "w") as f:
+        f.write(f"1\n{ni} {nj} 1\n")
+        for plane in (np.tile(xs, nj), np.repeat(ys, ni), np.zeros(ni * nj)):
+            f.write("\n".join(f"{v:.17g}" for v in plane) + "\n")
+
+
+HB_B = 0.127  # semi-chord of the HB binding (lib/libhbvlm3.cpp:560)
+HB_H = 5
+HB_OMEGA = 0.8
+HB_U_REF = 6.0
+
+
+def hb_bench(n_speeds, with_cpu):
+    import tempfile
+
+    import torch
+
+    from tinyvlm_amd.hb import HBVLM
+
+    d = tempfile.mkdtemp(prefix="tvlm_hb_")
+    chord, half_span = 2 * HB_B, 4 * 2 * HB_B
+    ys = np.linspace(0.0, half_span, 21)
+    wing, flap = os.path.join(d, "wing_20x20.x"), os.path.join(d, "flap_10x20.x")
+    write_plot3d(wing, np.linspace(0.0, chord * 2 / 3, 21), ys)
+    write_plot3d(flap, np.linspace(chord * 2 / 3, chord, 11), ys)
+    K = 2 * HB_H + 1
+    X = np.zeros((3, K))
+    X[0, 1] = 0.05 * 2.0  # heave 0.05 c = 0.1 b
+    X[1, 2] = math.radians(1.0)  # pitch 1 deg
+    speeds = np.linspace(0.5, 1.5, n_speeds) * HB_U_REF
+    hb = HBVLM("hip", [wing, flap])
+    hb.init(HB_H, 1.0 / HB_B)
+    F_cold = np.zeros((3, K))
+    hb.run(HB_OMEGA, float(speeds[0]), X, F_cold)  # cold start at the first airspeed (also warm-up)
+    hb.init(HB_H, 1.0 / HB_B)
+    torch.cuda.synchronize()
+    lat, iters = [], []
+    t0 = time.perf_counter()
+    for U in speeds:
+        t1 = time.perf_counter()
+        F = np.zeros((3, K))
+        hb.run(HB_OMEGA, float(U), X, F)
+        lat.append((time.perf_counter() - t1) * 1e3)
+        iters.append(hb.iterations)
+    wall = time.perf_counter() - t0
+    out = {
+        "workload": "hb_rect_wing_flap_20+10c_x_20s_H5", "panels": hb.N, "harmonics": HB_H, "time_instances": K,
+        "value": round(n_speeds / wall, 3), "unit": "HB solves/s", "airspeeds": n_speeds,
+        "latency_ms_per_solve": {"p50": round(float(np.percentile(lat, 50)), 2),
+                                 "p99": round(float(np.percentile(lat, 99)), 2)},
+        "anderson_iterations": iters, "dtype": "f64",
+        "note": "continuation: each airspeed warm-started from the previous solution; launch-bound driver",
+    }
+    if with_cpu:
+        from oracle import hb as ohb
+        from oracle import cpuref as C
+
+        o = ohb.HBVLMOracle([wing, flap])
+        o.init(HB_H, 1.0 / HB_B)
+        Fo = np.zeros((3, K))
+        t0 = time.perf_counter()
+        o.run(HB_OMEGA, float(speeds[0]), X, Fo)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(1.0 / el, 5), "unit": "HB solves/s", "cores": C.lib().orc_num_threads(),
+                               "kind": "port", "sample": f"one cold-start oracle HBVLM::run at the first airspeed "
+                                                         f"({el:.1f} s, {o.iterations} Anderson iterations)"}
+        out["accuracy"] = {"max_abs_dF_cold_gpu_vs_cpu": float(np.max(np.abs(F_cold - Fo))),
+                           "max_abs_F": float(np.max(np.abs(Fo)))}
+    return out
+
+
+TWO_DOF_MESHES = [("mesh/infinite_rectangular_2x2_c0.x", False), ("mesh/infinite_rectangular_2x2_c1.x", True)]
+
+
+def two_dof_bench(steps, with_cpu, large=False):
+    """The reference's 2 x 2 + 2 x 2 section, or (large) a chord-1 wing/flap of 20 + 10 chordwise x
+    20 spanwise panels over a half span of 4 chords (600 panels)."""
+    import tempfile
+
+    from tinyvlm_amd.two_dof import UVLM2DOF, Vars
+
+    rel = TWO_DOF_MESHES
+    if large:
+        d = tempfile.mkdtemp(prefix="tvlm_2dof_")
+        ys = np.linspace(0.0, 4.0, 21)
+        write_plot3d(os.path.join(d, "wing_20x20.x"), np.linspace(0.0, 2.0 / 3.0, 21), ys)
+        write_plot3d(os.path.join(d, "flap_10x20.x"), np.linspace(2.0 / 3.0, 1.0, 11), ys)
+        rel = [(os.path.join(d, "wing_20x20.x"), False), (os.path.join(d, "flap_10x20.x"), True)]
+    meshes = [(os.path.join(ROOT, p), l) for p, l in rel]
+    sim = UVLM2DOF("hip", meshes, dtype=np.float32)
+    sim.run(Vars(U_a=7.0), 1000.0, max_steps=5)  # warm-up
+    t0 = time.perf_counter()
+    r = sim.run(Vars(U_a=7.0), 1000.0, max_steps=steps)
+    wall = time.perf_counter() - t0
+    name = "uvlm_2dof_wing_flap_20+10c_x_20s_Ua7_fp32" if large else "uvlm_2dof_wing_flap_2x2+2x2_Ua7_fp32"
+    out = {"workload": name, "panels": sim.N, "steps": steps,
+           "value": round(steps / wall, 2), "unit": "coupled UVLM steps/s",
+           "newton_iterations_mean": round(float(np.mean(r["iters"])), 3), "dtype": "f32",
+           "note": "each step: Newmark increment + Newton load iterations, launch-bound driver"}
+    if with_cpu:
+        from oracle import two_dof as o2
+
+        cwd = os.getcwd()
+        os.chdir(ROOT)
+        try:
+            t0 = time.perf_counter()
+            o = o2.run(rel, o2.Vars(U_a=7.0), 1000.0, np.float32, max_steps=steps)
+            el = time.perf_counter() - t0
+        finally:
+            os.chdir(cwd)
+        n = min(60, steps)
+        from oracle import cpuref as C
+
+        out["cpu_baseline"] = {"value": round(steps / el, 2), "unit": "coupled UVLM steps/s",
+                               "cores": C.lib().orc_num_threads(),
+                               "kind": "port", "sample": f"oracle FP32 restatement, {steps} steps ({el:.2f} s)"}
+        out["accuracy"] = {f"max_rel_du_first_{n}_steps_vs_cpu_fp32": float(
+            np.max(np.abs(r["u"][:, :n].astype(np.float64) - o["u"][:, :n])) / np.max(np.abs(o["u"][:, :n])))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +321,11 @@ def main():
     ap.add_argument("--steady-reps", type=int, default=3, help="timed served steady batches")
     ap.add_argument("--steady-plain", type=int, default=4, help="timed plain FP64 steady solves")
     ap.add_argument("--no-steady", action="store_true")
+    ap.add_argument("--hb-speeds", type=int, default=8, help="airspeeds of the timed HB continuation")
+    ap.add_argument("--no-hb", action="store_true")
+    ap.add_argument("--two-dof-steps", type=int, default=400, help="timed coupled pitch/heave steps")
+    ap.add_argument("--two-dof-large-steps", type=int, default=100, help="timed steps of the 600-panel section")
+    ap.add_argument("--no-two-dof", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -301,6 +439,11 @@ def main():
     plain.close()
     if not args.no_steady:
         out["steady"] = steady_bench(args.steady_batch, args.steady_reps, args.steady_plain, not args.no_cpu)
+    if not args.no_hb:
+        out["hb"] = hb_bench(args.hb_speeds, not args.no_cpu)
+    if not args.no_two_dof:
+        out["two_dof"] = two_dof_bench(args.two_dof_steps, not args.no_cpu)
+        out["two_dof_600"] = two_dof_bench(args.two_dof_large_steps, not args.no_cpu, large=True)
     print(json.dumps(out))
 
 

@@ -140,7 +140,7 @@ class UVLM2DOF:
         gdelta = [z(ns, nc) for ns, nc in dims]
         gdt = [z(ns, nc) for ns, nc in dims]
         gprev = [z(ns, nc) for ns, nc in dims]
-        scal = torch.zeros(16, dtype=tdt, device="cuda")
+        scal = torch.zeros(6 * len(dims), dtype=tdt, device="cuda")
         a_h = float(T(vars.a_h))
         init_angle = float(T(3.0) * PI_F / T(180.0))  # to_radians(3.f)
         T0, _ = _rotation_y(1.0 + a_h, init_angle, 0.0)
@@ -259,17 +259,25 @@ class UVLM2DOF:
 
     @staticmethod
     def _coefficients(forces, verts, areas, ref, fs, la, rho, scal):
-        """coeff_cl_multibody / coeff_cm_multibody (vlm/src/vlm_backend.cpp:71-135)."""
+        """coeff_cl_multibody / coeff_cm_multibody (vlm/src/vlm_backend.cpp:71-135).  The areas and
+        MACs of all surfaces are read back together, then all CL/CM together: two host reads."""
+        n = len(forces)
+        for m in range(n):
+            H.sum_(areas[m], scal[2 * m: 2 * m + 1])
+            H.mesh_mac(verts[m], areas[m], scal[2 * m + 1: 2 * m + 2])
+        am = scal[: 2 * n].tolist()
+        base = 2 * n
+        for m in range(n):
+            area, mac = am[2 * m], am[2 * m + 1]
+            o = base + 4 * m
+            H.coeff_cl(forces[m], la, fs, rho, area, scal[o: o + 1])
+            H.coeff_cm(forces[m], verts[m], ref, fs, rho, area, mac, scal[o + 1: o + 4])
+        co = scal[base: base + 4 * n].tolist()
         cl = cmy = ta = tm = 0.0
-        for f, vv, ar in zip(forces, verts, areas):
-            H.sum_(ar, scal[0:1])
-            H.mesh_mac(vv, ar, scal[1:2])
-            area, mac = scal[0:2].tolist()
-            H.coeff_cl(f, la, fs, rho, area, scal[2:3])
-            H.coeff_cm(f, vv, ref, fs, rho, area, mac, scal[3:6])
-            c, _, cy, _ = scal[2:6].tolist()
-            cl += c * area
-            cmy += cy * area * mac
+        for m in range(n):
+            area, mac = am[2 * m], am[2 * m + 1]
+            cl += co[4 * m] * area
+            cmy += co[4 * m + 2] * area * mac
             ta += area
             tm += mac
         return cl / ta, cmy / (ta * tm)
