@@ -435,10 +435,11 @@ def test_error_status_no_launch(H, torch):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("n,batch", [(130, 5), (1500, 3)])
+@pytest.mark.parametrize("n,batch", [(130, 5), (1500, 3), (200, 9), (1100, 8), (2200, 8)])
 def test_lu_batched(H, torch, prec, n, batch):
     """A batch of independent matrices in one getrf/getrs pass equals factoring each alone
-    (LAPACK pivots for FP64, per-matrix info, solutions to the single-matrix bar)."""
+    (LAPACK pivots for FP64, per-matrix info, solutions to the single-matrix bar).  Batches of 8 or
+    more take the look-ahead factorisation (64-column panels, split trailing updates)."""
     import scipy.linalg as sl
 
     print(f"[case] lu batched n={n} batch={batch} {prec}", flush=True)

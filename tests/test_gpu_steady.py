@@ -59,7 +59,8 @@ def test_served_plan_matches_plain(root, oracle, mesh):
     from tinyvlm_amd.steady_plan import MODE_PLAIN_FP64, MODE_SERVED
 
     v = oracle.read_mesh(os.path.join(root, "mesh", mesh))
-    alphas = [_alpha(d) for d in (-5.0, -1.0, 0.5, 2.0, 5.0, 7.5, 10.0)]
+    # 9 requests: the batched FP32 LU takes its look-ahead form (batches of 8 or more)
+    alphas = [_alpha(d) for d in (-5.0, -1.0, 0.5, 2.0, 3.0, 5.0, 6.0, 7.5, 10.0)]
     print(f"[case] steady served vs plain {mesh}", flush=True)
     rp = _run([v], alphas, MODE_PLAIN_FP64)
     rs = _run([v], alphas, MODE_SERVED)

@@ -839,6 +839,81 @@ __global__ void k_zero_info_batch(int32_t* info, int batch) {
 
 // shape[2] > 1 factors a batch of matrices, matrix b at ptr + b * stride[2]; ipiv holds n entries
 // per matrix (matrix b at ipiv + b n), info one per matrix.
+// Side stream and events of the batched look-ahead, one set per device, created on first use
+// (default priority, non-blocking).
+struct LookaheadStreams {
+    hipStream_t side = nullptr;
+    hipEvent_t narrow_done = nullptr, rest_done = nullptr;
+};
+
+int lookahead_streams(LookaheadStreams** out) {
+    static std::mutex mu;
+    static LookaheadStreams per_dev[64];
+    int dev = 0;
+    TVLM_HIP(hipGetDevice(&dev));
+    TVLM_REQUIRE(dev >= 0 && dev < 64, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    LookaheadStreams& L = per_dev[dev];
+    if (!L.side) {
+        TVLM_HIP(hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking));
+        TVLM_HIP(hipEventCreateWithFlags(&L.narrow_done, hipEventDisableTiming));
+        TVLM_HIP(hipEventCreateWithFlags(&L.rest_done, hipEventDisableTiming));
+    }
+    *out = &L;
+    return TVLM_OK;
+}
+
+// Batched getrf with a one-panel look-ahead (batches of kWidePanelBatch or more, n <= kFastPanelRows,
+// 64-column panels).  After panel k's interchanges and U12 (k_swap_trsm), the trailing update is
+// split: the next panel's 64 columns are updated first on the caller's stream, which then factors
+// panel k+1 at once, while the rest of the trailing update runs on a side stream; the next
+// k_swap_trsm waits for both.  The panels (one work-group per matrix) thus run beside the
+// bandwidth-bound updates instead of between them.  Same arithmetic as the plain loop.
+template <typename T>
+int getrf_lookahead(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* ipiv, int32_t* info, int32_t* perm,
+                    hipStream_t st) {
+    LookaheadStreams* L = nullptr;
+    TVLM_TRY(lookahead_streams(&L));
+    auto swap_trsm = [&](int kp, int kb) -> int {
+        const int j1 = kp + kb;
+        const int nleft = (kp + 63) / 64, nright = (n - j1 + 63) / 64;
+        if (nleft + nright > 0) {
+            hipLaunchKernelGGL(k_swap_trsm<T>, dim3(nleft + nright, 1, batch), dim3(256), 0, st, a, lda, n, kp, kb,
+                               nleft, perm, sb);
+            TVLM_LAUNCHED();
+        }
+        return TVLM_OK;
+    };
+    int kp = 0, kb = n < NB ? n : NB;
+    TVLM_TRY(panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb));
+    TVLM_TRY(swap_trsm(kp, kb));
+    while (kp + kb < n) {
+        const int j1 = kp + kb, m = n - j1;
+        const int nb = m < NB ? m : NB;
+        const T* L21 = a + j1 + (int64_t)kp * lda;
+        const T* U12 = a + kp + (int64_t)j1 * lda;
+        launch_update<T>(a + j1 + (int64_t)j1 * lda, lda, L21, lda, U12, lda, m, nb, kb, sb, batch, st);
+        TVLM_LAUNCHED();
+        const bool rest = m > nb;
+        if (rest) {
+            TVLM_HIP(hipEventRecord(L->narrow_done, st));
+            TVLM_HIP(hipStreamWaitEvent(L->side, L->narrow_done, 0));
+        }
+        TVLM_TRY(panel_fast<T>(a, lda, n, j1, nb, ipiv, info, perm, st, batch, sb));
+        if (rest) {
+            launch_update<T>(a + j1 + (int64_t)(j1 + nb) * lda, lda, L21, lda, U12 + (int64_t)nb * lda, lda, m, m - nb,
+                             kb, sb, batch, L->side);
+            TVLM_LAUNCHED();
+            TVLM_HIP(hipEventRecord(L->rest_done, L->side));
+            TVLM_HIP(hipStreamWaitEvent(st, L->rest_done, 0));
+        }
+        TVLM_TRY(swap_trsm(j1, nb));
+        kp = j1;
+        kb = nb;
+    }
+    return TVLM_OK;
+}
+
 template <typename T>
 int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     TVLM_REQUIRE(av && av->ptr && ipiv && info, "null argument");
@@ -858,6 +933,7 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     void* ws = nullptr;
     TVLM_TRY(device_workspace(sizeof(int32_t) * kPermInts * (size_t)batch, &ws, kWsLuPerm));
     int32_t* perm = static_cast<int32_t*>(ws);
+    if (batch >= kWidePanelBatch && n <= kFastPanelRows) return getrf_lookahead<T>(a, lda, n, batch, sb, ipiv, info, perm, st);
     for (int k0 = 0; k0 < n;) {
         // tall panels of one matrix are narrower: the in-panel left-looking traffic grows with width
         // squared.  A batch is bound by the trailing updates instead, whose traffic (the whole
