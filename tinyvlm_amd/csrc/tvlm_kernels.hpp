@@ -405,11 +405,13 @@ __global__ __launch_bounds__(kThreads) void k_wake_lattice(TS* __restrict__ part
 }
 
 // Tile shape for k_wake_lattice: sw <= 4 * kLatChunkMax spans, rh in {1, 2, 4} rows, chosen so
-// that the launch has about kTargetGroups work-groups.
+// that the launch has about kLatTargetGroups work-groups.
+constexpr int kLatTargetGroups = 1024;
+
 inline void lattice_tile_shape(int ns, int rows, int recv_blocks, int& sw, int& rh) {
     sw = ns < kWaves * kLatChunkMax ? ns : kWaves * kLatChunkMax;
     const int tiles_span = (ns + sw - 1) / sw;
-    int want_row_tiles = kTargetGroups / (recv_blocks * tiles_span);
+    int want_row_tiles = kLatTargetGroups / (recv_blocks * tiles_span);
     if (want_row_tiles < 1) want_row_tiles = 1;
     const int need = (rows + want_row_tiles - 1) / want_row_tiles;
     rh = need <= 1 ? 1 : need <= 2 ? 2 : 4;
