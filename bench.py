@@ -59,9 +59,9 @@ COPY_PROBE_BYTES = 1 << 30
 # FETCH_SIZE / WRITE_SIZE counter passes over `bench.py --probe`, each scaled by the same counter's
 # reading for a 1 GiB device copy (FETCH x 2.000, WRITE x 1.000); and the executed FP64 flops per
 # pair from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 pass.  Counters cannot be read in-process.
-WAKE_TRAFFIC_BYTES = 3282540
+WAKE_TRAFFIC_BYTES = 3008144
 WAKE_ALGORITHMIC_BYTES = 349384
-WAKE_EXECUTED_FLOPS_PER_PAIR = 286.7
+WAKE_EXECUTED_FLOPS_PER_PAIR = 252.7
 NS, NC, HALF_SPAN, CHORD = 40, 10, 2.0, 1.0
 T_STEPS = 501  # UVLM::run loops t_steps-1 times -> 500 time steps
 A_AMP = float(np.float32(2) * np.float32(3.141593) / np.float32(180))
