@@ -17,7 +17,9 @@
 // applies the interchanges and both blocked substitutions with the column in LDS
 // (k_solve_blocked); larger n uses block-row kernels (k_diag_solve / k_off_update).
 #include <mutex>
+#include <vector>
 
+#include <hip/hip_ext.h>
 #include <rocblas/rocblas.h>
 
 #include "tvlm_internal.hpp"
@@ -840,7 +842,10 @@ __global__ void k_zero_info_batch(int32_t* info, int batch) {
 // shape[2] > 1 factors a batch of matrices, matrix b at ptr + b * stride[2]; ipiv holds n entries
 // per matrix (matrix b at ipiv + b n), info one per matrix.
 // Side stream and events of the batched look-ahead, one set per device, created on first use
-// (default priority, non-blocking).
+// (default priority, non-blocking).  The side stream is created with an all-CUs mask: such a
+// stream gets a hardware queue of its own, so its updates can overlap the caller's panels whatever
+// queue the caller's stream shares (streams without a mask share a small pool of queues, and two
+// streams on one queue run one after the other).
 struct LookaheadStreams {
     hipStream_t side = nullptr;
     hipEvent_t narrow_done = nullptr, rest_done = nullptr;
@@ -855,7 +860,11 @@ int lookahead_streams(LookaheadStreams** out) {
     std::lock_guard<std::mutex> lock(mu);
     LookaheadStreams& L = per_dev[dev];
     if (!L.side) {
-        TVLM_HIP(hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking));
+        int cus = 0;
+        TVLM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        std::vector<uint32_t> mask((cus + 31) / 32, 0u);
+        for (int c = 0; c < cus; c++) mask[c / 32] |= 1u << (c % 32);
+        TVLM_HIP(hipExtStreamCreateWithCUMask(&L.side, (uint32_t)mask.size(), mask.data()));
         TVLM_HIP(hipEventCreateWithFlags(&L.narrow_done, hipEventDisableTiming));
         TVLM_HIP(hipEventCreateWithFlags(&L.rest_done, hipEventDisableTiming));
     }
