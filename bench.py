@@ -26,6 +26,9 @@ from U[-5, 10] deg, seed 0: seeded FP64 influence matrices, one batched FP32 LU,
 against the plain FP64 path (one request at a time, exact FP64 assembly, FP64 LU), with |dCL| between
 them on the same angles, per-batch latency, and one CPU solve of the oracle.
 
+Also reported under "assembly": the influence-matrix assembly of the same elliptic mesh in FP32
+(rsqrt form) and FP64 (exact form), TFLOP/s at 398 flops per pair against the VALU rates.
+
 Also reported under "hb": the harmonic-balance driver (the libhbvlm3 interface) on a rectangular
 wing/flap section of chord 2b = 0.254 and span 8 chords (wing 20 + flap 10 chordwise x 20 spanwise =
 600 panels, H = 5: 11 time instances), heave 0.05 c and pitch 1 deg, omega = 0.8, as a continuation
@@ -181,6 +184,48 @@ def steady_bench(batch, reps, plain_n, with_cpu):
         out["cpu_baseline"] = {"value": round(1.0 / el, 4), "unit": "steady solves/s", "cores": C.lib().orc_num_threads(),
                                "kind": "port", "sample": f"one oracle FP64 VLM::run at alpha 5 deg ({el:.2f} s)",
                                "abs_dCL_plain_gpu_vs_cpu": abs(float(res_p["cl"][0]) - cl_cpu)}
+    return out
+
+
+PEAK_FP32_VALU_PACKED_TFLOPS = 157.3  # MI355X FP32 vector peak with packed FP32 (not used: see DESIGN)
+
+
+def assembly_bench(reps=5):
+    """Influence-matrix assembly (tvlm_lhs_assemble_*, the reference's lhs_assemble) of the elliptic
+    64 x 64 mesh (N = 4096, 16.8 M receiver x ring pairs, 398 flops each) in FP32 (rsqrt form) and
+    FP64 (exact form), timed with HIP events on torch's current stream."""
+    import torch
+
+    from tinyvlm_amd import capi as H
+    from tinyvlm_amd.mesh_io import read_plot3d
+
+    v = read_plot3d(STEADY_MESH)
+    ns, nc = v.shape[0] - 1, v.shape[1] - 1
+    N = ns * nc
+    out = {"workload": "lhs_assemble_elliptic_64x64_N4096", "pairs": N * N, "flops_per_pair": FLOPS_PER_PAIR}
+    for name, dt, peak in (("f32", torch.float32, PEAK_FP64_VALU_TFLOPS), ("f64", torch.float64, PEAK_FP64_VALU_TFLOPS)):
+        verts = [H.to_device(v, dt)]
+        colloc, normals, areas = [H.colmajor((ns, nc, 3), dt)], [H.colmajor((ns, nc, 3), dt)], [H.colmajor((ns, nc), dt)]
+        H.mesh_metrics(0.0, verts, colloc, normals, areas)
+        wake = [H.colmajor((ns + 1, 2, 4), dt)]
+        lhs = H.colmajor((N, N), dt)
+        H.lhs_assemble(lhs, colloc, normals, verts, wake, 0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            H.lhs_assemble(lhs, colloc, normals, verts, wake, 0)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        tf = N * N * FLOPS_PER_PAIR / (ms * 1e-3) / 1e12
+        out[name] = {"ms": round(ms, 4), "achieved_tflops": round(tf, 2), "peak_tflops": peak,
+                     "frac": round(tf / peak, 4)}
+        if name == "f32":
+            out[name]["frac_of_packed_fp32_peak"] = round(tf / PEAK_FP32_VALU_PACKED_TFLOPS, 4)
+        del lhs
+    out["note"] = ("FP32 rsqrt form against the non-packed FP32 VALU rate (78.6 TFLOP/s: packed FP32 "
+                   "instructions are switched off in every kernel); FP64 exact form against the FP64 VALU peak")
     return out
 
 
@@ -439,6 +484,8 @@ def main():
     plain.close()
     if not args.no_steady:
         out["steady"] = steady_bench(args.steady_batch, args.steady_reps, args.steady_plain, not args.no_cpu)
+    if not args.no_steady:
+        out["assembly"] = assembly_bench()
     if not args.no_hb:
         out["hb"] = hb_bench(args.hb_speeds, not args.no_cpu)
     if not args.no_two_dof:
