@@ -57,6 +57,14 @@ const char* tvlm_last_error(void);
 /* create_backend name registry (vlm/src/vlm_backend.cpp:15-38): "hip" */
 const char* tvlm_backend_name(void);
 int tvlm_stream_synchronize(void* stream);
+/* Streams for callers that run several plans at once (no counterpart in the reference, whose
+ * backends use one implicit stream): tvlm_stream_create makes a default-priority stream with a
+ * hardware queue of its own (all-CUs mask; unmasked streams share a small pool of queues and two
+ * streams on one queue run one after the other).  tvlm_stream_query sets *done = 1 once all
+ * work queued on the stream has finished, 0 while some is still running; it does not block. */
+int tvlm_stream_create(void** stream);
+int tvlm_stream_destroy(void* stream);
+int tvlm_stream_query(void* stream, int32_t* done);
 
 /* ---- Memory (vlm/include/vlm_memory.hpp:22-35) ---------------------------------------------- */
 /* location: 0 = Device, 1 = Host (pinned) -- enum class Location { Device, Host } */
@@ -210,8 +218,9 @@ int tvlm_plot3d_read(const char* path, int32_t quarter_chord, int32_t parse_f64,
  * The caller of the hot path VLM::run (vlm/src/vlm.cpp:97-151) for `batch` angles of attack on one
  * mesh, every step on the device.  mode 0: plain FP64, requests one after another (exact FP64
  * assembly, FP64 LU + substitution).  mode 1: served -- the FP64 influence matrices of all requests
- * in the seeded form, one batched FP32 LU, FP64 iterative refinement against the FP64 matrix
- * (at most 8192 panels).  CM is taken about cm_ref moved with the wing (the reference returns 0). */
+ * in the seeded form, one batched FP32 LU, FP64 iterative refinement against the FP64 matrix.
+ * CM is taken about cm_ref moved with the wing (the reference returns 0).  Plans on different
+ * streams may run concurrently (the library's scratch buffers are per stream). */
 typedef struct tvlm_vlm_plan tvlm_vlm_plan;
 /* verts_init[m]: host {ns+1, nc+1, 4} column-major double (w = 1), e.g. a plot3d mesh after the
  * quarter-chord shift */
@@ -221,9 +230,22 @@ int tvlm_vlm_create(tvlm_vlm_plan** plan, int32_t n_surf, const int64_t* ns, con
  * asynchronous on the plan's stream */
 int tvlm_vlm_solve(tvlm_vlm_plan* plan, const double* alpha, const double* beta, double u_inf,
                    double rho, const double* cm_ref);
-/* synchronises; cl[batch], cm[batch][3], gamma[batch][N], info[batch] (getrf status per request);
- * null pointers are skipped */
+/* the first n_active (1..batch) requests only: alpha, beta host [n_active]; a serving loop fills a
+ * plan with however many requests of its mesh are waiting */
+int tvlm_vlm_solve_n(tvlm_vlm_plan* plan, int32_t n_active, const double* alpha, const double* beta, double u_inf,
+                     double rho, const double* cm_ref);
+/* waits for the plan's last solve; cl[n], cm[n][3], gamma[n][N], info[n] (getrf status per request)
+ * for the n requests of the last solve; null pointers are skipped */
 int tvlm_vlm_results(tvlm_vlm_plan* plan, double* cl, double* cm, double* gamma, int32_t* info);
+/* moves the plan to another stream: its later solves queue there after its own latest work (a
+ * device-side wait; the host does not block).  A serving loop keeps plans per mesh and runs each
+ * batch on whichever stream is free.  Not while a solve of the plan is being queued elsewhere. */
+int tvlm_vlm_set_stream(tvlm_vlm_plan* plan, void* stream);
+/* *done = 1 once the plan's latest solve (or creation) has finished on the device, else 0; does
+ * not block */
+int tvlm_vlm_query(const tvlm_vlm_plan* plan, int32_t* done);
+/* device bytes the plan holds (its admission cost in a serving loop) */
+int tvlm_vlm_footprint(const tvlm_vlm_plan* plan, int64_t* device_bytes);
 int tvlm_vlm_destroy(tvlm_vlm_plan* plan);
 
 #ifdef __cplusplus

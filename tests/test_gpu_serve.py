@@ -1,0 +1,132 @@
+"""GPU checks of continuous batching (tinyvlm_amd.serve.SteadyServer) on the steady-solve mix.
+
+- alone vs in the mix: every request served in a mix of meshes, arriving while others run,
+  equals the same request solved alone (a one-request served plan): circulation within 1e-10
+  relative to its largest value, |dCL| <= 1e-10, CM within 1e-10;
+- against the plain FP64 path and the oracle (VLM::run restatement) for requests on the smallest
+  mesh: circulation within 1e-10 relative, |dCL| <= 1e-10;
+- overlap: requests were admitted while others were still on the device (more than one lane busy
+  at once, batches of more than one request);
+- memory admission: with a budget that holds only two plans, plans are evicted and re-created and
+  every result is still the alone result.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def mix(tmp_path_factory):
+    from tinyvlm_amd import capi as H
+    from tinyvlm_amd import serve as S
+
+    H.init(0)
+    d = str(tmp_path_factory.mktemp("mix"))
+    t = S.read_trace()
+    meshes, reqs = {}, []
+    for i in range(len(t["gap"])):
+        n = int(t["n"][i])
+        if n > 45:
+            continue
+        key = f"{S.MIX_FAMILIES[t['family'][i]]}_{n}"
+        if key not in meshes:
+            meshes[key] = [S.mix_mesh(S.MIX_FAMILIES[t["family"][i]], n, d)]
+        reqs.append((i, key, float(np.deg2rad(np.float64(t["alpha_deg"][i]))), float(t["gap"][i])))
+        if len(reqs) == 72:
+            break
+    return meshes, reqs
+
+
+def _requests(reqs, gap_scale):
+    from tinyvlm_amd.serve import Request
+
+    out, t = [], 0.0
+    for rid, key, a, g in reqs:
+        t += g * gap_scale
+        out.append(Request(rid, key, a, arrival=t))
+    return out
+
+
+def _alone(meshes, r):
+    from tinyvlm_amd.steady_plan import MODE_SERVED, SteadyPlan
+
+    p = SteadyPlan(meshes[r.mesh], 1, mode=MODE_SERVED)
+    p.solve([r.alpha])
+    res = p.results()
+    p.close()
+    return res
+
+
+def _check_same(r, alone):
+    g = alone["gamma"][0]
+    assert r.info == 0
+    assert np.max(np.abs(r.gamma - g)) <= TOL * np.max(np.abs(g)), r.rid
+    assert abs(r.cl - alone["cl"][0]) <= TOL, r.rid
+    assert np.max(np.abs(r.cm - alone["cm"][0])) <= TOL, r.rid
+
+
+def test_mix_equals_alone_and_overlaps(mix):
+    from tinyvlm_amd.serve import SteadyServer
+
+    meshes, reqs = mix
+    print(f"[case] serve mix {len(reqs)} requests over {len(meshes)} meshes", flush=True)
+    requests = _requests(reqs, 2e-3)
+    with SteadyServer(meshes, lanes=4, max_batch=8, keep_gamma=True) as srv:
+        st = srv.run(requests)
+    assert all(np.isfinite(r.finished) for r in requests)
+    assert st.max_running >= 2 and max(st.batch_sizes) >= 2, (st.max_running, st.batch_sizes)
+    # some request was admitted while another one, admitted earlier, was still running
+    spans = sorted((r.admitted, r.finished, r.lane) for r in requests)
+    assert any(b[0] < a[1] and b[2] != a[2] for a, b in zip(spans, spans[1:]))
+    for r in requests:
+        assert r.admitted >= r.arrival - 1e-9 and r.finished > r.admitted
+        _check_same(r, _alone(meshes, r))
+
+
+def test_mix_smallest_mesh_matches_plain_and_oracle(mix, oracle):
+    from tinyvlm_amd.serve import SteadyServer
+    from tinyvlm_amd.steady_plan import MODE_PLAIN_FP64, SteadyPlan
+
+    meshes, reqs = mix
+    requests = _requests(reqs, 0.0)
+    with SteadyServer(meshes, lanes=3, max_batch=16, keep_gamma=True) as srv:
+        srv.run(requests, realtime=False)
+    small = [r for r in requests if r.mesh == "elliptic_16"][:4]
+    assert small
+    print(f"[case] serve vs plain/oracle elliptic_16 x {len(small)}", flush=True)
+    p = SteadyPlan(meshes["elliptic_16"], len(small), mode=MODE_PLAIN_FP64)
+    p.solve([r.alpha for r in small])
+    plain = p.results()
+    p.close()
+    for j, r in enumerate(small):
+        g = plain["gamma"][j]
+        assert np.max(np.abs(r.gamma - g)) <= TOL * np.max(np.abs(g))
+        assert abs(r.cl - plain["cl"][j]) <= TOL
+        go, clo, _, _ = oracle.vlm_run(meshes["elliptic_16"], r.alpha)
+        assert np.max(np.abs(r.gamma - go)) <= TOL * np.max(np.abs(go))
+        assert abs(r.cl - clo) <= TOL
+
+
+def test_mix_under_small_memory_budget(mix):
+    from tinyvlm_amd.serve import SteadyServer
+
+    meshes, reqs = mix
+    reqs = reqs[:36]
+    sizes = {}
+    with SteadyServer(meshes, lanes=2, max_batch=4, budget_bytes=1 << 40) as probe:
+        for k in meshes:
+            sizes[k] = probe._estimate(k)
+    budget = 2 * max(sizes.values()) + 1
+    print(f"[case] serve mix with budget {budget} bytes", flush=True)
+    requests = _requests(reqs, 1e-3)
+    with SteadyServer(meshes, lanes=2, max_batch=4, budget_bytes=budget, keep_gamma=True) as srv:
+        st = srv.run(requests)
+        assert srv.used_bytes <= budget
+    assert st.plans_evicted > 0 and st.peak_bytes <= budget
+    for r in requests:
+        _check_same(r, _alone(meshes, r))

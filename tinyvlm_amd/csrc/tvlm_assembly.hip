@@ -152,8 +152,8 @@ int wake_influence(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_
     wt.total_tiles = tiles;
     if (tiles == 0 || total == 0) return TVLM_OK;
     void* ws = nullptr;
-    TVLM_TRY(device_workspace(sizeof(T) * (size_t)tiles * (size_t)total, &ws, kWsWakePartials));
     hipStream_t st = as_stream(stream);
+    TVLM_TRY(device_workspace(sizeof(T) * (size_t)tiles * (size_t)total, &ws, kWsWakePartials, st));
     dim3 grid((total + kRecv - 1) / kRecv, tiles);
     if constexpr (sizeof(T) == 4) {
         if (sigma != T(0))

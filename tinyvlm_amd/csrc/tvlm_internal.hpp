@@ -49,12 +49,17 @@ inline bool plane_contiguous(const tvlm_view& v) {
     return v.ptr != nullptr && v.stride[0] == 1 && v.stride[1] == v.shape[0];
 }
 
-// Per-device scratch buffers owned by the library (grow-only), one per slot: wake-induction tile
-// partials, LU row-interchange lists, triangular-solve block inverses.  Not safe for concurrent
-// callers on one device (two streams using the same slot), like the reference's CUDA scratch
-// scalars (vlm_backend_cuda.hpp:45-54).
+// Scratch buffers owned by the library (grow-only), one per device, slot and stream: wake-induction
+// tile partials, LU row-interchange lists, triangular-solve block inverses.  Work queued on
+// different streams never shares a buffer, so concurrent solves on separate streams are safe;
+// work on one stream is ordered by the stream.
 enum WorkspaceSlot { kWsWakePartials = 0, kWsLuPerm = 1, kWsSolveInv = 2, kWorkspaceSlots = 3 };
-int device_workspace(size_t bytes, void** out, int slot);
+int device_workspace(size_t bytes, void** out, int slot, hipStream_t stream);
+
+// A stream with a hardware queue of its own: created with an all-CUs mask (streams created
+// without a mask share a small pool of queues, and two streams on one queue run one after the
+// other).  Default priority.
+int create_queue_stream(hipStream_t* out);
 
 // lhs_assemble in the FP64 seeded form (not part of the C ABI; the served steady solve uses it)
 int lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, const tvlm_view* normals,

@@ -16,6 +16,7 @@
 // diagonal blocks of L and U inverted (k_solve_prep), then one work-group per right-hand side
 // applies the interchanges and both blocked substitutions with the column in LDS
 // (k_solve_blocked); larger n uses block-row kernels (k_diag_solve / k_off_update).
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -700,28 +701,32 @@ __global__ __launch_bounds__(256) void k_update_mfma(T* __restrict__ C, int64_t 
 // The trailing update is a plain strided-batched GEMM, C -= A B with k = panel width; for
 // m >= kBlasUpdateRows it goes to rocBLAS (?gemm_strided_batched on the matrix cores, which moves
 // the trailing matrix at about 4 TB/s here), smaller updates stay in k_update_mfma.  One rocBLAS
-// handle per device, created on first use; calls are serialised because the handle's stream is
-// set per call.
+// handle per (device, stream), created on first use and bound to that stream: a handle owns a
+// device workspace, so GEMMs of plans running concurrently on different streams must not share one.
 constexpr int kBlasUpdateRows = 512;
 
-rocblas_handle blas_handle() {
+rocblas_handle blas_handle(hipStream_t st) {
     static std::mutex mu;
-    static rocblas_handle handles[64] = {};
+    static std::map<std::pair<int, hipStream_t>, rocblas_handle> handles;
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> lock(mu);
-    if (!handles[dev] && rocblas_create_handle(&handles[dev]) != rocblas_status_success) handles[dev] = nullptr;
-    return handles[dev];
+    rocblas_handle& h = handles[std::make_pair(dev, st)];
+    if (!h) {
+        if (rocblas_create_handle(&h) != rocblas_status_success) {
+            h = nullptr;
+            return nullptr;
+        }
+        if (rocblas_set_stream(h, st) != rocblas_status_success) return nullptr;
+    }
+    return h;
 }
 
 template <typename T>
 bool update_blas(T* C, int64_t ldc, const T* A, int64_t lda, const T* B, int64_t ldb, int m, int n, int kb, int64_t sb,
                  int batch, hipStream_t st) {
-    static std::mutex mu;
-    rocblas_handle h = blas_handle();
+    rocblas_handle h = blas_handle(st);
     if (!h) return false;
-    std::lock_guard<std::mutex> lock(mu);
-    if (rocblas_set_stream(h, st) != rocblas_status_success) return false;
     const T alpha = T(-1), beta = T(1);
     const rocblas_stride str = batch > 1 ? (rocblas_stride)sb : 0;
     rocblas_status rs;
@@ -841,30 +846,26 @@ __global__ void k_zero_info_batch(int32_t* info, int batch) {
 
 // shape[2] > 1 factors a batch of matrices, matrix b at ptr + b * stride[2]; ipiv holds n entries
 // per matrix (matrix b at ipiv + b n), info one per matrix.
-// Side stream and events of the batched look-ahead, one set per device, created on first use
-// (default priority, non-blocking).  The side stream is created with an all-CUs mask: such a
-// stream gets a hardware queue of its own, so its updates can overlap the caller's panels whatever
-// queue the caller's stream shares (streams without a mask share a small pool of queues, and two
-// streams on one queue run one after the other).
+// Side stream and events of the batched look-ahead, one set per (device, caller stream), created
+// on first use (default priority).  Per caller stream, so that factorisations running
+// concurrently on different streams each have their own side chain.  The side stream is created
+// with an all-CUs mask: such a stream gets a hardware queue of its own, so its updates can overlap
+// the caller's panels whatever queue the caller's stream shares (streams without a mask share a
+// small pool of queues, and two streams on one queue run one after the other).
 struct LookaheadStreams {
     hipStream_t side = nullptr;
     hipEvent_t narrow_done = nullptr, rest_done = nullptr;
 };
 
-int lookahead_streams(LookaheadStreams** out) {
+int lookahead_streams(hipStream_t caller, LookaheadStreams** out) {
     static std::mutex mu;
-    static LookaheadStreams per_dev[64];
+    static std::map<std::pair<int, hipStream_t>, LookaheadStreams> sets;
     int dev = 0;
     TVLM_HIP(hipGetDevice(&dev));
-    TVLM_REQUIRE(dev >= 0 && dev < 64, "device %d out of range", dev);
     std::lock_guard<std::mutex> lock(mu);
-    LookaheadStreams& L = per_dev[dev];
+    LookaheadStreams& L = sets[std::make_pair(dev, caller)];
     if (!L.side) {
-        int cus = 0;
-        TVLM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        std::vector<uint32_t> mask((cus + 31) / 32, 0u);
-        for (int c = 0; c < cus; c++) mask[c / 32] |= 1u << (c % 32);
-        TVLM_HIP(hipExtStreamCreateWithCUMask(&L.side, (uint32_t)mask.size(), mask.data()));
+        TVLM_TRY(create_queue_stream(&L.side));
         TVLM_HIP(hipEventCreateWithFlags(&L.narrow_done, hipEventDisableTiming));
         TVLM_HIP(hipEventCreateWithFlags(&L.rest_done, hipEventDisableTiming));
     }
@@ -882,7 +883,7 @@ template <typename T>
 int getrf_lookahead(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* ipiv, int32_t* info, int32_t* perm,
                     hipStream_t st) {
     LookaheadStreams* L = nullptr;
-    TVLM_TRY(lookahead_streams(&L));
+    TVLM_TRY(lookahead_streams(st, &L));
     auto swap_trsm = [&](int kp, int kb) -> int {
         const int j1 = kp + kb;
         const int nleft = (kp + 63) / 64, nright = (n - j1 + 63) / 64;
@@ -940,7 +941,7 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     hipLaunchKernelGGL(k_zero_info_batch<T>, dim3((batch + 255) / 256), dim3(256), 0, st, info, batch);
     TVLM_LAUNCHED();
     void* ws = nullptr;
-    TVLM_TRY(device_workspace(sizeof(int32_t) * kPermInts * (size_t)batch, &ws, kWsLuPerm));
+    TVLM_TRY(device_workspace(sizeof(int32_t) * kPermInts * (size_t)batch, &ws, kWsLuPerm, st));
     int32_t* perm = static_cast<int32_t*>(ws);
     if (batch >= kWidePanelBatch && n <= kFastPanelRows) return getrf_lookahead<T>(a, lda, n, batch, sb, ipiv, info, perm, st);
     for (int k0 = 0; k0 < n;) {
@@ -1168,7 +1169,7 @@ int getrs(const tvlm_view* av, const int32_t* ipiv, const tvlm_view* xv, void* s
         const int nblk = (n + kSB - 1) / kSB;
         void* ws = nullptr;
         const size_t inv_bytes = sizeof(T) * 2 * (size_t)nblk * kSB * kSB * batch;
-        TVLM_TRY(device_workspace(inv_bytes + sizeof(int32_t) * (size_t)nblk * kSolvePairs * batch, &ws, kWsSolveInv));
+        TVLM_TRY(device_workspace(inv_bytes + sizeof(int32_t) * (size_t)nblk * kSolvePairs * batch, &ws, kWsSolveInv, st));
         T* inv = static_cast<T*>(ws);
         int32_t* pairs = reinterpret_cast<int32_t*>(static_cast<char*>(ws) + inv_bytes);
         hipLaunchKernelGGL(k_solve_prep<T>, dim3(nblk, 3, batch), dim3(64), 0, st, a, lda, n, ipiv, pairs, inv, sa);

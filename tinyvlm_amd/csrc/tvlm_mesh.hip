@@ -10,6 +10,11 @@
 #include <string>
 #include <vector>
 
+#include <map>
+#include <tuple>
+
+#include <hip/hip_ext.h>
+
 #include "tvlm_internal.hpp"
 
 namespace tvlm {
@@ -26,25 +31,33 @@ void set_error(const char* fmt, ...) {
     g_last_error = buf;
 }
 
-// ---- per-device grow-only workspace -------------------------------------------------------------
+// ---- grow-only workspaces, one per (device, slot, stream) ----------------------------------------
 namespace {
 struct Workspace {
     void* ptr = nullptr;
     size_t bytes = 0;
 };
 std::mutex g_ws_mutex;
-std::vector<Workspace> g_ws;  // [device * kWorkspaceSlots + slot]
+std::map<std::tuple<int, int, void*>, Workspace> g_ws;
 
 }  // namespace
 
-int device_workspace(size_t bytes, void** out, int slot) {
+int create_queue_stream(hipStream_t* out) {
+    int dev = 0, cus = 0;
+    TVLM_HIP(hipGetDevice(&dev));
+    TVLM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    std::vector<uint32_t> mask((cus + 31) / 32, 0u);
+    for (int c = 0; c < cus; c++) mask[c / 32] |= 1u << (c % 32);
+    TVLM_HIP(hipExtStreamCreateWithCUMask(out, (uint32_t)mask.size(), mask.data()));
+    return TVLM_OK;
+}
+
+int device_workspace(size_t bytes, void** out, int slot, hipStream_t stream) {
     TVLM_REQUIRE(slot >= 0 && slot < kWorkspaceSlots, "workspace slot %d out of range", slot);
     int dev = 0;
     TVLM_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_ws_mutex);
-    const size_t idx = (size_t)dev * kWorkspaceSlots + slot;
-    if (g_ws.size() <= idx) g_ws.resize(idx + 1);
-    Workspace& w = g_ws[idx];
+    Workspace& w = g_ws[std::make_tuple(dev, slot, static_cast<void*>(stream))];
     if (w.bytes < bytes) {
         if (w.ptr) {
             // a previous launch may still read the old buffer
@@ -430,6 +443,29 @@ int tvlm_init(int32_t device) {
     TVLM_HIP(hipGetDeviceCount(&n));
     TVLM_REQUIRE(device >= 0 && device < n, "device %d not present (%d devices)", device, n);
     TVLM_HIP(hipSetDevice(device));
+    return TVLM_OK;
+}
+
+int tvlm_stream_create(void** stream) {
+    TVLM_REQUIRE(stream, "null argument");
+    hipStream_t s = nullptr;
+    TVLM_TRY(create_queue_stream(&s));
+    *stream = s;
+    return TVLM_OK;
+}
+
+int tvlm_stream_destroy(void* stream) {
+    if (!stream) return TVLM_OK;
+    TVLM_HIP(hipStreamSynchronize(as_stream(stream)));
+    TVLM_HIP(hipStreamDestroy(as_stream(stream)));
+    return TVLM_OK;
+}
+
+int tvlm_stream_query(void* stream, int32_t* done) {
+    TVLM_REQUIRE(done, "null argument");
+    const hipError_t e = hipStreamQuery(as_stream(stream));
+    *done = e == hipSuccess ? 1 : 0;
+    if (e != hipErrorNotReady) TVLM_HIP(e);
     return TVLM_OK;
 }
 

@@ -28,11 +28,13 @@ constexpr int kResidMaxN = 8192;  // x in LDS
 }  // namespace
 
 struct tvlm_vlm_plan {
-    int32_t mode = 0, n_surf = 0, N = 0, batch = 0;
+    int32_t mode = 0, n_surf = 0, N = 0, batch = 0, active = 0;
     hipStream_t stream = nullptr;
     int64_t ns[kMaxS] = {}, nc[kMaxS] = {}, off[kMaxS] = {};  // off: first panel of surface m
     int64_t vsz[kMaxS] = {}, wsz[kMaxS] = {}, psz[kMaxS] = {};  // elements per request: verts, wake, panels
     std::vector<void*> allocs;
+    int64_t bytes = 0;                                           // device bytes held
+    hipEvent_t done = nullptr;  // recorded after the plan's own latest work (creation, solve, results)
     double* init[kMaxS] = {};                                    // {ns+1, nc+1, 4}
     double *verts[kMaxS] = {}, *wake[kMaxS] = {};                // per request b: + b * vsz / wsz
     double *colloc[kMaxS] = {}, *normals[kMaxS] = {}, *areas[kMaxS] = {};
@@ -48,6 +50,7 @@ struct tvlm_vlm_plan {
     int alloc(void** p, size_t bytes) {
         TVLM_HIP(hipMalloc(p, bytes));
         allocs.push_back(*p);
+        this->bytes += (int64_t)bytes;
         return TVLM_OK;
     }
 };
@@ -55,21 +58,25 @@ struct tvlm_vlm_plan {
 namespace tvlm {
 namespace {
 
-// r = rhs - A x per request (FP64), one row per thread, x staged in LDS
+// r = rhs - A x per request (FP64), one row per thread; x staged in LDS up to kResidMaxN rows,
+// read from global memory (the same for every thread, L2-resident) beyond
+template <bool STAGED>
 __global__ __launch_bounds__(kResidRows) void k_residual(const double* __restrict__ A, int n, const double* __restrict__ rhs,
                                                          const double* __restrict__ x, double* __restrict__ r) {
-    __shared__ double xs[kResidMaxN];
+    __shared__ double xs[STAGED ? kResidMaxN : 1];
     const int64_t b = blockIdx.y;
     A += b * (int64_t)n * n;
     rhs += b * n;
     x += b * n;
     r += b * n;
-    for (int i = threadIdx.x; i < n; i += kResidRows) xs[i] = x[i];
-    __syncthreads();
+    if constexpr (STAGED) {
+        for (int i = threadIdx.x; i < n; i += kResidRows) xs[i] = x[i];
+        __syncthreads();
+    }
     const int i = blockIdx.x * kResidRows + threadIdx.x;
     if (i >= n) return;
     double acc = 0.0;
-    for (int j = 0; j < n; j++) acc += A[i + (int64_t)j * n] * xs[j];
+    for (int j = 0; j < n; j++) acc += A[i + (int64_t)j * n] * (STAGED ? xs[j] : x[j]);
     r[i] = rhs[i] - acc;
 }
 
@@ -206,17 +213,13 @@ int tvlm_vlm_create(tvlm_vlm_plan** plan, int32_t n_surf, const int64_t* ns, con
         P->psz[m] = ns[m] * nc[m];
         N += ns[m] * nc[m];
     }
-    if (mode == 1 && N > kResidMaxN) {
-        delete P;
-        set_error("served steady solve supports at most %d panels (got %lld)", kResidMaxN, (long long)N);
-        return TVLM_ERR_UNSUPPORTED;
-    }
     P->N = (int32_t)N;
     const int64_t NN = N * N;
     const int64_t nb = mode == 1 ? batch : 1;  // plain mode reuses one matrix for every request
     auto fail = [&](int rc) {
         for (void* p : P->allocs) (void)hipFree(p);
         if (P->tr_host) (void)hipHostFree(P->tr_host);
+        if (P->done) (void)hipEventDestroy(P->done);
         delete P;
         return rc;
     };
@@ -258,6 +261,14 @@ int tvlm_vlm_create(tvlm_vlm_plan** plan, int32_t n_surf, const int64_t* ns, con
                 rc = tvlm_fill_f64(0, P->wake[m] + b * P->wsz[m] + 3 * (P->ns[m] + 1) * 2, 1, 1.0, (P->ns[m] + 1) * 2,
                                    stream);
     }
+    if (rc == TVLM_OK && hipEventCreateWithFlags(&P->done, hipEventDisableTiming) != hipSuccess) {
+        set_error("hipEventCreateWithFlags failed");
+        rc = TVLM_ERR_HIP;
+    }
+    if (rc == TVLM_OK && hipEventRecord(P->done, P->stream) != hipSuccess) {
+        set_error("hipEventRecord failed");
+        rc = TVLM_ERR_HIP;
+    }
     if (rc != TVLM_OK) return fail(rc);
     P->alpha.assign(batch, 0.0);
     P->beta.assign(batch, 0.0);
@@ -268,12 +279,20 @@ int tvlm_vlm_create(tvlm_vlm_plan** plan, int32_t n_surf, const int64_t* ns, con
 
 int tvlm_vlm_solve(tvlm_vlm_plan* P, const double* alpha, const double* beta, double u_inf, double rho,
                    const double* cm_ref) {
+    TVLM_REQUIRE(P, "null plan");
+    return tvlm_vlm_solve_n(P, P->batch, alpha, beta, u_inf, rho, cm_ref);
+}
+
+int tvlm_vlm_solve_n(tvlm_vlm_plan* P, int32_t n_active, const double* alpha, const double* beta, double u_inf,
+                     double rho, const double* cm_ref) {
     TVLM_REQUIRE(P && alpha, "null argument");
-    const int n = P->n_surf, B = P->batch;
+    TVLM_REQUIRE(n_active >= 1 && n_active <= P->batch, "n_active %d out of range 1..%d", n_active, P->batch);
+    const int n = P->n_surf, B = n_active;
+    P->active = B;
     P->u_inf = u_inf;
     P->rho = rho;
     const double c0[3] = {cm_ref ? cm_ref[0] : 0.25, cm_ref ? cm_ref[1] : 0.0, cm_ref ? cm_ref[2] : 0.0};
-    TVLM_HIP(hipStreamSynchronize(P->stream));  // the pinned transform buffer may still be read
+    TVLM_HIP(hipEventSynchronize(P->done));  // the pinned transform buffer may still be read
     for (int b = 0; b < B; b++) {
         P->alpha[b] = alpha[b];
         P->beta[b] = beta ? beta[b] : 0.0;
@@ -315,8 +334,13 @@ int tvlm_vlm_solve(tvlm_vlm_plan* P, const double* alpha, const double* beta, do
             if (it == 0) {
                 hipLaunchKernelGGL(k_to_f32, dim3(grid_for(N * B)), dim3(256), 0, st, P->rhs, P->r32, N * B);
             } else {
-                hipLaunchKernelGGL(k_residual, dim3((unsigned)((N + kResidRows - 1) / kResidRows), B), dim3(kResidRows),
-                                   0, st, P->lhs, (int)N, P->rhs, P->x, P->res);
+                const dim3 rg((unsigned)((N + kResidRows - 1) / kResidRows), B);
+                if (N <= kResidMaxN)
+                    hipLaunchKernelGGL(k_residual<true>, rg, dim3(kResidRows), 0, st, P->lhs, (int)N, P->rhs, P->x,
+                                       P->res);
+                else
+                    hipLaunchKernelGGL(k_residual<false>, rg, dim3(kResidRows), 0, st, P->lhs, (int)N, P->rhs, P->x,
+                                       P->res);
                 TVLM_LAUNCHED();
                 hipLaunchKernelGGL(k_to_f32, dim3(grid_for(N * B)), dim3(256), 0, st, P->res, P->r32, N * B);
             }
@@ -327,17 +351,24 @@ int tvlm_vlm_solve(tvlm_vlm_plan* P, const double* alpha, const double* beta, do
         }
         for (int b = 0; b < B; b++) TVLM_TRY(post(P, b));
     }
+    TVLM_HIP(hipEventRecord(P->done, st));
     P->solved = true;
     return TVLM_OK;
 }
 
-// Synchronises.  cl[batch], cm[batch][3], gamma[batch][N], info[batch]; null pointers are skipped.
+// Waits for the plan's last solve.  cl[n], cm[n][3], gamma[n][N], info[n] for the n requests of
+// the last solve; null pointers are skipped.  The copies are queued on the plan's stream (not the
+// null stream), so reading one plan's results does not wait for work on other streams.
 int tvlm_vlm_results(tvlm_vlm_plan* P, double* cl, double* cm, double* gamma, int32_t* info) {
     TVLM_REQUIRE(P && P->solved, "no solve has run on this plan");
-    TVLM_HIP(hipStreamSynchronize(P->stream));
-    const int n = P->n_surf, B = P->batch;
+    const int n = P->n_surf, B = P->active;
+    hipStream_t st = P->stream;
     std::vector<double> coef((size_t)8 * n * B);
-    TVLM_HIP(hipMemcpy(coef.data(), P->coef, sizeof(double) * coef.size(), hipMemcpyDeviceToHost));
+    TVLM_HIP(hipMemcpyAsync(coef.data(), P->coef, sizeof(double) * coef.size(), hipMemcpyDeviceToHost, st));
+    if (gamma) TVLM_HIP(hipMemcpyAsync(gamma, P->x, sizeof(double) * P->N * B, hipMemcpyDeviceToHost, st));
+    if (info) TVLM_HIP(hipMemcpyAsync(info, P->info, sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
+    TVLM_HIP(hipEventRecord(P->done, st));
+    TVLM_HIP(hipEventSynchronize(P->done));
     for (int b = 0; b < B; b++) {
         // per surface: cf[2] = sum(F . l) / q and cf[3..5] = sum(r x F) / q (q = rho |V|^2 / 2,
         // vlm_backend_cpu.cpp:323-362); CL area-weighted, CM area*MAC-weighted over the surfaces
@@ -353,15 +384,40 @@ int tvlm_vlm_results(tvlm_vlm_plan* P, double* cl, double* cm, double* gamma, in
         if (cm)
             for (int k = 0; k < 3; k++) cm[3 * b + k] = cm_sum[k] / (area_sum * mac_sum);
     }
-    if (gamma) TVLM_HIP(hipMemcpy(gamma, P->x, sizeof(double) * P->N * B, hipMemcpyDeviceToHost));
-    if (info) TVLM_HIP(hipMemcpy(info, P->info, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    return TVLM_OK;
+}
+
+// Moves the plan to another stream: later solves queue on `stream`, after the plan's own latest
+// work on its previous stream (a device-side wait on the plan's event; the host does not block, and
+// whatever else runs on the previous stream is not waited for).
+int tvlm_vlm_set_stream(tvlm_vlm_plan* P, void* stream) {
+    TVLM_REQUIRE(P, "null plan");
+    if (as_stream(stream) == P->stream) return TVLM_OK;
+    TVLM_HIP(hipStreamWaitEvent(as_stream(stream), P->done, 0));
+    P->stream = as_stream(stream);
+    return TVLM_OK;
+}
+
+// *done = 1 once the plan's latest work has finished (does not block)
+int tvlm_vlm_query(const tvlm_vlm_plan* P, int32_t* done) {
+    TVLM_REQUIRE(P && done, "null argument");
+    const hipError_t e = hipEventQuery(P->done);
+    *done = e == hipSuccess ? 1 : 0;
+    if (e != hipErrorNotReady) TVLM_HIP(e);
+    return TVLM_OK;
+}
+
+int tvlm_vlm_footprint(const tvlm_vlm_plan* P, int64_t* device_bytes) {
+    TVLM_REQUIRE(P && device_bytes, "null argument");
+    *device_bytes = P->bytes;
     return TVLM_OK;
 }
 
 int tvlm_vlm_destroy(tvlm_vlm_plan* P) {
     if (!P) return TVLM_OK;
-    (void)hipStreamSynchronize(P->stream);
+    (void)hipEventSynchronize(P->done);
     for (void* p : P->allocs) (void)hipFree(p);
+    (void)hipEventDestroy(P->done);
     if (P->tr_host) (void)hipHostFree(P->tr_host);
     delete P;
     return TVLM_OK;

@@ -1,0 +1,327 @@
+"""Continuous batching of steady solves over a mix of meshes: the serving form of the hot path's
+caller VLM::run (vlm/src/vlm.cpp:97-151), one request = one (mesh, angle of attack) solve.
+
+Structure
+- lanes: `lanes` streams, each with a hardware queue of its own (tvlm_stream_create).  A lane runs
+  one batch at a time; batches on different lanes run at the same time on the device.
+- plans: per mesh, a small pool of served-mode plans (tvlm_vlm_create, mode 1: seeded FP64
+  matrices, one batched FP32 LU, FP64 refinement), created when first needed and moved to whichever
+  lane is free (tvlm_vlm_set_stream).  A plan's batch cap follows from its mesh size:
+  min(max_batch, plan_bytes // (12 N^2)) requests (FP64 + FP32 copy of each request's matrix).
+- memory: the device bytes of all plans (tvlm_vlm_footprint) stay under `budget_bytes`.  A new plan
+  that would exceed it first evicts the least recently used idle plans; if that is not enough, its
+  mesh waits for memory while other meshes are served.
+- admission: whenever a lane is free, the mesh whose oldest waiting request arrived first is
+  served: an idle plan of that mesh (or a new one) takes up to its cap of that mesh's waiting
+  requests, in arrival order.  Requests that arrive while others are on the device are admitted as
+  soon as a lane and a plan (memory) are free, whatever else is running.
+
+A request's result does not depend on the batch it shared: each matrix of a batched LU is factored
+independently, and the FP64 refinement drives every solution to the FP64 solution of its own FP64
+system; only FP32 rounding inside the LU (panel width, GEMM kernel choice, which follow the batch
+size) differs, and the refinement removes it to ~1e-14 relative.
+
+The request mix used to measure the server is `steady_mix_trace`: 4096 requests, PCG64 seed 0,
+exponential inter-arrival gaps (unit rate, scaled by 1/lambda when run), mesh family in
+{elliptic fixture, rectangular, 25-degree leading-edge sweep; chord 1, half-span 5} x n x n panels
+with n in {16, 32, 45, 64, 90, 128}, alpha ~ U[-5, 10] degrees drawn in float32.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import tempfile
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import ROOT_DIR
+from . import capi as H
+from .mesh_io import read_plot3d
+from .steady_plan import MODE_SERVED, SteadyPlan
+
+MIX_FAMILIES = ("elliptic", "rectangular", "swept25")
+MIX_SIZES = (16, 32, 45, 64, 90, 128)
+MIX_CHORD, MIX_HALF_SPAN, MIX_SWEEP_DEG = 1.0, 5.0, 25.0
+TRACE_PATH = os.path.join(ROOT_DIR, "tinyvlm_amd", "data", "steady_mix_4096.csv")
+
+
+# ---- request mix --------------------------------------------------------------------------------
+def steady_mix_trace(n_requests: int = 4096, seed: int = 0) -> dict:
+    """The request mix: arrays gap (unit-rate exponential inter-arrival gaps), family (index into
+    MIX_FAMILIES), n (panels per side), alpha_deg (float32 degrees).  One PCG64 generator, drawn in
+    the order gaps, families, sizes, angles."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    gap = rng.exponential(1.0, n_requests)
+    fam = rng.integers(0, len(MIX_FAMILIES), n_requests)
+    size = np.asarray(MIX_SIZES)[rng.integers(0, len(MIX_SIZES), n_requests)]
+    alpha = rng.uniform(-5.0, 10.0, n_requests).astype(np.float32)
+    return {"gap": gap, "family": fam.astype(np.int64), "n": size.astype(np.int64), "alpha_deg": alpha}
+
+
+def write_trace(path: str, trace: dict) -> None:
+    with open(path, "w") as f:
+        f.write("rid,family,n,alpha_deg,gap\n")
+        for i in range(len(trace["gap"])):
+            f.write(f"{i},{MIX_FAMILIES[trace['family'][i]]},{trace['n'][i]},"
+                    f"{float(trace['alpha_deg'][i])!r},{float(trace['gap'][i])!r}\n")
+
+
+def read_trace(path: str = TRACE_PATH) -> dict:
+    fam, n, alpha, gap = [], [], [], []
+    with open(path) as f:
+        header = f.readline().strip().split(",")
+        if header != ["rid", "family", "n", "alpha_deg", "gap"]:
+            raise ValueError(f"{path}: unexpected header {header}")
+        for line in f:
+            _, fa, nn, al, ga = line.strip().split(",")
+            fam.append(MIX_FAMILIES.index(fa))
+            n.append(int(nn))
+            alpha.append(float(al))
+            gap.append(float(ga))
+    return {"gap": np.asarray(gap), "family": np.asarray(fam, np.int64), "n": np.asarray(n, np.int64),
+            "alpha_deg": np.asarray(alpha, np.float32)}
+
+
+def write_grid_plot3d(path: str, x: np.ndarray, y: np.ndarray) -> None:
+    """Single-block plot3d text file of the flat grid x[i, j], y[i, j] (i chordwise, fastest;
+    j spanwise), z = 0."""
+    ni, nj = x.shape
+    with open(path, "w") as f:
+        f.write(f"1\n{ni} {nj} 1\n")
+        for plane in (x.T.ravel(), y.T.ravel(), np.zeros(ni * nj)):
+            f.write("\n".join(f"{v:.17g}" for v in plane) + "\n")
+
+
+def mix_mesh(family: str, n: int, cache_dir: str | None = None) -> np.ndarray:
+    """verts of one mesh of the mix, through the plot3d reader (float32 parse, quarter-chord shift,
+    as VLM::run reads its mesh).  elliptic: the committed fixture mesh/elliptic_{n}x{n}.x;
+    rectangular / swept25: n x n uniform panels, chord 1, half-span 5, leading edge at
+    x = y tan(sweep)."""
+    if family == "elliptic":
+        return read_plot3d(os.path.join(ROOT_DIR, "mesh", f"elliptic_{n}x{n}.x"))
+    if family not in ("rectangular", "swept25"):
+        raise ValueError(f"unknown mesh family {family!r}")
+    d = cache_dir or tempfile.gettempdir()
+    path = os.path.join(d, f"tvlm_mix_{family}_{n}x{n}.x")
+    xs = np.linspace(0.0, MIX_CHORD, n + 1)
+    ys = np.linspace(0.0, MIX_HALF_SPAN, n + 1)
+    x, y = np.meshgrid(xs, ys, indexing="ij")
+    if family == "swept25":
+        x = x + y * math.tan(math.radians(MIX_SWEEP_DEG))
+    write_grid_plot3d(path, x, y)
+    return read_plot3d(path)
+
+
+# ---- requests and lanes -------------------------------------------------------------------------
+@dataclass
+class Request:
+    rid: int
+    mesh: str
+    alpha: float  # radians
+    arrival: float = 0.0  # seconds after the start of the run
+    admitted: float = math.nan
+    finished: float = math.nan
+    batch: int = 0  # requests in the batch it ran in
+    lane: int = -1
+    cl: float = math.nan
+    cm: np.ndarray | None = None
+    gamma: np.ndarray | None = None
+    info: int = 0
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def stream_create() -> int:
+    s = ctypes.c_void_p()
+    H.check(H.lib().tvlm_stream_create(ctypes.byref(s)), "tvlm_stream_create")
+    return int(s.value)
+
+
+def stream_destroy(s: int) -> None:
+    H.check(H.lib().tvlm_stream_destroy(ctypes.c_void_p(s)), "tvlm_stream_destroy")
+
+
+@dataclass
+class _Slot:
+    plan: SteadyPlan
+    bytes: int
+    busy: bool = False
+    last_used: float = 0.0
+
+
+@dataclass
+class ServeStats:
+    makespan: float = 0.0
+    batches: int = 0
+    plans_created: int = 0
+    plans_evicted: int = 0
+    peak_bytes: int = 0
+    max_running: int = 0  # most lanes busy at once
+    batch_sizes: list = field(default_factory=list)
+
+
+class SteadyServer:
+    def __init__(self, meshes: dict, lanes: int = 4, max_batch: int = 32, plan_bytes: int = 4 << 30,
+                 budget_bytes: int | None = None, plans_per_mesh: int | None = None, u_inf: float = 1.0,
+                 rho: float = 1.0, keep_gamma: bool = False):
+        """meshes: name -> list of verts arrays (one per surface, as read_plot3d returns them)."""
+        if lanes < 1 or max_batch < 1:
+            raise ValueError("lanes and max_batch must be positive")
+        self.meshes = {k: [np.asarray(v, dtype=np.float64) for v in vl] for k, vl in meshes.items()}
+        self.N = {k: sum((v.shape[0] - 1) * (v.shape[1] - 1) for v in vl) for k, vl in self.meshes.items()}
+        self.cap = {k: int(max(1, min(max_batch, plan_bytes // (12 * n * n)))) for k, n in self.N.items()}
+        if budget_bytes is None:
+            import torch
+
+            budget_bytes = int(0.6 * torch.cuda.mem_get_info()[1])
+        self.budget = int(budget_bytes)
+        self.plans_per_mesh = plans_per_mesh or lanes
+        self.u_inf, self.rho, self.keep_gamma = float(u_inf), float(rho), bool(keep_gamma)
+        self.lanes = [stream_create() for _ in range(lanes)]
+        self.slots: dict[str, list[_Slot]] = {k: [] for k in self.meshes}
+        self.used_bytes = 0
+        self.stats = ServeStats()
+
+    # -- memory ------------------------------------------------------------------------------------
+    def _estimate(self, key: str) -> int:
+        n, b = self.N[key], self.cap[key]
+        return 12 * n * n * b + 400 * n * b
+
+    def _evict_for(self, need: int, now: float) -> bool:
+        idle = sorted((s.last_used, k, i) for k, ss in self.slots.items() for i, s in enumerate(ss) if not s.busy)
+        victims = []
+        free = self.budget - self.used_bytes
+        for _, k, i in idle:
+            if free >= need:
+                break
+            victims.append((k, i))
+            free += self.slots[k][i].bytes
+        if free < need:
+            return False
+        for k, i in sorted(victims, key=lambda v: -v[1]):
+            s = self.slots[k].pop(i)
+            s.plan.close()
+            self.used_bytes -= s.bytes
+            self.stats.plans_evicted += 1
+        return True
+
+    def _slot_for(self, key: str, lane: int, now: float) -> _Slot | None:
+        for s in self.slots[key]:
+            if not s.busy:
+                return s
+        if len(self.slots[key]) >= self.plans_per_mesh:
+            return None
+        need = self._estimate(key)
+        if need > self.budget:
+            raise RuntimeError(f"mesh {key}: one plan ({need} bytes) exceeds the memory budget {self.budget}")
+        if self.used_bytes + need > self.budget and not self._evict_for(need, now):
+            return None
+        plan = SteadyPlan(self.meshes[key], self.cap[key], mode=MODE_SERVED, stream=self.lanes[lane])
+        s = _Slot(plan, plan.footprint())
+        self.slots[key].append(s)
+        self.used_bytes += s.bytes
+        self.stats.plans_created += 1
+        self.stats.peak_bytes = max(self.stats.peak_bytes, self.used_bytes)
+        return s
+
+    # -- serving loop ------------------------------------------------------------------------------
+    def run(self, requests: list[Request], realtime: bool = True, poll_s: float = 20e-6) -> ServeStats:
+        """Serves every request; fills in admitted / finished (seconds after the start), cl, cm,
+        info (and gamma with keep_gamma).  realtime: request r becomes visible at r.arrival on the
+        wall clock; otherwise all requests wait from the start (a saturation run)."""
+        for r in requests:
+            if r.mesh not in self.meshes:
+                raise KeyError(f"request {r.rid}: unknown mesh {r.mesh!r}")
+        order = sorted(requests, key=lambda r: (r.arrival, r.rid)) if realtime else list(requests)
+        waiting: dict[str, deque] = {k: deque() for k in self.meshes}
+        free_lanes = deque(range(len(self.lanes)))
+        running: dict[int, tuple[_Slot, list[Request]]] = {}
+        nxt, finished = 0, 0
+        st = self.stats
+        t0 = time.perf_counter()
+        while finished < len(order):
+            now = time.perf_counter() - t0
+            while nxt < len(order) and (not realtime or order[nxt].arrival <= now):
+                waiting[order[nxt].mesh].append(order[nxt])
+                nxt += 1
+            progressed = False
+            for lane in list(running):
+                slot, batch = running[lane]
+                if not slot.plan.done():
+                    continue
+                res = slot.plan.results(gamma=self.keep_gamma)
+                t = time.perf_counter() - t0
+                for j, r in enumerate(batch):
+                    r.finished = t
+                    r.cl = float(res["cl"][j])
+                    r.cm = res["cm"][j].copy()
+                    r.info = int(res["info"][j])
+                    if self.keep_gamma:
+                        r.gamma = res["gamma"][j].copy()
+                slot.busy = False
+                slot.last_used = t
+                del running[lane]
+                free_lanes.append(lane)
+                finished += len(batch)
+                progressed = True
+            while free_lanes:
+                heads = sorted((q[0].arrival, q[0].rid, k) for k, q in waiting.items() if q)
+                chosen = None
+                for _, _, k in heads:
+                    slot = self._slot_for(k, free_lanes[0], now)
+                    if slot is not None:
+                        chosen = (k, slot)
+                        break
+                if chosen is None:
+                    break
+                k, slot = chosen
+                lane = free_lanes.popleft()
+                q = waiting[k]
+                batch = [q.popleft() for _ in range(min(self.cap[k], len(q)))]
+                slot.plan.set_stream(self.lanes[lane])
+                slot.busy = True
+                slot.plan.solve(np.asarray([r.alpha for r in batch]), u_inf=self.u_inf, rho=self.rho)
+                t = time.perf_counter() - t0
+                for r in batch:
+                    r.admitted = t
+                    r.batch = len(batch)
+                    r.lane = lane
+                running[lane] = (slot, batch)
+                st.batches += 1
+                st.batch_sizes.append(len(batch))
+                st.max_running = max(st.max_running, len(running))
+                progressed = True
+            if not progressed:
+                if not running and nxt < len(order) and not any(waiting.values()):
+                    time.sleep(max(0.0, order[nxt].arrival - (time.perf_counter() - t0)))
+                else:
+                    time.sleep(poll_s)
+        st.makespan = time.perf_counter() - t0
+        return st
+
+    def close(self):
+        for ss in self.slots.values():
+            for s in ss:
+                s.plan.close()
+            ss.clear()
+        self.used_bytes = 0
+        for s in self.lanes:
+            stream_destroy(s)
+        self.lanes = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def latency_percentiles(requests: list[Request]) -> tuple[float, float]:
+    lat = np.asarray([r.latency for r in requests])
+    return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
