@@ -50,10 +50,10 @@ inline bool plane_contiguous(const tvlm_view& v) {
 }
 
 // Scratch buffers owned by the library (grow-only), one per device, slot and stream: wake-induction
-// tile partials, LU row-interchange lists, triangular-solve block inverses.  Work queued on
-// different streams never shares a buffer, so concurrent solves on separate streams are safe;
-// work on one stream is ordered by the stream.
-enum WorkspaceSlot { kWsWakePartials = 0, kWsLuPerm = 1, kWsSolveInv = 2, kWorkspaceSlots = 3 };
+// tile partials, LU row-interchange lists, triangular-solve block inverses, the exchange buffers of
+// the cooperative LU panels.  Work queued on different streams never shares a buffer, so concurrent
+// solves on separate streams are safe; work on one stream is ordered by the stream.
+enum WorkspaceSlot { kWsWakePartials = 0, kWsLuPerm = 1, kWsSolveInv = 2, kWsLuCoop = 3, kWorkspaceSlots = 4 };
 int device_workspace(size_t bytes, void** out, int slot, hipStream_t stream);
 
 // A stream with a hardware queue of its own: created with an all-CUs mask (streams created

@@ -6,18 +6,23 @@
 //
 // getrf is blocked right-looking; each panel (64 columns, 32 while more than 2048 rows remain):
 //   1. k_panel_reg  : one work-group factors the panel (register-resident sub-panels, DPP argmax,
-//                     LDS-only barriers) and lists the panel's composed row interchanges;
+//                     LDS-only barriers) and lists the panel's composed row interchanges; panels
+//                     of more than 1024 rows of one matrix (or of a few) are factored by several
+//                     work-groups together instead (k_panel_coop, 64 columns, one exchange through
+//                     global memory per column);
 //   2. k_swap_trsm  : the interchanges on the columns left and right of the panel, and
 //                     U12 = L11^{-1} A12 for the columns right of it;
 //   3. A22 -= L21 U12 on the matrix cores: rocBLAS ?gemm_strided_batched for trailing blocks of at
 //                     least 512 rows, k_update_mfma below that.
-// Panels taller than 8192 rows use the column-by-column k_panel + k_laswp + k_trsm instead.
+// Panels taller than the cooperative groups hold (and taller than 8192 rows) use the
+// column-by-column k_panel + k_laswp + k_trsm instead.
 // getrs (n <= 16384): the interchanges are composed per block of 64 pivots and the 64 x 64
 // diagonal blocks of L and U inverted (k_solve_prep), then one work-group per right-hand side
 // applies the interchanges and both blocked substitutions with the column in LDS
 // (k_solve_blocked); larger n uses block-row kernels (k_diag_solve / k_off_update).
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -551,6 +556,179 @@ __global__ __launch_bounds__(NT) void k_panel_reg(T* __restrict__ a, int64_t lda
     if (t == 0) perm[0] = np;
 }
 
+// ---- cooperative tall panel ---------------------------------------------------------------------
+// k_panel_coop: G work-groups of NT threads factor the panel rows [k0, n) x columns [k0, k0 + kb)
+// together (kb <= NB).  Thread t of group g owns row k0 + g NT + t; the group keeps its NT rows of
+// the panel in LDS, column-major (consecutive threads, consecutive banks), from the first column to
+// the last.  Per column, one exchange through global memory: every group publishes its local pivot
+// candidate (|value|, row) and that row's values, group 0 publishes the diagonal row, then the
+// groups meet at a per-column counter (one atomic add each, then a bounded spin).  Every group then
+// picks the same pivot (largest |value|, first row on ties: LAPACK's choice), the owners of the
+// pivot and diagonal rows exchange them, and each group scales and updates its own rows.  Exchange
+// buffers (per matrix): cval / crow [NB][kCoopMaxGroups], xrow [NB][kCoopMaxGroups + 1][NB] (slot
+// kCoopMaxGroups: the diagonal row), counters ctr [NB], zeroed before the launch.  A spin that runs
+// out (the groups were not all resident) leaves info = kCoopStalled instead of waiting for ever.
+// Group 0 writes ipiv and the panel's composed interchanges for k_swap_trsm, as k_panel_reg does.
+constexpr int kCoopMaxGroups = 64;   // work-groups of one launch; all of them must be resident at once
+constexpr int kCoopMinRows = 1024;   // shorter panels stay in one work-group (k_panel_reg)
+constexpr unsigned kCoopSpinLimit = 1u << 22;
+constexpr int kCoopStalled = -999;
+
+template <typename T>
+__device__ __forceinline__ void st_agent(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_agent(T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// threads (= rows) per cooperative work-group: NT rows x NB columns of T = 128 KB of LDS
+template <typename T>
+constexpr int coop_threads() {
+    return sizeof(T) == 4 ? 512 : 256;
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t lda, int n, int k0, int kb, int G,
+                                                   int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
+                                                   int32_t* __restrict__ perm, int64_t sb, int32_t* ctr, T* cval,
+                                                   int32_t* crow, T* xrow) {
+    constexpr int NW = NT / 64, MG = kCoopMaxGroups;
+    const int z = blockIdx.z, g = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
+    a += (int64_t)z * sb;
+    ipiv += (int64_t)z * n;
+    info += z;
+    perm += (int64_t)z * kPermInts;
+    ctr += (int64_t)z * NB;
+    cval += (int64_t)z * NB * MG;
+    crow += (int64_t)z * NB * MG;
+    xrow += (int64_t)z * NB * (MG + 1) * NB;
+    const int m = n - k0, i = g * NT + t;  // this thread's row: k0 + i
+    const bool mine = i < m;
+    extern __shared__ unsigned char coop_lds[];
+    T* xs = reinterpret_cast<T*>(coop_lds);  // xs[jj * NT + t]: column jj of this group's row t
+    __shared__ T s_val[NW];
+    __shared__ int s_idx[NW];
+    __shared__ T s_prow[NB], s_crow[NB];
+    __shared__ int s_p, s_fail, s_zero, s_arg[NB], s_piv[NB];
+    __shared__ int s_pd[2 * NB], s_ps[2 * NB], s_pcnt;
+    if (t == 0) {
+        s_fail = 0;
+        s_zero = 0;
+    }
+    for (int jj = 0; jj < kb; jj++) xs[jj * NT + t] = mine ? a[k0 + i + (int64_t)(k0 + jj) * lda] : T(0);
+    __syncthreads();
+    for (int j = 0; j < kb; j++) {
+        const int c = k0 + j;
+        // this group's candidate: largest |a(r, c)| over its rows r >= c, first row on ties
+        T best = T(-1);
+        int bi = n;
+        if (mine && i >= j) {
+            best = (T)fabs(xs[j * NT + t]);
+            bi = k0 + i;
+        }
+        wave_argmax(best, bi);
+        if (lane == 0) {
+            s_val[w] = best;
+            s_idx[w] = bi;
+        }
+        __syncthreads();
+        best = s_val[0];
+        bi = s_idx[0];
+#pragma unroll
+        for (int ww = 1; ww < NW; ww++) argmax_merge(best, bi, s_val[ww], s_idx[ww]);
+        // publish the candidate, its row and (group 0) the diagonal row, then meet
+        T* xr = xrow + (int64_t)j * (MG + 1) * NB;
+        bool wrote = false;
+        if (bi < n && t < kb) {  // bi is one of this group's rows
+            st_agent(&xr[g * NB + t], xs[t * NT + (bi - k0 - g * NT)]);
+            wrote = true;
+        }
+        if (g == 0 && t >= 64 && t - 64 < kb) {  // row c = k0 + j: group 0, local row j
+            st_agent(&xr[MG * NB + (t - 64)], xs[(t - 64) * NT + j]);
+            wrote = true;
+        }
+        if (t == 0) {
+            st_agent(&cval[j * MG + g], best);
+            st_agent(&crow[j * MG + g], bi);
+            wrote = true;
+        }
+        if (wrote) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (t == 0) {
+            __hip_atomic_fetch_add(&ctr[j], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned spins = 0;
+            while (__hip_atomic_load(&ctr[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < G) {
+                if (++spins > kCoopSpinLimit) {
+                    s_fail = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __syncthreads();
+        // the pivot: the same reduction in every group
+        if (w == 0) {
+            T v = T(-1);
+            int ri = n;
+            if (lane < G) {
+                v = ld_agent(&cval[j * MG + lane]);
+                ri = ld_agent(&crow[j * MG + lane]);
+            }
+            wave_argmax(v, ri);
+            if (lane == 0) s_p = ri < n ? ri : c;
+        }
+        __syncthreads();
+        const int p = s_p;
+        const int gw = (p - k0) / NT;
+        if (t < kb) s_prow[t] = ld_agent(&xr[gw * NB + t]);
+        if (t >= 64 && t - 64 < kb) s_crow[t - 64] = ld_agent(&xr[MG * NB + (t - 64)]);
+        __syncthreads();
+        const T piv = s_prow[j];
+        const bool nz = piv != T(0);
+        if (g == 0 && t == 0) {
+            s_arg[j] = p;
+            s_piv[j] = nz ? p : c;
+            if (!nz && s_zero == 0) s_zero = c + 1;
+        }
+        if (nz) {
+            if (p != c) {
+                if (g == 0 && t < kb) xs[t * NT + j] = s_prow[t];
+                if (gw == g && t < kb) xs[t * NT + (p - k0 - g * NT)] = s_crow[t];
+                __syncthreads();
+            }
+            if (mine && i > j) {
+                const T l = xs[j * NT + t] * (T(1) / piv);
+                xs[j * NT + t] = l;
+                for (int jj = j + 1; jj < kb; jj++) xs[jj * NT + t] -= l * s_prow[jj];
+            }
+        }
+        __syncthreads();  // s_val / s_prow / s_crow / s_p reused by the next column
+    }
+    if (mine)
+        for (int jj = 0; jj < kb; jj++) a[k0 + i + (int64_t)(k0 + jj) * lda] = xs[jj * NT + t];
+    if (g != 0) return;
+    if (t < kb) ipiv[k0 + t] = s_arg[t] + 1;
+    if (t == 0) {
+        if (s_fail)
+            info[0] = kCoopStalled;
+        else if (s_zero != 0 && info[0] == 0)
+            info[0] = s_zero;
+    }
+    if (t < 64) {
+        const int np = wave_compose_swaps(k0, kb, s_piv, s_pd, s_ps);
+        if (t == 0) s_pcnt = np;
+    }
+    __syncthreads();
+    const int np = s_pcnt;
+    for (int e = t; e < np; e += NT) {
+        perm[1 + 2 * e] = s_pd[e];
+        perm[2 + 2 * e] = s_ps[e];
+    }
+    if (t == 0) perm[0] = np;
+}
+
 // Row interchanges of one panel on the columns outside it, then for the columns right of the panel
 // U12 = L11^{-1} A12 (L11 unit lower).  A work-group of 256 threads takes 64 columns: the swap pairs
 // are gathered for all columns at once, then scattered; U12's 64 x 64 block and L11 sit in LDS and
@@ -844,6 +1022,41 @@ __global__ void k_zero_info_batch(int32_t* info, int batch) {
     if (i < batch) info[i] = 0;
 }
 
+// Work-groups for a cooperative panel of m rows, or 0 when the panel stays in one work-group (short
+// panels, batches whose groups would not all fit in kCoopMaxGroups, or panels taller than the
+// groups can hold).
+template <typename T>
+int coop_groups(int m, int batch) {
+    if (m <= kCoopMinRows) return 0;
+    const int per_group = coop_threads<T>();
+    const int G = (m + per_group - 1) / per_group;
+    if (G < 2 || G * batch > kCoopMaxGroups) return 0;
+    return G;
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+constexpr size_t kCoopLdsBytes = 128 * 1024;  // NT rows x NB columns (coop_threads)
+
+template <typename T>
+int panel_coop(T* a, int64_t lda, int n, int k0, int kb, int G, int32_t* ipiv, int32_t* info, int32_t* perm,
+               hipStream_t st, int batch, int64_t sb, int32_t* ctr, T* cval, int32_t* crow, T* xrow) {
+    constexpr int NT = coop_threads<T>();
+    static const hipError_t lds_attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_panel_coop<T, NT>),
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                           (int)kCoopLdsBytes);
+    TVLM_HIP(lds_attr);
+    TVLM_REQUIRE(G >= 2 && G * batch <= kCoopMaxGroups && kb <= NB && (int64_t)G * NT >= (int64_t)(n - k0),
+                 "cooperative panel: %d groups for %d rows", G, n - k0);
+    hipLaunchKernelGGL((k_panel_coop<T, NT>), dim3(G, 1, batch), dim3(NT), kCoopLdsBytes, st, a, lda, n, k0, kb, G,
+                       ipiv, info, perm, sb, ctr, cval, crow, xrow);
+    TVLM_LAUNCHED();
+    return TVLM_OK;
+}
+
 // shape[2] > 1 factors a batch of matrices, matrix b at ptr + b * stride[2]; ipiv holds n entries
 // per matrix (matrix b at ipiv + b n), info one per matrix.
 // Side stream and events of the batched look-ahead, one set per (device, caller stream), created
@@ -944,17 +1157,43 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     TVLM_TRY(device_workspace(sizeof(int32_t) * kPermInts * (size_t)batch, &ws, kWsLuPerm, st));
     int32_t* perm = static_cast<int32_t*>(ws);
     if (batch >= kWidePanelBatch && n <= kFastPanelRows) return getrf_lookahead<T>(a, lda, n, batch, sb, ipiv, info, perm, st);
-    for (int k0 = 0; k0 < n;) {
+    // exchange buffers of the cooperative tall panels: per panel and matrix NB counters (zeroed
+    // here, once for the whole factorisation), per matrix the candidates and published rows
+    const int panels = (n + NB - 1) / NB;
+    const bool any_coop = coop_groups<T>(n, batch) > 0;
+    int32_t* ctr = nullptr;
+    T *cval = nullptr, *xrow = nullptr;
+    int32_t* crow = nullptr;
+    if (any_coop) {
+        const size_t nctr = (size_t)panels * batch * NB, ncand = (size_t)batch * NB * kCoopMaxGroups;
+        const size_t nrow = (size_t)batch * NB * (kCoopMaxGroups + 1) * NB;
+        void* cws = nullptr;
+        TVLM_TRY(device_workspace(sizeof(int32_t) * (nctr + ncand) + sizeof(T) * (ncand + nrow) + 256, &cws, kWsLuCoop,
+                                  st));
+        ctr = static_cast<int32_t*>(cws);
+        crow = ctr + nctr;
+        cval = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(crow + ncand) + 255) & ~(uintptr_t)255);
+        xrow = cval + ncand;
+        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nctr + 255) / 256)), dim3(256), 0, st, ctr, (int64_t)nctr);
+        TVLM_LAUNCHED();
+    }
+    for (int k0 = 0, pi = 0; k0 < n; pi++) {
         // tall panels of one matrix are narrower: the in-panel left-looking traffic grows with width
         // squared.  A batch is bound by the trailing updates instead, whose traffic (the whole
-        // trailing matrix read and written per panel) halves with the full width.
-        const int width = (n - k0 > 2048 && batch < kWidePanelBatch) ? NB / 2 : NB;
+        // trailing matrix read and written per panel) halves with the full width.  Cooperative
+        // panels have no in-panel traffic and take the full width.
+        const int G = coop_groups<T>(n - k0, batch);
+        const int width = (G == 0 && n - k0 > 2048 && batch < kWidePanelBatch) ? NB / 2 : NB;
         const int kb = (n - k0 < width) ? n - k0 : width;
         const int j1 = k0 + kb;
         const int kp = k0;  // this panel's first column
         k0 = j1;
-        if (n - kp <= kFastPanelRows) {
-            TVLM_TRY(panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb));
+        if (G > 0 || n - kp <= kFastPanelRows) {
+            if (G > 0)
+                TVLM_TRY(panel_coop<T>(a, lda, n, kp, kb, G, ipiv, info, perm, st, batch, sb,
+                                       ctr + (size_t)pi * batch * NB, cval, crow, xrow));
+            else
+                TVLM_TRY(panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb));
             const int nleft = (kp + 63) / 64, nright = (n - j1 + 63) / 64;
             if (nleft + nright > 0) {
                 hipLaunchKernelGGL(k_swap_trsm<T>, dim3(nleft + nright, 1, batch), dim3(256), 0, st, a, lda, n, kp, kb,
