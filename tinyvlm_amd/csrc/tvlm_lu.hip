@@ -19,7 +19,9 @@
 // getrs (n <= 16384): the interchanges are composed per block of 64 pivots and the 64 x 64
 // diagonal blocks of L and U inverted (k_solve_prep), then one work-group per right-hand side
 // applies the interchanges and both blocked substitutions with the column in LDS
-// (k_solve_blocked); larger n uses block-row kernels (k_diag_solve / k_off_update).
+// (k_solve_blocked); from n = 1024 with few right-hand sides, the interchanges are applied alone
+// (k_solve_perm) and each 64-row block step runs on all trailing row blocks at once
+// (k_trsv_step); larger n uses block-row kernels (k_diag_solve / k_off_update).
 #include <map>
 #include <mutex>
 #include <type_traits>
@@ -567,7 +569,8 @@ __global__ __launch_bounds__(NT) void k_panel_reg(T* __restrict__ a, int64_t lda
 // pivot and diagonal rows exchange them, and each group scales and updates its own rows.  Exchange
 // buffers (per matrix): cval / crow [NB][kCoopMaxGroups], xrow [NB][kCoopMaxGroups + 1][NB] (slot
 // kCoopMaxGroups: the diagonal row), counters ctr [NB], zeroed before the launch.  A spin that runs
-// out (the groups were not all resident) leaves info = kCoopStalled instead of waiting for ever.
+// out (the groups were not all resident), or a pivot row outside [c, n) read back from the exchange,
+// leaves info = kCoopStalled instead of waiting for ever or indexing out of bounds.
 // Group 0 writes ipiv and the panel's composed interchanges for k_swap_trsm, as k_panel_reg does.
 constexpr int kCoopMaxGroups = 64;   // work-groups of one launch; all of them must be resident at once
 constexpr int kCoopMinRows = 1024;   // shorter panels stay in one work-group (k_panel_reg)
@@ -677,7 +680,13 @@ __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t ld
                 ri = ld_agent(&crow[j * MG + lane]);
             }
             wave_argmax(v, ri);
-            if (lane == 0) s_p = ri < n ? ri : c;
+            if (lane == 0) {
+                // a pivot row outside [c, n) can only come from a failed exchange: keep every
+                // index below in bounds and report it
+                const bool ok = ri >= c && ri < n;
+                s_p = ok ? ri : c;
+                if (!ok) s_fail = 1;
+            }
         }
         __syncthreads();
         const int p = s_p;
@@ -1051,6 +1060,7 @@ int panel_coop(T* a, int64_t lda, int n, int k0, int kb, int G, int32_t* ipiv, i
     TVLM_HIP(lds_attr);
     TVLM_REQUIRE(G >= 2 && G * batch <= kCoopMaxGroups && kb <= NB && (int64_t)G * NT >= (int64_t)(n - k0),
                  "cooperative panel: %d groups for %d rows", G, n - k0);
+    TVLM_REQUIRE(ctr && cval && crow && xrow, "cooperative panel without exchange buffers");
     hipLaunchKernelGGL((k_panel_coop<T, NT>), dim3(G, 1, batch), dim3(NT), kCoopLdsBytes, st, a, lda, n, k0, kb, G,
                        ipiv, info, perm, sb, ctr, cval, crow, xrow);
     TVLM_LAUNCHED();
@@ -1157,8 +1167,11 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     TVLM_TRY(device_workspace(sizeof(int32_t) * kPermInts * (size_t)batch, &ws, kWsLuPerm, st));
     int32_t* perm = static_cast<int32_t*>(ws);
     if (batch >= kWidePanelBatch && n <= kFastPanelRows) return getrf_lookahead<T>(a, lda, n, batch, sb, ipiv, info, perm, st);
-    // exchange buffers of the cooperative tall panels: per panel and matrix NB counters (zeroed
-    // here, once for the whole factorisation), per matrix the candidates and published rows
+    // Cooperative tall panels only when the first panel already is one; then every panel of more
+    // than kCoopMinRows rows is (the group count falls with the rows), all NB wide.  A batch whose
+    // first panels are too tall for them keeps single-group panels throughout.  Exchange buffers:
+    // per cooperative panel and matrix NB counters (zeroed here, once for the whole
+    // factorisation), per matrix the candidates and published rows.
     const int panels = (n + NB - 1) / NB;
     const bool any_coop = coop_groups<T>(n, batch) > 0;
     int32_t* ctr = nullptr;
@@ -1177,21 +1190,26 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
         hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nctr + 255) / 256)), dim3(256), 0, st, ctr, (int64_t)nctr);
         TVLM_LAUNCHED();
     }
-    for (int k0 = 0, pi = 0; k0 < n; pi++) {
+    for (int k0 = 0, ci = 0; k0 < n;) {
         // tall panels of one matrix are narrower: the in-panel left-looking traffic grows with width
         // squared.  A batch is bound by the trailing updates instead, whose traffic (the whole
         // trailing matrix read and written per panel) halves with the full width.  Cooperative
         // panels have no in-panel traffic and take the full width.
-        const int G = coop_groups<T>(n - k0, batch);
+        const int G = any_coop ? coop_groups<T>(n - k0, batch) : 0;
         const int width = (G == 0 && n - k0 > 2048 && batch < kWidePanelBatch) ? NB / 2 : NB;
         const int kb = (n - k0 < width) ? n - k0 : width;
         const int j1 = k0 + kb;
         const int kp = k0;  // this panel's first column
         k0 = j1;
         if (G > 0 || n - kp <= kFastPanelRows) {
-            if (G > 0)
+            if (G > 0) {
+                // cooperative panels are NB wide (the last one may be narrower): at most `panels`
+                // of them, whatever narrower panels came first
+                TVLM_REQUIRE(ci < panels, "cooperative panel %d of at most %d", ci, panels);
                 TVLM_TRY(panel_coop<T>(a, lda, n, kp, kb, G, ipiv, info, perm, st, batch, sb,
-                                       ctr + (size_t)pi * batch * NB, cval, crow, xrow));
+                                       ctr + (size_t)ci * batch * NB, cval, crow, xrow));
+                ci++;
+            }
             else
                 TVLM_TRY(panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb));
             const int nleft = (kp + 63) / 64, nright = (n - j1 + 63) / 64;
@@ -1385,6 +1403,88 @@ __global__ __launch_bounds__(1024) void k_solve_blocked(const T* __restrict__ a,
     for (int i = t; i < n; i += 1024) xc[i] = xs[i];
 }
 
+// Multi-work-group form for large n and few right-hand sides (one work-group streaming the whole
+// factor per right-hand side is bound by what one CU can read).  k_solve_perm applies the
+// interchange lists with the column in LDS (one work-group per rhs); then one k_trsv_step launch per
+// 64-row block and triangle.  Forward step b: work-group j takes row block r = b + 1 + j and
+// subtracts L_{r,b} x_b (x_b final from the previous launch); the group of r = b + 1 then finishes
+// x_r = Linv_r (its updated value).  Backward step b: row block r = b - 1 - j, U_{r,b} x_b, and the
+// group of r = b - 1 finishes x_r = Uinv_r (...).  The first launch of each triangle (b = -1 forward,
+// b = nblk backward) only finishes its first block.  Each off-diagonal block is read once per rhs:
+// 64 consecutive rows per column by each wave, 4 waves splitting the 64 columns.
+constexpr int kTrsvMinN = 1024;     // below this, one work-group per rhs is as fast
+constexpr int kTrsvMaxRhs = 256;    // rhs x matrices; more right-hand sides fill the device anyway
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_solve_perm(int n, const int32_t* __restrict__ pairs, T* __restrict__ x,
+                                                     int64_t ldx, int64_t sx) {
+    __shared__ T xs[kSolveMaxN];
+    const int t = threadIdx.x;
+    const int nblk = (n + kSB - 1) / kSB;
+    pairs += (int64_t)blockIdx.z * nblk * kSolvePairs;
+    T* xc = x + (int64_t)blockIdx.z * sx + (int64_t)blockIdx.x * ldx;
+    for (int i = t; i < n; i += 1024) xs[i] = xc[i];
+    __syncthreads();
+    for (int b = 0; b < nblk; b++) {
+        const int32_t* pl = pairs + (int64_t)b * kSolvePairs;
+        const int np = pl[0];
+        T v = T(0);
+        if (t < np) v = xs[pl[2 + 2 * t]];
+        __syncthreads();
+        if (t < np) xs[pl[1 + 2 * t]] = v;
+        __syncthreads();
+    }
+    for (int i = t; i < n; i += 1024) xc[i] = xs[i];
+}
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_step(const T* __restrict__ a, int64_t lda, int n,
+                                                   const T* __restrict__ inv, T* __restrict__ x, int64_t ldx,
+                                                   int64_t sa, int64_t sx, int b) {
+    __shared__ T xb[kSB];
+    __shared__ T red[4][kSB];
+    __shared__ T yb[kSB];
+    const int t = threadIdx.x, rl = t & 63, g = t >> 6;
+    const int nblk = (n + kSB - 1) / kSB;
+    a += (int64_t)blockIdx.z * sa;
+    inv += (int64_t)blockIdx.z * 2 * nblk * kSB * kSB + (LOWER ? 0 : (int64_t)nblk * kSB * kSB);
+    T* xc = x + (int64_t)blockIdx.z * sx + (int64_t)blockIdx.y * ldx;
+    const int j = blockIdx.x;
+    const int r = LOWER ? b + 1 + j : b - 1 - j;  // this group's row block
+    const int r0 = r * kSB, c0 = b * kSB;
+    const bool has_b = LOWER ? b >= 0 : b < nblk;
+    T y = T(0);
+    if (t < kSB) y = r0 + t < n ? xc[r0 + t] : T(0);
+    if (has_b) {
+        if (t < kSB) xb[t] = c0 + t < n ? xc[c0 + t] : T(0);
+        __syncthreads();
+        T acc = T(0);
+        if (r0 + rl < n) {
+            const T* ab = a + (r0 + rl) + (int64_t)c0 * lda;
+#pragma unroll 4
+            for (int c = 16 * g; c < 16 * g + 16; c++)
+                if (c0 + c < n) acc += ab[(int64_t)c * lda] * xb[c];
+        }
+        red[g][rl] = acc;
+        __syncthreads();
+        if (t < kSB) y -= red[0][t] + red[1][t] + red[2][t] + red[3][t];
+    }
+    if (j != 0) {  // a later block: store the partial update
+        if (t < kSB && r0 + t < n) xc[r0 + t] = y;
+        return;
+    }
+    // the next block in order: finish it with its diagonal-block inverse
+    if (t < kSB) yb[t] = y;
+    __syncthreads();
+    const T* ib = inv + (int64_t)r * kSB * kSB;
+    T s = T(0);
+#pragma unroll
+    for (int p = 16 * g; p < 16 * g + 16; p++) s += ib[rl + p * kSB] * yb[p];
+    red[g][rl] = s;
+    __syncthreads();
+    if (t < kSB && r0 + t < n) xc[r0 + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+}
+
 // shape[2] > 1 solves a batch: matrix b at A.ptr + b A.stride[2] with pivots ipiv + b n, its
 // right-hand sides at X.ptr + b X.stride[2].
 template <typename T>
@@ -1413,9 +1513,26 @@ int getrs(const tvlm_view* av, const int32_t* ipiv, const tvlm_view* xv, void* s
         int32_t* pairs = reinterpret_cast<int32_t*>(static_cast<char*>(ws) + inv_bytes);
         hipLaunchKernelGGL(k_solve_prep<T>, dim3(nblk, 3, batch), dim3(64), 0, st, a, lda, n, ipiv, pairs, inv, sa);
         TVLM_LAUNCHED();
-        hipLaunchKernelGGL(k_solve_blocked<T>, dim3(nrhs, 1, batch), dim3(1024), 0, st, a, lda, n, pairs, inv, x, ldx,
-                           sa, sx);
+        if (n < kTrsvMinN || (int64_t)nrhs * batch > kTrsvMaxRhs) {
+            hipLaunchKernelGGL(k_solve_blocked<T>, dim3(nrhs, 1, batch), dim3(1024), 0, st, a, lda, n, pairs, inv, x,
+                               ldx, sa, sx);
+            TVLM_LAUNCHED();
+            return TVLM_OK;
+        }
+        hipLaunchKernelGGL(k_solve_perm<T>, dim3(nrhs, 1, batch), dim3(1024), 0, st, n, pairs, x, ldx, sx);
         TVLM_LAUNCHED();
+        for (int b = -1; b < nblk - 1; b++) {
+            const int groups = b < 0 ? 1 : nblk - 1 - b;
+            hipLaunchKernelGGL((k_trsv_step<T, true>), dim3(groups, nrhs, batch), dim3(256), 0, st, a, lda, n, inv, x,
+                               ldx, sa, sx, b);
+            TVLM_LAUNCHED();
+        }
+        for (int b = nblk; b > 0; b--) {
+            const int groups = b == nblk ? 1 : b;
+            hipLaunchKernelGGL((k_trsv_step<T, false>), dim3(groups, nrhs, batch), dim3(256), 0, st, a, lda, n, inv, x,
+                               ldx, sa, sx, b);
+            TVLM_LAUNCHED();
+        }
         return TVLM_OK;
     }
     for (int z = 0; z < batch; z++) {

@@ -6,7 +6,7 @@ Structure
   one batch at a time; batches on different lanes run at the same time on the device.
 - plans: per mesh, a small pool of served-mode plans (tvlm_vlm_create, mode 1: seeded FP64
   matrices, one batched FP32 LU, FP64 refinement), created when first needed and moved to whichever
-  lane is free (tvlm_vlm_set_stream).  A plan's batch cap follows from its mesh size:
+  lane is free (tvlm_vlm_set_stream).  A plan's batch cap follows from its mesh size (batch_cap):
   min(max_batch, plan_bytes // (12 N^2)) requests (FP64 + FP32 copy of each request's matrix).
 - memory: the device bytes of all plans (tvlm_vlm_footprint) stay under `budget_bytes`.  A new plan
   that would exceed it first evicts the least recently used idle plans; if that is not enough, its
@@ -137,6 +137,17 @@ class Request:
         return self.finished - self.arrival
 
 
+def batch_cap(N: int, max_batch: int = 32, plan_bytes: int = 4 << 30) -> int:
+    """Requests per plan for a mesh of N panels: as many as plan_bytes holds (12 N^2 bytes each),
+    at most max_batch.  Below 8 requests, tall matrices (N > 1024) keep the batch small enough for
+    the LU's cooperative panels (a batch of b matrices with ceil(N / 512) work-groups each, at most
+    64 work-groups in all); 8 or more take the batched look-ahead LU."""
+    cap = int(max(1, min(max_batch, plan_bytes // (12 * N * N))))
+    if N > 1024 and cap < 8:
+        cap = max(1, min(cap, 64 // ((N + 511) // 512)))
+    return cap
+
+
 def stream_create() -> int:
     s = ctypes.c_void_p()
     H.check(H.lib().tvlm_stream_create(ctypes.byref(s)), "tvlm_stream_create")
@@ -175,7 +186,7 @@ class SteadyServer:
             raise ValueError("lanes and max_batch must be positive")
         self.meshes = {k: [np.asarray(v, dtype=np.float64) for v in vl] for k, vl in meshes.items()}
         self.N = {k: sum((v.shape[0] - 1) * (v.shape[1] - 1) for v in vl) for k, vl in self.meshes.items()}
-        self.cap = {k: int(max(1, min(max_batch, plan_bytes // (12 * n * n)))) for k, n in self.N.items()}
+        self.cap = {k: batch_cap(n, max_batch, plan_bytes) for k, n in self.N.items()}
         if budget_bytes is None:
             import torch
 
