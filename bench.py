@@ -187,6 +187,100 @@ def steady_bench(batch, reps, plain_n, with_cpu):
     return out
 
 
+def serve_mix_bench(span_hi_s, span_lo_s, sat_requests, lanes):
+    """Continuous batching over the committed steady-solve mix (18 meshes, N = 256 .. 16384).
+
+    plain FP64 rate of the mix: one plain solve per (family, n) cell (the cell's first request of
+    the trace), each timed after an untimed solve on the same plan; the mix is uniform over the
+    cells, so the rate is 18 / (sum of the cell times).  Served: one SteadyServer (`lanes` lanes,
+    plans kept warm between runs) serves the first requests of the trace with arrivals at
+    lambda = 1.5x and 0.5x that plain rate (a prefix spanning about span_hi_s / span_lo_s seconds
+    of arrivals), then the first sat_requests requests all at once (capacity).  Latency = completion
+    seen by the host - arrival."""
+    import tempfile
+
+    from tinyvlm_amd import serve as S
+    from tinyvlm_amd.steady_plan import MODE_PLAIN_FP64, SteadyPlan
+
+    trace = S.read_trace()
+    d = tempfile.mkdtemp(prefix="tvlm_mix_")
+    meshes, first = {}, {}
+    reqs = []
+    for i in range(len(trace["gap"])):
+        fam, n = S.MIX_FAMILIES[trace["family"][i]], int(trace["n"][i])
+        key = f"{fam}_{n}"
+        if key not in meshes:
+            meshes[key] = [S.mix_mesh(fam, n, d)]
+        a = float(np.deg2rad(np.float64(trace["alpha_deg"][i])))
+        first.setdefault(key, (i, a))
+        reqs.append((i, key, a, float(trace["gap"][i])))
+    plain_ms, plain_res = {}, {}
+    for key, (i, a) in first.items():
+        p = SteadyPlan(meshes[key], 1, mode=MODE_PLAIN_FP64)
+        p.solve([a])
+        p.results(gamma=False)
+        t0 = time.perf_counter()
+        p.solve([a])
+        plain_res[i] = p.results(gamma=True)
+        plain_ms[key] = (time.perf_counter() - t0) * 1e3
+        p.close()
+    plain_rate = len(first) / (sum(plain_ms.values()) / 1e3)
+
+    def requests(count, lam):
+        out, t = [], 0.0
+        for rid, key, a, g in reqs[:count]:
+            t += g / lam if lam else 0.0
+            out.append(S.Request(rid, key, a, arrival=t))
+        return out
+
+    def summary(rs, st, lam):
+        p50, p99 = S.latency_percentiles(rs)
+        span = max(r.finished for r in rs) - min(r.arrival for r in rs)
+        return {"requests": len(rs), "lambda_per_s": None if lam is None else round(lam, 3),
+                "value": round(len(rs) / span, 2), "unit": "steady solves/s",
+                "latency_ms": {"p50": round(p50 * 1e3, 2), "p99": round(p99 * 1e3, 2)},
+                "batches": st.batches, "mean_batch": round(float(np.mean(st.batch_sizes)), 2),
+                "max_lanes_busy": st.max_running, "plans_created": st.plans_created,
+                "plans_evicted": st.plans_evicted, "peak_plan_GiB": round(st.peak_bytes / 2**30, 2)}
+
+    out = {"workload": "steady_mix_4096_trace (18 meshes: elliptic fixture / rectangular / 25deg swept, "
+                       "n x n panels, n in 16..128)",
+           "trace": "tinyvlm_amd/data/steady_mix_4096.csv (PCG64 seed 0)", "lanes": lanes,
+           "plain_fp64": {"value": round(plain_rate, 3), "unit": "steady solves/s",
+                          "sample": "one timed plain solve per (family, n) cell",
+                          "ms_per_cell": {k: round(v, 2) for k, v in sorted(plain_ms.items())}}}
+    with S.SteadyServer(meshes, lanes=lanes, keep_gamma=True) as srv:
+        # warm-up: one request per mesh creates the plans and sizes the lanes' scratch buffers
+        srv.run([S.Request(i, k, a) for k, (i, a) in first.items()], realtime=False)
+        runs = {}
+        for name, factor, span in (("lambda_1.5x_plain", 1.5, span_hi_s), ("lambda_0.5x_plain", 0.5, span_lo_s)):
+            lam = factor * plain_rate
+            count = int(min(len(reqs), max(32, span * lam)))
+            rs = requests(count, lam)
+            srv.stats = S.ServeStats()
+            st = srv.run(rs)
+            runs[name] = summary(rs, st, lam)
+            if name == "lambda_1.5x_plain":
+                hi = rs
+        rs = requests(min(len(reqs), sat_requests), None)
+        srv.stats = S.ServeStats()
+        st = srv.run(rs, realtime=False)
+        runs["saturation"] = summary(rs, st, None)
+        runs["saturation"]["speedup_vs_plain_fp64"] = round(runs["saturation"]["value"] / plain_rate, 3)
+    out.update(runs)
+    dcl, dg, info = 0.0, 0.0, 0
+    for r in hi:
+        if r.rid in plain_res:
+            pr = plain_res[r.rid]
+            dcl = max(dcl, abs(r.cl - float(pr["cl"][0])))
+            dg = max(dg, float(np.max(np.abs(r.gamma - pr["gamma"][0])) / np.max(np.abs(pr["gamma"][0]))))
+        info = max(info, abs(r.info))
+    out["accuracy"] = {"compared_requests": sum(1 for r in hi if r.rid in plain_res),
+                       "max_abs_dCL_vs_plain_fp64": dcl, "max_rel_dgamma_vs_plain_fp64": dg, "bar": 1e-6,
+                       "accepted": bool(dcl <= 1e-6 and info == 0)}
+    return out
+
+
 PEAK_FP32_VALU_PACKED_TFLOPS = 157.3  # MI355X FP32 vector peak with packed FP32 (not used: see DESIGN)
 
 
@@ -366,6 +460,11 @@ def main():
     ap.add_argument("--steady-reps", type=int, default=3, help="timed served steady batches")
     ap.add_argument("--steady-plain", type=int, default=4, help="timed plain FP64 steady solves")
     ap.add_argument("--no-steady", action="store_true")
+    ap.add_argument("--mix-span-hi", type=float, default=20.0, help="seconds of arrivals at 1.5x the plain rate")
+    ap.add_argument("--mix-span-lo", type=float, default=15.0, help="seconds of arrivals at 0.5x the plain rate")
+    ap.add_argument("--mix-saturation", type=int, default=384, help="requests of the saturation run")
+    ap.add_argument("--mix-lanes", type=int, default=4)
+    ap.add_argument("--no-mix", action="store_true")
     ap.add_argument("--hb-speeds", type=int, default=8, help="airspeeds of the timed HB continuation")
     ap.add_argument("--no-hb", action="store_true")
     ap.add_argument("--two-dof-steps", type=int, default=400, help="timed coupled pitch/heave steps")
@@ -486,6 +585,8 @@ def main():
         out["steady"] = steady_bench(args.steady_batch, args.steady_reps, args.steady_plain, not args.no_cpu)
     if not args.no_steady:
         out["assembly"] = assembly_bench()
+    if not args.no_mix:
+        out["served_mix"] = serve_mix_bench(args.mix_span_hi, args.mix_span_lo, args.mix_saturation, args.mix_lanes)
     if not args.no_hb:
         out["hb"] = hb_bench(args.hb_speeds, not args.no_cpu)
     if not args.no_two_dof:
