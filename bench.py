@@ -192,7 +192,7 @@ def serve_mix_bench(span_hi_s, span_lo_s, sat_requests, lanes):
 
     plain FP64 rate of the mix: one plain solve per (family, n) cell (the cell's first request of
     the trace), each timed after an untimed solve on the same plan; the mix is uniform over the
-    cells, so the rate is 18 / (sum of the cell times).  Served: one SteadyServer (`lanes` lanes,
+    cells, so the rate is 18 / (sum of the cell times).  Served: one Server (`lanes` lanes,
     plans kept warm between runs) serves the first requests of the trace with arrivals at
     lambda = 1.5x and 0.5x that plain rate (a prefix spanning about span_hi_s / span_lo_s seconds
     of arrivals), then the first sat_requests requests all at once (capacity).  Latency = completion
@@ -249,7 +249,7 @@ def serve_mix_bench(span_hi_s, span_lo_s, sat_requests, lanes):
            "plain_fp64": {"value": round(plain_rate, 3), "unit": "steady solves/s",
                           "sample": "one timed plain solve per (family, n) cell",
                           "ms_per_cell": {k: round(v, 2) for k, v in sorted(plain_ms.items())}}}
-    with S.SteadyServer(meshes, lanes=lanes, keep_gamma=True) as srv:
+    with S.Server(meshes, lanes=lanes, keep_gamma=True) as srv:
         # warm-up: one request per mesh creates the plans and sizes the lanes' scratch buffers
         srv.run([S.Request(i, k, a) for k, (i, a) in first.items()], realtime=False)
         runs = {}
@@ -279,6 +279,65 @@ def serve_mix_bench(span_hi_s, span_lo_s, sat_requests, lanes):
                        "max_abs_dCL_vs_plain_fp64": dcl, "max_rel_dgamma_vs_plain_fp64": dg, "bar": 1e-6,
                        "accepted": bool(dcl <= 1e-6 and info == 0)}
     return out
+
+
+def lu_probe(n, reps=2):
+    """Single-matrix getrf (LU::factorize) of a random n x n matrix, FP32 and FP64, on a non-default
+    stream: one untimed factorisation, then `reps` timed ones between HIP events."""
+    import torch
+
+    from tinyvlm_amd import capi as H
+
+    out = {"probe": "getrf", "n": n}
+    st = torch.cuda.Stream()
+    for name, dt in (("f32", torch.float32), ("f64", torch.float64)):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(0)
+        a0 = torch.randn(n, n, dtype=dt, device="cuda", generator=g).t().contiguous().t()
+        a = a0.clone()
+        ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+        info = torch.zeros(1, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        with torch.cuda.stream(st):
+            a.copy_(a0)
+            H.getrf(a, ipiv, info, stream=st)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ms = 0.0
+            for _ in range(reps):
+                a.copy_(a0)
+                e0.record(st)
+                H.getrf(a, ipiv, info, stream=st)
+                e1.record(st)
+                e1.synchronize()
+                ms += e0.elapsed_time(e1)
+        ms /= reps
+        out[name] = {"ms": round(ms, 3), "tflops": round(2.0 / 3.0 * n ** 3 / (ms * 1e-3) / 1e12, 2),
+                     "info": int(info.item())}
+        del a, a0
+    return out
+
+
+def serve_uvlm_bench(verts, dt, cl_alone, plain_value, n_requests, lanes):
+    """Unsteady requests through the serving loop: n_requests simulations of the headline workload
+    (same wing, motion and 500 steps) all waiting at the start, `lanes` lanes, one served UVLM plan
+    per lane (hipGraph replay).  value = n_requests x 500 / (last completion - start); each CL(t)
+    history is compared with the single served simulation of the headline measurement."""
+    from tinyvlm_amd import serve as S
+
+    reqs = [S.Request(i, "ar4_10x40", kind="unsteady", motion=dict(MOTION)) for i in range(n_requests)]
+    with S.Server({"ar4_10x40": [verts]}, lanes=lanes, unsteady={"ar4_10x40": (T_STEPS, dt)}) as srv:
+        srv.run([S.Request(-1 - k, "ar4_10x40", kind="unsteady", motion=dict(MOTION)) for k in range(lanes)],
+                realtime=False)  # warm-up: one plan per lane, graphs captured
+        srv.stats = S.ServeStats()
+        st = srv.run(reqs, realtime=False)
+    span = max(r.finished for r in reqs)
+    p50, p99 = S.latency_percentiles(reqs)
+    value = n_requests * (T_STEPS - 1) / span
+    return {"workload": "uvlm_pitch_heave_ar4_10c_x_40s_500steps, concurrent requests", "requests": n_requests,
+            "lanes": lanes, "value": round(value, 1), "unit": "UVLM steps/s (all requests)",
+            "latency_ms_per_sim": {"p50": round(p50 * 1e3, 2), "p99": round(p99 * 1e3, 2)},
+            "max_lanes_busy": st.max_running, "speedup_vs_plain_fp64": round(value / plain_value, 3),
+            "cl_history_bitwise_equal_to_alone": bool(all(np.array_equal(r.cl_t, cl_alone) for r in reqs))}
 
 
 PEAK_FP32_VALU_PACKED_TFLOPS = 157.3  # MI355X FP32 vector peak with packed FP32 (not used: see DESIGN)
@@ -465,6 +524,9 @@ def main():
     ap.add_argument("--mix-saturation", type=int, default=384, help="requests of the saturation run")
     ap.add_argument("--mix-lanes", type=int, default=4)
     ap.add_argument("--no-mix", action="store_true")
+    ap.add_argument("--uvlm-requests", type=int, default=32, help="concurrent UVLM requests of the served_uvlm run")
+    ap.add_argument("--lu-probe", type=int, default=0, help="only time getrf of one N x N matrix (FP32, FP64)")
+    ap.add_argument("--mix-only", action="store_true", help="only the served_mix section")
     ap.add_argument("--hb-speeds", type=int, default=8, help="airspeeds of the timed HB continuation")
     ap.add_argument("--no-hb", action="store_true")
     ap.add_argument("--two-dof-steps", type=int, default=400, help="timed coupled pitch/heave steps")
@@ -484,6 +546,12 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     capi.init(0)
+    if args.lu_probe:
+        print(json.dumps(lu_probe(args.lu_probe)))
+        return
+    if args.mix_only:
+        print(json.dumps(serve_mix_bench(args.mix_span_hi, args.mix_span_lo, args.mix_saturation, args.mix_lanes)))
+        return
     verts = rectangular_wing_f32(NS, NC, CHORD, HALF_SPAN)
     dt = time_step(verts, MOTION["u"])
     steps_per_sim = T_STEPS - 1
@@ -581,6 +649,9 @@ def main():
         out["cpu_baseline"] = None
     fast.close()
     plain.close()
+    if not args.no_mix:
+        out["served_uvlm"] = serve_uvlm_bench(verts, dt, res_fast["cl"], plain_value, args.uvlm_requests,
+                                              args.mix_lanes)
     if not args.no_steady:
         out["steady"] = steady_bench(args.steady_batch, args.steady_reps, args.steady_plain, not args.no_cpu)
     if not args.no_steady:

@@ -1,4 +1,5 @@
-"""GPU checks of continuous batching (tinyvlm_amd.serve.SteadyServer) on the steady-solve mix.
+"""GPU checks of continuous batching (tinyvlm_amd.serve.Server) on the steady-solve mix, and on
+steady and unsteady requests served together.
 
 - alone vs in the mix: every request served in a mix of meshes, arriving while others run,
   equals the same request solved alone (a one-request served plan): circulation within 1e-10
@@ -8,8 +9,13 @@
 - overlap: requests were admitted while others were still on the device (more than one lane busy
   at once, batches of more than one request);
 - memory admission: with a budget that holds only two plans, plans are evicted and re-created and
-  every result is still the alone result.
+  every result is still the alone result;
+- unsteady requests (whole UVLM simulations, 10 x 8 wing, 60 steps, different motions) served
+  between steady ones on the same lanes: every CL(t) history equals the same simulation run alone
+  (bitwise: the same kernels on the same operands) and the oracle's UVLM::run within 1e-9, and the
+  steady requests still equal their alone results.
 """
+import math
 import os
 
 import numpy as np
@@ -71,12 +77,12 @@ def _check_same(r, alone):
 
 
 def test_mix_equals_alone_and_overlaps(mix):
-    from tinyvlm_amd.serve import SteadyServer
+    from tinyvlm_amd.serve import Server
 
     meshes, reqs = mix
     print(f"[case] serve mix {len(reqs)} requests over {len(meshes)} meshes", flush=True)
     requests = _requests(reqs, 2e-3)
-    with SteadyServer(meshes, lanes=4, max_batch=8, keep_gamma=True) as srv:
+    with Server(meshes, lanes=4, max_batch=8, keep_gamma=True) as srv:
         st = srv.run(requests)
     assert all(np.isfinite(r.finished) for r in requests)
     assert st.max_running >= 2 and max(st.batch_sizes) >= 2, (st.max_running, st.batch_sizes)
@@ -89,12 +95,12 @@ def test_mix_equals_alone_and_overlaps(mix):
 
 
 def test_mix_smallest_mesh_matches_plain_and_oracle(mix, oracle):
-    from tinyvlm_amd.serve import SteadyServer
+    from tinyvlm_amd.serve import Server
     from tinyvlm_amd.steady_plan import MODE_PLAIN_FP64, SteadyPlan
 
     meshes, reqs = mix
     requests = _requests(reqs, 0.0)
-    with SteadyServer(meshes, lanes=3, max_batch=16, keep_gamma=True) as srv:
+    with Server(meshes, lanes=3, max_batch=16, keep_gamma=True) as srv:
         srv.run(requests, realtime=False)
     small = [r for r in requests if r.mesh == "elliptic_16"][:4]
     assert small
@@ -113,20 +119,60 @@ def test_mix_smallest_mesh_matches_plain_and_oracle(mix, oracle):
 
 
 def test_mix_under_small_memory_budget(mix):
-    from tinyvlm_amd.serve import SteadyServer
+    from tinyvlm_amd.serve import Server
 
     meshes, reqs = mix
     reqs = reqs[:36]
     sizes = {}
-    with SteadyServer(meshes, lanes=2, max_batch=4, budget_bytes=1 << 40) as probe:
+    with Server(meshes, lanes=2, max_batch=4, budget_bytes=1 << 40) as probe:
         for k in meshes:
             sizes[k] = probe._estimate(k)
     budget = 2 * max(sizes.values()) + 1
     print(f"[case] serve mix with budget {budget} bytes", flush=True)
     requests = _requests(reqs, 1e-3)
-    with SteadyServer(meshes, lanes=2, max_batch=4, budget_bytes=budget, keep_gamma=True) as srv:
+    with Server(meshes, lanes=2, max_batch=4, budget_bytes=budget, keep_gamma=True) as srv:
         st = srv.run(requests)
         assert srv.used_bytes <= budget
     assert st.plans_evicted > 0 and st.peak_bytes <= budget
     for r in requests:
         _check_same(r, _alone(meshes, r))
+
+
+def test_unsteady_requests_in_the_mix(mix, oracle):
+    from tinyvlm_amd.serve import Request, Server
+    from tinyvlm_amd.uvlm_plan import MODE_FAST, UVLMPlan, time_step
+
+    meshes, reqs = mix
+    wing = oracle.rectangular_wing(8, 10, chord=1.0, half_span=2.0)
+    ts, dt = 61, time_step(wing, 1.0)
+    meshes = dict(meshes, wing10x8=[wing])
+    a_amp = 2.0 * math.pi / 180.0
+    motions = [dict(u=1.0, h_amp=0.1 * (1 + k % 3), h_phase=0.0, a_amp=a_amp * (k % 2 + 1), a_phase=math.pi / 2,
+                    omega=1.0 + 0.25 * k, x_pivot=0.25) for k in range(6)]
+    requests = []
+    t = 0.0
+    for j, (rid, key, a, g) in enumerate(reqs[:30]):
+        t += g * 2e-3
+        requests.append(Request(rid, key, a, arrival=t))
+        if j % 5 == 0:
+            requests.append(Request(10000 + j, "wing10x8", arrival=t, kind="unsteady", motion=motions[j // 5]))
+    print(f"[case] serve {len(requests)} steady + unsteady requests", flush=True)
+    with Server(meshes, lanes=4, max_batch=8, keep_gamma=True, unsteady={"wing10x8": (ts, dt)}) as srv:
+        st = srv.run(requests)
+    assert st.max_running >= 2
+    unsteady = [r for r in requests if r.kind == "unsteady"]
+    assert len(unsteady) == 6
+    for r in unsteady:
+        p = UVLMPlan([wing], ts, dt, mode=MODE_FAST)
+        p.set_pitch_heave(**r.motion)
+        p.run(use_graph=True)
+        alone = p.results()
+        p.close()
+        assert np.array_equal(r.cl_t, alone["cl"]), r.rid
+        assert np.array_equal(r.cm_t, alone["cm"]), r.rid
+        mot = [r.motion[k] for k in ("u", "h_amp", "h_phase", "a_amp", "a_phase", "omega", "x_pivot")]
+        cl_o, _, _, _ = oracle.uvlm_run([wing], mot, t_steps=ts)
+        assert np.max(np.abs(r.cl_t[1:] - cl_o[1:])) <= 1e-9, r.rid
+    for r in requests:
+        if r.kind == "steady":
+            _check_same(r, _alone(meshes, r))

@@ -132,6 +132,10 @@ def views(ts) -> ctypes.Array:
 
 
 def stream_ptr(stream=None):
+    """hipStream_t of a torch stream, of a raw handle (int, e.g. from tvlm_stream_create) or of
+    torch's current stream."""
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
     torch = _torch()
     s = stream if stream is not None else torch.cuda.current_stream()
     return ctypes.c_void_p(s.cuda_stream)

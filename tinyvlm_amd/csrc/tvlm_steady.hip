@@ -23,8 +23,8 @@
 namespace {
 constexpr int kMaxS = TVLM_MAX_SURFACES;
 constexpr int kRefine = 3;
-constexpr int kResidRows = 256;
-constexpr int kResidMaxN = 8192;  // x in LDS
+constexpr int kResidRows = 64;     // rows per work-group of the residual
+constexpr int kResidWaves = 8;     // waves splitting its columns
 }  // namespace
 
 struct tvlm_vlm_plan {
@@ -58,26 +58,37 @@ struct tvlm_vlm_plan {
 namespace tvlm {
 namespace {
 
-// r = rhs - A x per request (FP64), one row per thread; x staged in LDS up to kResidMaxN rows,
-// read from global memory (the same for every thread, L2-resident) beyond
-template <bool STAGED>
-__global__ __launch_bounds__(kResidRows) void k_residual(const double* __restrict__ A, int n, const double* __restrict__ rhs,
-                                                         const double* __restrict__ x, double* __restrict__ r) {
-    __shared__ double xs[STAGED ? kResidMaxN : 1];
+// r = rhs - A x per request (FP64).  A work-group takes 64 rows (one per lane) and its 8 waves
+// split the columns (wave w: columns w, w + 8, ...), so each wave reads 64 consecutive rows of a
+// column (512 contiguous bytes) and N / 8 work-groups' worth of waves keep enough loads in flight
+// to stream the FP64 matrix at HBM rate; x[j] is the same for the whole wave; the 8 partial sums
+// are added in LDS in a fixed order.
+__global__ __launch_bounds__(kResidRows * kResidWaves) void k_residual(const double* __restrict__ A, int n,
+                                                                      const double* __restrict__ rhs,
+                                                                      const double* __restrict__ x,
+                                                                      double* __restrict__ r) {
+    __shared__ double part[kResidWaves][kResidRows];
     const int64_t b = blockIdx.y;
     A += b * (int64_t)n * n;
     rhs += b * n;
     x += b * n;
     r += b * n;
-    if constexpr (STAGED) {
-        for (int i = threadIdx.x; i < n; i += kResidRows) xs[i] = x[i];
-        __syncthreads();
-    }
-    const int i = blockIdx.x * kResidRows + threadIdx.x;
-    if (i >= n) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * kResidRows + lane;
     double acc = 0.0;
-    for (int j = 0; j < n; j++) acc += A[i + (int64_t)j * n] * (STAGED ? xs[j] : x[j]);
-    r[i] = rhs[i] - acc;
+    if (i < n) {
+        const double* Ai = A + i;
+#pragma unroll 4
+        for (int j = w; j < n; j += kResidWaves) acc += Ai[(int64_t)j * n] * x[j];
+    }
+    part[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && i < n) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < kResidWaves; q++) s += part[q][lane];
+        r[i] = rhs[i] - s;
+    }
 }
 
 __global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst, int64_t count) {
@@ -335,12 +346,8 @@ int tvlm_vlm_solve_n(tvlm_vlm_plan* P, int32_t n_active, const double* alpha, co
                 hipLaunchKernelGGL(k_to_f32, dim3(grid_for(N * B)), dim3(256), 0, st, P->rhs, P->r32, N * B);
             } else {
                 const dim3 rg((unsigned)((N + kResidRows - 1) / kResidRows), B);
-                if (N <= kResidMaxN)
-                    hipLaunchKernelGGL(k_residual<true>, rg, dim3(kResidRows), 0, st, P->lhs, (int)N, P->rhs, P->x,
-                                       P->res);
-                else
-                    hipLaunchKernelGGL(k_residual<false>, rg, dim3(kResidRows), 0, st, P->lhs, (int)N, P->rhs, P->x,
-                                       P->res);
+                hipLaunchKernelGGL(k_residual, rg, dim3(kResidRows * kResidWaves), 0, st, P->lhs, (int)N, P->rhs,
+                                   P->x, P->res);
                 TVLM_LAUNCHED();
                 hipLaunchKernelGGL(k_to_f32, dim3(grid_for(N * B)), dim3(256), 0, st, P->res, P->r32, N * B);
             }

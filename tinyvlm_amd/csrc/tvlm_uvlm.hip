@@ -851,15 +851,17 @@ int tvlm_uvlm_time_wake(tvlm_uvlm_plan* P, double* total_ms, int32_t* launches, 
 }
 
 // Copies (synchronously) CL[t_steps-1], CM[t_steps-1][3] (step 0 entries are 0), the last
-// circulation gamma[N] and the LU info; any pointer may be null.
+// circulation gamma[N] and the LU info; any pointer may be null.  The copies are queued on the
+// plan's stream (not the null stream), so they do not wait for work on other streams.
 int tvlm_uvlm_results(tvlm_uvlm_plan* P, double* cl, double* cm, double* gamma, int32_t* info) {
     TVLM_REQUIRE(P, "null plan");
-    TVLM_HIP(hipStreamSynchronize(P->stream));
+    hipStream_t st = P->stream;
     const size_t n = (size_t)(P->t_steps - 1);
-    if (cl) TVLM_HIP(hipMemcpy(cl, P->cl, sizeof(double) * n, hipMemcpyDeviceToHost));
-    if (cm) TVLM_HIP(hipMemcpy(cm, P->cm, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
-    if (gamma) TVLM_HIP(hipMemcpy(gamma, P->gamma, sizeof(double) * P->N, hipMemcpyDeviceToHost));
-    if (info) TVLM_HIP(hipMemcpy(info, P->info, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (cl) TVLM_HIP(hipMemcpyAsync(cl, P->cl, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (cm) TVLM_HIP(hipMemcpyAsync(cm, P->cm, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+    if (gamma) TVLM_HIP(hipMemcpyAsync(gamma, P->gamma, sizeof(double) * P->N, hipMemcpyDeviceToHost, st));
+    if (info) TVLM_HIP(hipMemcpyAsync(info, P->info, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    TVLM_HIP(hipStreamSynchronize(st));
     return TVLM_OK;
 }
 

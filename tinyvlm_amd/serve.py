@@ -1,5 +1,7 @@
-"""Continuous batching of steady solves over a mix of meshes: the serving form of the hot path's
-caller VLM::run (vlm/src/vlm.cpp:97-151), one request = one (mesh, angle of attack) solve.
+"""Continuous batching of steady and unsteady solves over a mix of meshes: the serving form of the
+hot path's callers VLM::run (vlm/src/vlm.cpp:97-151; a steady request = one (mesh, angle of attack)
+solve) and UVLM::run (vlm/src/vlm.cpp:358-509; an unsteady request = one whole prescribed-motion
+simulation).
 
 Structure
 - lanes: `lanes` streams, each with a hardware queue of its own (tvlm_stream_create).  A lane runs
@@ -11,15 +13,20 @@ Structure
 - memory: the device bytes of all plans (tvlm_vlm_footprint) stay under `budget_bytes`.  A new plan
   that would exceed it first evicts the least recently used idle plans; if that is not enough, its
   mesh waits for memory while other meshes are served.
-- admission: whenever a lane is free, the mesh whose oldest waiting request arrived first is
-  served: an idle plan of that mesh (or a new one) takes up to its cap of that mesh's waiting
-  requests, in arrival order.  Requests that arrive while others are on the device are admitted as
-  soon as a lane and a plan (memory) are free, whatever else is running.
+- unsteady requests: one device-resident UVLM plan (tvlm_uvlm_*, served mode, hipGraph of the time
+  loop) per (mesh, lane), created on that lane when first needed and kept there (its graph and the
+  scratch buffers it captured belong to the lane's stream); one simulation per launch.
+- admission: whenever a lane is free, the queue (steady: per mesh; unsteady: per mesh) whose oldest
+  waiting request arrived first is served: for steady requests an idle plan of that mesh (or a new
+  one) takes up to its cap of that mesh's waiting requests, in arrival order.  Requests that arrive
+  while others are on the device are admitted as soon as a lane and a plan (memory) are free,
+  whatever else is running.
 
 A request's result does not depend on the batch it shared: each matrix of a batched LU is factored
 independently, and the FP64 refinement drives every solution to the FP64 solution of its own FP64
 system; only FP32 rounding inside the LU (panel width, GEMM kernel choice, which follow the batch
-size) differs, and the refinement removes it to ~1e-14 relative.
+size) differs, and the refinement removes it to ~1e-14 relative.  An unsteady simulation runs the
+same kernels on the same operands whatever runs beside it: its CL(t) is bitwise the alone result.
 
 The request mix used to measure the server is `steady_mix_trace`: 4096 requests, PCG64 seed 0,
 exponential inter-arrival gaps (unit rate, scaled by 1/lambda when run), mesh family in
@@ -42,6 +49,7 @@ from . import ROOT_DIR
 from . import capi as H
 from .mesh_io import read_plot3d
 from .steady_plan import MODE_SERVED, SteadyPlan
+from .uvlm_plan import MODE_FAST, UVLMPlan
 
 MIX_FAMILIES = ("elliptic", "rectangular", "swept25")
 MIX_SIZES = (16, 32, 45, 64, 90, 128)
@@ -119,10 +127,15 @@ def mix_mesh(family: str, n: int, cache_dir: str | None = None) -> np.ndarray:
 # ---- requests and lanes -------------------------------------------------------------------------
 @dataclass
 class Request:
+    """A steady request (alpha) or, with kind = "unsteady", one UVLM simulation of `mesh` under the
+    pitch/heave motion `motion` (UVLMPlan.set_pitch_heave's arguments); its results are the CL / CM
+    histories cl_t [t_steps - 1] and cm_t [t_steps - 1, 3]."""
     rid: int
     mesh: str
-    alpha: float  # radians
+    alpha: float = 0.0  # radians (steady)
     arrival: float = 0.0  # seconds after the start of the run
+    kind: str = "steady"
+    motion: dict | None = None
     admitted: float = math.nan
     finished: float = math.nan
     batch: int = 0  # requests in the batch it ran in
@@ -131,6 +144,8 @@ class Request:
     cm: np.ndarray | None = None
     gamma: np.ndarray | None = None
     info: int = 0
+    cl_t: np.ndarray | None = None
+    cm_t: np.ndarray | None = None
 
     @property
     def latency(self) -> float:
@@ -160,10 +175,11 @@ def stream_destroy(s: int) -> None:
 
 @dataclass
 class _Slot:
-    plan: SteadyPlan
+    plan: object  # SteadyPlan, or UVLMPlan for unsteady requests
     bytes: int
     busy: bool = False
     last_used: float = 0.0
+    unsteady: bool = False
 
 
 @dataclass
@@ -177,11 +193,12 @@ class ServeStats:
     batch_sizes: list = field(default_factory=list)
 
 
-class SteadyServer:
+class Server:
     def __init__(self, meshes: dict, lanes: int = 4, max_batch: int = 32, plan_bytes: int = 4 << 30,
                  budget_bytes: int | None = None, plans_per_mesh: int | None = None, u_inf: float = 1.0,
-                 rho: float = 1.0, keep_gamma: bool = False):
-        """meshes: name -> list of verts arrays (one per surface, as read_plot3d returns them)."""
+                 rho: float = 1.0, keep_gamma: bool = False, unsteady: dict | None = None):
+        """meshes: name -> list of verts arrays (one per surface, as read_plot3d returns them).
+        unsteady: name -> (t_steps, dt) for the meshes that take unsteady requests."""
         if lanes < 1 or max_batch < 1:
             raise ValueError("lanes and max_batch must be positive")
         self.meshes = {k: [np.asarray(v, dtype=np.float64) for v in vl] for k, vl in meshes.items()}
@@ -196,6 +213,11 @@ class SteadyServer:
         self.u_inf, self.rho, self.keep_gamma = float(u_inf), float(rho), bool(keep_gamma)
         self.lanes = [stream_create() for _ in range(lanes)]
         self.slots: dict[str, list[_Slot]] = {k: [] for k in self.meshes}
+        self.unsteady = dict(unsteady or {})
+        for k in self.unsteady:
+            if k not in self.meshes:
+                raise KeyError(f"unsteady mesh {k!r} is not among the meshes")
+        self.uslots: dict[tuple[str, int], _Slot] = {}
         self.used_bytes = 0
         self.stats = ServeStats()
 
@@ -241,16 +263,38 @@ class SteadyServer:
         self.stats.peak_bytes = max(self.stats.peak_bytes, self.used_bytes)
         return s
 
+    def _uslot_for(self, key: str, lane: int, now: float) -> _Slot | None:
+        s = self.uslots.get((key, lane))
+        if s is not None:
+            return None if s.busy else s
+        t_steps, dt = self.unsteady[key]
+        n = self.N[key]
+        need = 16 * n * n + 8 * n * t_steps + 256 * n + (1 << 20)  # LHS, inverse, rhs of all steps, wake
+        if self.used_bytes + need > self.budget and not self._evict_for(need, now):
+            return None
+        plan = UVLMPlan(self.meshes[key], t_steps, dt, mode=MODE_FAST, stream=self.lanes[lane])
+        s = _Slot(plan, need, unsteady=True)
+        self.uslots[(key, lane)] = s
+        self.used_bytes += need
+        self.stats.plans_created += 1
+        self.stats.peak_bytes = max(self.stats.peak_bytes, self.used_bytes)
+        return s
+
     # -- serving loop ------------------------------------------------------------------------------
     def run(self, requests: list[Request], realtime: bool = True, poll_s: float = 20e-6) -> ServeStats:
         """Serves every request; fills in admitted / finished (seconds after the start), cl, cm,
-        info (and gamma with keep_gamma).  realtime: request r becomes visible at r.arrival on the
-        wall clock; otherwise all requests wait from the start (a saturation run)."""
+        info (and gamma with keep_gamma; cl_t, cm_t for unsteady requests).  realtime: request r
+        becomes visible at r.arrival on the wall clock; otherwise all requests wait from the start
+        (a saturation run)."""
         for r in requests:
-            if r.mesh not in self.meshes:
+            if r.kind == "steady" and r.mesh not in self.meshes:
                 raise KeyError(f"request {r.rid}: unknown mesh {r.mesh!r}")
+            if r.kind == "unsteady" and (r.mesh not in self.unsteady or r.motion is None):
+                raise KeyError(f"request {r.rid}: mesh {r.mesh!r} takes no unsteady requests, or no motion")
+            if r.kind not in ("steady", "unsteady"):
+                raise ValueError(f"request {r.rid}: unknown kind {r.kind!r}")
         order = sorted(requests, key=lambda r: (r.arrival, r.rid)) if realtime else list(requests)
-        waiting: dict[str, deque] = {k: deque() for k in self.meshes}
+        waiting: dict[tuple[str, str], deque] = {}
         free_lanes = deque(range(len(self.lanes)))
         running: dict[int, tuple[_Slot, list[Request]]] = {}
         nxt, finished = 0, 0
@@ -259,12 +303,28 @@ class SteadyServer:
         while finished < len(order):
             now = time.perf_counter() - t0
             while nxt < len(order) and (not realtime or order[nxt].arrival <= now):
-                waiting[order[nxt].mesh].append(order[nxt])
+                waiting.setdefault((order[nxt].kind, order[nxt].mesh), deque()).append(order[nxt])
                 nxt += 1
             progressed = False
             for lane in list(running):
                 slot, batch = running[lane]
                 if not slot.plan.done():
+                    continue
+                if slot.unsteady:
+                    res = slot.plan.results()
+                    r = batch[0]
+                    r.finished = time.perf_counter() - t0
+                    r.cl_t, r.cm_t = res["cl"], res["cm"]
+                    r.cl = float(res["cl"][-1])
+                    r.cm = res["cm"][-1].copy()
+                    if self.keep_gamma:
+                        r.gamma = res["gamma"]
+                    slot.busy = False
+                    slot.last_used = r.finished
+                    del running[lane]
+                    free_lanes.append(lane)
+                    finished += 1
+                    progressed = True
                     continue
                 res = slot.plan.results(gamma=self.keep_gamma)
                 t = time.perf_counter() - t0
@@ -285,16 +345,33 @@ class SteadyServer:
                 heads = sorted((q[0].arrival, q[0].rid, k) for k, q in waiting.items() if q)
                 chosen = None
                 for _, _, k in heads:
-                    slot = self._slot_for(k, free_lanes[0], now)
+                    kind, mesh = k
+                    if kind == "unsteady":
+                        slot = self._uslot_for(mesh, free_lanes[0], now)
+                    else:
+                        slot = self._slot_for(mesh, free_lanes[0], now)
                     if slot is not None:
                         chosen = (k, slot)
                         break
                 if chosen is None:
                     break
-                k, slot = chosen
+                (kind, mesh), slot = chosen
                 lane = free_lanes.popleft()
-                q = waiting[k]
-                batch = [q.popleft() for _ in range(min(self.cap[k], len(q)))]
+                q = waiting[(kind, mesh)]
+                if kind == "unsteady":
+                    r = q.popleft()
+                    slot.busy = True
+                    slot.plan.set_pitch_heave(**r.motion)
+                    slot.plan.run(use_graph=True)
+                    r.admitted = time.perf_counter() - t0
+                    r.batch, r.lane = 1, lane
+                    running[lane] = (slot, [r])
+                    st.batches += 1
+                    st.batch_sizes.append(1)
+                    st.max_running = max(st.max_running, len(running))
+                    progressed = True
+                    continue
+                batch = [q.popleft() for _ in range(min(self.cap[mesh], len(q)))]
                 slot.plan.set_stream(self.lanes[lane])
                 slot.busy = True
                 slot.plan.solve(np.asarray([r.alpha for r in batch]), u_inf=self.u_inf, rho=self.rho)
@@ -321,6 +398,9 @@ class SteadyServer:
             for s in ss:
                 s.plan.close()
             ss.clear()
+        for s in self.uslots.values():
+            s.plan.close()
+        self.uslots.clear()
         self.used_bytes = 0
         for s in self.lanes:
             stream_destroy(s)

@@ -49,6 +49,8 @@ class UVLMPlan:
         self.t_steps = int(t_steps)
         self.dt = float(dt)
         self.mode = mode
+        # a torch stream or a raw hipStream_t (int); the plan's graph is captured on it, and the
+        # library's scratch buffers it uses belong to it, so a plan stays on its stream
         self.stream = stream if stream is not None else torch.cuda.Stream()
         self.N = sum((v.shape[0] - 1) * (v.shape[1] - 1) for v in verts_init_list)
         self._verts = [np.asfortranarray(np.asarray(v, dtype=np.float64)) for v in verts_init_list]
@@ -93,6 +95,12 @@ class UVLMPlan:
         H.check(H.lib().tvlm_uvlm_time_wake(self._plan, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(pairs)),
                 "tvlm_uvlm_time_wake")
         return ms.value, n.value, pairs.value
+
+    def done(self) -> bool:
+        """True once everything queued on the plan's stream has finished (does not block)."""
+        d = ctypes.c_int32()
+        H.check(H.lib().tvlm_stream_query(H.stream_ptr(self.stream), ctypes.byref(d)), "tvlm_stream_query")
+        return bool(d.value)
 
     def close(self):
         if self._plan:
