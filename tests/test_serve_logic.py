@@ -1,0 +1,182 @@
+"""CPU checks of the serving loop's host logic (tinyvlm_amd.serve.Server) with stand-in plans in
+place of the device plans: every request is answered from its own batch slot, batches respect the
+mesh's cap and the lane count, no plan takes a second batch while one is running, the memory budget
+holds (least recently used idle plans are evicted), admission is oldest-first across meshes, and
+unsteady requests run on plans bound to one lane."""
+import itertools
+
+import numpy as np
+import pytest
+
+
+class FakePlan:
+    live = []
+
+    def __init__(self, verts_list, batch, mode=1, stream=None):
+        self.N = sum((v.shape[0] - 1) * (v.shape[1] - 1) for v in verts_list)
+        self.batch, self.stream = batch, stream
+        self.polls = 0
+        self.busy = False
+        self.closed = False
+        self.alphas = None
+        FakePlan.live.append(self)
+
+    def solve(self, alpha, u_inf=1.0, rho=1.0):
+        assert not self.busy and not self.closed
+        assert 1 <= len(alpha) <= self.batch
+        self.alphas = np.asarray(alpha, dtype=np.float64)
+        self.busy = True
+        self.polls = 3
+
+    def done(self):
+        self.polls -= 1
+        return self.polls <= 0
+
+    def results(self, gamma=True):
+        assert self.busy
+        self.busy = False
+        a = self.alphas
+        out = {"cl": 2 * np.pi * a, "cm": np.stack([a, -a, 0 * a], axis=1), "info": np.zeros(len(a), np.int32)}
+        if gamma:
+            out["gamma"] = np.repeat(a[:, None], self.N, axis=1)
+        return out
+
+    def set_stream(self, s):
+        assert not self.busy
+        self.stream = s
+
+    def footprint(self):
+        return 12 * self.N * self.N * self.batch + 400 * self.N * self.batch
+
+    def close(self):
+        assert not self.busy
+        self.closed = True
+
+
+class FakeUVLM:
+    def __init__(self, verts_list, t_steps, dt, mode=1, stream=None):
+        self.stream, self.t_steps = stream, t_steps
+        self.motion = None
+        self.polls = 0
+
+    def set_pitch_heave(self, **m):
+        self.motion = m
+
+    def run(self, use_graph=True):
+        self.polls = 2
+
+    def done(self):
+        self.polls -= 1
+        return self.polls <= 0
+
+    def results(self):
+        n = self.t_steps - 1
+        cl = np.full(n, self.motion["h_amp"])
+        return {"cl": cl, "cm": np.zeros((n, 3)), "gamma": np.zeros(4)}
+
+    def close(self):
+        pass
+
+
+@pytest.fixture
+def S(monkeypatch):
+    from tinyvlm_amd import serve
+
+    counter = itertools.count(1000)
+    monkeypatch.setattr(serve, "stream_create", lambda: next(counter))
+    monkeypatch.setattr(serve, "stream_destroy", lambda s: None)
+    monkeypatch.setattr(serve, "SteadyPlan", FakePlan)
+    monkeypatch.setattr(serve, "UVLMPlan", FakeUVLM)
+    FakePlan.live = []
+    return serve
+
+
+def _mesh(n):
+    v = np.zeros((n + 1, n + 1, 4))
+    return [v]
+
+
+def _requests(S, spec):
+    return [S.Request(i, key, alpha, arrival=0.0) for i, (key, alpha) in enumerate(spec)]
+
+
+def test_results_batches_and_lanes(S):
+    meshes = {"a": _mesh(16), "b": _mesh(32), "c": _mesh(64)}
+    rng = np.random.default_rng(0)
+    spec = [(str(rng.choice(list(meshes))), float(rng.uniform(-0.1, 0.2))) for _ in range(200)]
+    reqs = _requests(S, spec)
+    with S.Server(meshes, lanes=3, max_batch=5, budget_bytes=1 << 40, keep_gamma=True) as srv:
+        st = srv.run(reqs, realtime=False)
+        assert st.max_running <= 3
+        assert max(st.batch_sizes) <= 5 and sum(st.batch_sizes) == 200
+    for r in reqs:
+        assert r.cl == pytest.approx(2 * np.pi * r.alpha)
+        assert r.cm[0] == pytest.approx(r.alpha) and r.cm[1] == pytest.approx(-r.alpha)
+        assert np.all(r.gamma == r.alpha)
+        assert r.finished >= r.admitted >= r.arrival
+
+
+def test_batch_cap_follows_mesh_size(S):
+    assert S.batch_cap(256) == 32 and S.batch_cap(4096) == 21
+    assert S.batch_cap(8100) == 4 and S.batch_cap(16384) == 1
+    for n in (1500, 2025, 8100, 16384):  # batches under 8 keep the LU's panels cooperative
+        cap = S.batch_cap(n)
+        assert cap >= 8 or cap * ((n + 511) // 512) <= 64
+
+
+def test_memory_budget_and_eviction(S):
+    meshes = {"a": _mesh(32), "b": _mesh(32), "c": _mesh(32)}  # any two plans fit, three do not
+    with S.Server(meshes, lanes=2, max_batch=4, budget_bytes=1 << 40) as probe:
+        est = {k: probe._estimate(k) for k in meshes}
+    budget = 2 * max(est.values())
+    spec = [(k, 0.01 * i) for i, k in enumerate(["a", "b", "c"] * 20)]
+    reqs = _requests(S, spec)
+    with S.Server(meshes, lanes=2, max_batch=4, budget_bytes=budget) as srv:
+        st = srv.run(reqs, realtime=False)
+        assert srv.used_bytes <= budget
+    assert st.plans_evicted > 0 and st.peak_bytes <= budget
+    assert all(np.isfinite(r.cl) for r in reqs)
+    with pytest.raises(RuntimeError, match="exceeds the memory budget"):
+        with S.Server(meshes, lanes=1, budget_bytes=10) as srv:
+            srv.run(_requests(S, [("a", 0.0)]), realtime=False)
+
+
+def test_oldest_request_first(S):
+    meshes = {"a": _mesh(8), "b": _mesh(8)}
+    reqs = [S.Request(0, "b", 0.1, arrival=0.0), S.Request(1, "a", 0.2, arrival=0.001),
+            S.Request(2, "b", 0.3, arrival=0.002)]
+    with S.Server(meshes, lanes=1, max_batch=8, budget_bytes=1 << 40) as srv:
+        srv.run(reqs, realtime=False)
+    # one lane: the b batch (requests 0 and 2, both waiting) goes first, then a
+    assert reqs[0].admitted == reqs[2].admitted < reqs[1].admitted
+    assert reqs[0].batch == 2 and reqs[1].batch == 1
+
+
+def test_unsteady_requests_stay_on_their_lane(S):
+    meshes = {"w": _mesh(8), "s": _mesh(16)}
+    reqs = []
+    for i in range(12):
+        reqs.append(S.Request(i, "w", kind="unsteady", motion=dict(u=1.0, h_amp=0.01 * i)))
+        reqs.append(S.Request(100 + i, "s", 0.05 * i))
+    with S.Server(meshes, lanes=3, budget_bytes=1 << 40, unsteady={"w": (11, 0.1)}) as srv:
+        srv.run(reqs, realtime=False)
+        for (key, lane), slot in srv.uslots.items():
+            assert slot.plan.stream == srv.lanes[lane]
+    for r in reqs:
+        if r.kind == "unsteady":
+            assert len(r.cl_t) == 10 and np.all(r.cl_t == r.motion["h_amp"])
+        else:
+            assert r.cl == pytest.approx(2 * np.pi * r.alpha)
+
+
+def test_request_errors(S):
+    meshes = {"w": _mesh(8)}
+    with S.Server(meshes, lanes=1, budget_bytes=1 << 40) as srv:
+        with pytest.raises(KeyError):
+            srv.run([S.Request(0, "nope", 0.0)], realtime=False)
+        with pytest.raises(KeyError):
+            srv.run([S.Request(0, "w", kind="unsteady", motion={"u": 1.0})], realtime=False)
+        with pytest.raises(ValueError):
+            srv.run([S.Request(0, "w", kind="other")], realtime=False)
+    with pytest.raises(KeyError):
+        S.Server(meshes, lanes=1, budget_bytes=1 << 40, unsteady={"x": (11, 0.1)})
