@@ -535,7 +535,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the workload is independent per request and would shard by request, see DESIGN.md)",
+              "(the requests are independent and would shard across GPUs by request, one process per GPU)",
               file=sys.stderr)
         sys.exit(2)
 
