@@ -562,9 +562,10 @@ __global__ __launch_bounds__(NT) void k_panel_reg(T* __restrict__ a, int64_t lda
 // k_panel_coop: G work-groups of NT threads factor the panel rows [k0, n) x columns [k0, k0 + kb)
 // together (kb <= NB).  Thread t of group g owns row k0 + g NT + t; the group keeps its NT rows of
 // the panel in LDS, column-major (consecutive threads, consecutive banks), from the first column to
-// the last.  Per column, one exchange through global memory: every group publishes its local pivot
-// candidate (|value|, row) and that row's values, group 0 publishes the diagonal row, then the
-// groups meet at a per-column counter (one atomic add each, then a bounded spin).  Every group then
+// the last.  Per column, one exchange through global memory: the first wave of every group
+// publishes the group's local pivot candidate (|value|, row) and that row's values (group 0 also
+// the diagonal row) from LDS, releases once, and counts the group in at a per-column counter (one
+// atomic add each, then a bounded spin).  Every group then
 // picks the same pivot (largest |value|, first row on ties: LAPACK's choice), the owners of the
 // pivot and diagonal rows exchange them, and each group scales and updates its own rows.  Exchange
 // buffers (per matrix): cval / crow [NB][kCoopMaxGroups], xrow [NB][kCoopMaxGroups + 1][NB] (slot
@@ -641,33 +642,28 @@ __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t ld
         bi = s_idx[0];
 #pragma unroll
         for (int ww = 1; ww < NW; ww++) argmax_merge(best, bi, s_val[ww], s_idx[ww]);
-        // publish the candidate, its row and (group 0) the diagonal row, then meet
+        // publish and meet: wave 0 writes this group's candidate row and (group 0) the diagonal row
+        // straight from LDS, then the candidate, releases once, and its lane 0 counts the group in
+        // and waits for the others; the other waves go straight to the barrier
         T* xr = xrow + (int64_t)j * (MG + 1) * NB;
-        bool wrote = false;
-        if (bi < n && t < kb) {  // bi is one of this group's rows
-            st_agent(&xr[g * NB + t], xs[t * NT + (bi - k0 - g * NT)]);
-            wrote = true;
-        }
-        if (g == 0 && t >= 64 && t - 64 < kb) {  // row c = k0 + j: group 0, local row j
-            st_agent(&xr[MG * NB + (t - 64)], xs[(t - 64) * NT + j]);
-            wrote = true;
-        }
-        if (t == 0) {
-            st_agent(&cval[j * MG + g], best);
-            st_agent(&crow[j * MG + g], bi);
-            wrote = true;
-        }
-        if (wrote) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (t == 0) {
-            __hip_atomic_fetch_add(&ctr[j], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            unsigned spins = 0;
-            while (__hip_atomic_load(&ctr[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < G) {
-                if (++spins > kCoopSpinLimit) {
-                    s_fail = 1;
-                    break;
+        if (w == 0) {
+            if (bi < n && lane < kb) st_agent(&xr[g * NB + lane], xs[lane * NT + (bi - k0 - g * NT)]);
+            if (g == 0 && lane < kb) st_agent(&xr[MG * NB + lane], xs[lane * NT + j]);  // row c: local row j
+            if (lane == 0) {
+                st_agent(&cval[j * MG + g], best);
+                st_agent(&crow[j * MG + g], bi);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane == 0) {
+                __hip_atomic_fetch_add(&ctr[j], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                unsigned spins = 0;
+                while (__hip_atomic_load(&ctr[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < G) {
+                    if (++spins > kCoopSpinLimit) {
+                        s_fail = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
                 }
-                __builtin_amdgcn_s_sleep(1);
             }
         }
         __syncthreads();
