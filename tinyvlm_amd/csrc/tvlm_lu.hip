@@ -1246,7 +1246,7 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
 //
 // k_solve_prep, y = 0: composes the interchanges of pivot block b (64 positions) into (dst, src)
 //   pairs (one wave); y = 1 / 2: inverts the 64 x 64 diagonal block b of L (unit lower) / U (upper),
-//   one column per lane.
+//   one column per lane, each row's dot product split over 4 waves.
 // k_solve_blocked: the rhs column lives in LDS; the interchange lists are applied block after
 //   block; then x_b = Linv_b (x_b - L_{b,<b} x_{<b}) for b ascending and x_b = Uinv_b (x_b -
 //   U_{b,>b} x_{>b}) for b descending, each off-diagonal product split over 16 thread groups
@@ -1256,59 +1256,72 @@ constexpr int kSolveMaxN = 16384;  // rhs column in LDS
 constexpr int kSolvePairs = 1 + 2 * 2 * kSB;
 
 template <typename T>
-__global__ __launch_bounds__(64) void k_solve_prep(const T* __restrict__ a, int64_t lda, int n,
-                                                   const int32_t* __restrict__ ipiv, int32_t* __restrict__ pairs,
-                                                   T* __restrict__ inv, int64_t sa) {
+__global__ __launch_bounds__(256) void k_solve_prep(const T* __restrict__ a, int64_t lda, int n,
+                                                    const int32_t* __restrict__ ipiv, int32_t* __restrict__ pairs,
+                                                    T* __restrict__ inv, int64_t sa) {
     __shared__ int s_p[kSB];
+    __shared__ int s_pd[2 * kSB], s_ps[2 * kSB];
+    __shared__ T s_a[kSB][kSB + 1];
+    __shared__ T s_x[kSB][kSB + 1];  // the inverse, row by row
+    __shared__ T s_red[4][kSB];
     const int nb_ = (n + kSB - 1) / kSB;  // batch: one matrix per blockIdx.z
     a += (int64_t)blockIdx.z * sa;
     ipiv += (int64_t)blockIdx.z * n;
     pairs += (int64_t)blockIdx.z * nb_ * kSolvePairs;
     inv += (int64_t)blockIdx.z * 2 * nb_ * kSB * kSB;
-    __shared__ int s_pd[2 * kSB], s_ps[2 * kSB];
-    __shared__ T s_a[kSB][kSB + 1];
-    const int b = blockIdx.x, lane = threadIdx.x;
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, g = t >> 6;
     const int r0 = b * kSB, rb = min(kSB, n - r0);
     if (blockIdx.y == 0) {
-        if (lane < rb) s_p[lane] = ipiv[r0 + lane] - 1;
+        if (t < kSB && lane < rb) s_p[lane] = ipiv[r0 + lane] - 1;
         __syncthreads();
-        const int np = wave_compose_swaps(r0, rb, s_p, s_pd, s_ps);
-        int32_t* out = pairs + (int64_t)b * kSolvePairs;
-        for (int e = lane; e < np; e += 64) {
-            out[1 + 2 * e] = s_pd[e];
-            out[2 + 2 * e] = s_ps[e];
+        if (t < kSB) {
+            const int np = wave_compose_swaps(r0, rb, s_p, s_pd, s_ps);
+            int32_t* out = pairs + (int64_t)b * kSolvePairs;
+            for (int e = lane; e < np; e += 64) {
+                out[1 + 2 * e] = s_pd[e];
+                out[2 + 2 * e] = s_ps[e];
+            }
+            if (lane == 0) out[0] = np;
         }
-        if (lane == 0) out[0] = np;
         return;
     }
     const bool lower = blockIdx.y == 1;
-    for (int e = lane; e < kSB * kSB; e += 64) {
+    for (int e = t; e < kSB * kSB; e += 256) {
         const int r = e % kSB, c = e / kSB;
         s_a[r][c] = (r < rb && c < rb) ? a[(r0 + r) + (int64_t)(r0 + c) * lda] : (r == c ? T(1) : T(0));
     }
     __syncthreads();
+    // Column j of the inverse belongs to lane j; row i of it is formed from the rows already done,
+    // the dot product over them split over the 4 waves (wave g takes p = g, g + 4, ...) and summed
+    // in a fixed order.  Unit lower: X_ij = delta_ij - sum_{j <= p < i} L_ip X_pj, rows ascending;
+    // upper: X_ij = (delta_ij - sum_{i < p <= j} U_ip X_pj) / U_ii, rows descending.
     const int j = lane;
-    T col[kSB];
-    if (lower) {
-#pragma unroll
-        for (int i = 0; i < kSB; i++) {
-            T v = i == j ? T(1) : T(0);
-#pragma unroll
-            for (int p = 0; p < i; p++) v -= (p >= j ? s_a[i][p] * col[p] : T(0));
-            col[i] = i < j ? T(0) : v;
+    for (int k = 0; k < kSB; k++) {
+        const int i = lower ? k : kSB - 1 - k;
+        T part = T(0);
+        if (lower) {
+            for (int p = j + ((g - j) % 4 + 4) % 4; p < i; p += 4) part += s_a[i][p] * s_x[p][j];
+        } else {
+            for (int p = i + 1 + g; p <= j; p += 4) part += s_a[i][p] * s_x[p][j];
         }
-    } else {
-#pragma unroll
-        for (int i = kSB - 1; i >= 0; i--) {
-            T v = i == j ? T(1) : T(0);
-#pragma unroll
-            for (int p = i + 1; p < kSB; p++) v -= (p <= j ? s_a[i][p] * col[p] : T(0));
-            col[i] = i > j ? T(0) : v / s_a[i][i];
+        s_red[g][j] = part;
+        __syncthreads();
+        if (g == 0) {
+            const T sum = (s_red[0][j] + s_red[1][j]) + (s_red[2][j] + s_red[3][j]);
+            T v;
+            if (lower)
+                v = i < j ? T(0) : (i == j ? T(1) : -sum);
+            else
+                v = i > j ? T(0) : ((i == j ? T(1) : T(0)) - sum) / s_a[i][i];
+            s_x[i][j] = v;
         }
+        __syncthreads();
     }
     T* out = inv + ((int64_t)(lower ? 0 : 1) * ((n + kSB - 1) / kSB) + b) * kSB * kSB;
-#pragma unroll
-    for (int i = 0; i < kSB; i++) out[i + (int64_t)j * kSB] = col[i];
+    for (int e = t; e < kSB * kSB; e += 256) {
+        const int r = e % kSB, c = e / kSB;
+        out[r + (int64_t)c * kSB] = s_x[r][c];
+    }
 }
 
 template <typename T>
@@ -1507,7 +1520,7 @@ int getrs(const tvlm_view* av, const int32_t* ipiv, const tvlm_view* xv, void* s
         TVLM_TRY(device_workspace(inv_bytes + sizeof(int32_t) * (size_t)nblk * kSolvePairs * batch, &ws, kWsSolveInv, st));
         T* inv = static_cast<T*>(ws);
         int32_t* pairs = reinterpret_cast<int32_t*>(static_cast<char*>(ws) + inv_bytes);
-        hipLaunchKernelGGL(k_solve_prep<T>, dim3(nblk, 3, batch), dim3(64), 0, st, a, lda, n, ipiv, pairs, inv, sa);
+        hipLaunchKernelGGL(k_solve_prep<T>, dim3(nblk, 3, batch), dim3(256), 0, st, a, lda, n, ipiv, pairs, inv, sa);
         TVLM_LAUNCHED();
         if (n < kTrsvMinN || (int64_t)nrhs * batch > kTrsvMaxRhs) {
             hipLaunchKernelGGL(k_solve_blocked<T>, dim3(nrhs, 1, batch), dim3(1024), 0, st, a, lda, n, pairs, inv, x,
