@@ -61,7 +61,9 @@ int tvlm_stream_synchronize(void* stream);
  * backends use one implicit stream): tvlm_stream_create makes a default-priority stream with a
  * hardware queue of its own (all-CUs mask; unmasked streams share a small pool of queues and two
  * streams on one queue run one after the other).  tvlm_stream_query sets *done = 1 once all
- * work queued on the stream has finished, 0 while some is still running; it does not block. */
+ * work queued on the stream has finished, 0 while some is still running; it does not block.
+ * tvlm_stream_destroy waits for the stream and also frees what the library kept for it (scratch
+ * buffers, its rocBLAS handle, its LU look-ahead side stream). */
 int tvlm_stream_create(void** stream);
 int tvlm_stream_destroy(void* stream);
 int tvlm_stream_query(void* stream, int32_t* done);

@@ -55,6 +55,10 @@ inline bool plane_contiguous(const tvlm_view& v) {
 // solves on separate streams are safe; work on one stream is ordered by the stream.
 enum WorkspaceSlot { kWsWakePartials = 0, kWsLuPerm = 1, kWsSolveInv = 2, kWsLuCoop = 3, kWorkspaceSlots = 4 };
 int device_workspace(size_t bytes, void** out, int slot, hipStream_t stream);
+// Frees what the library keeps for one stream on the current device (its scratch buffers, its
+// rocBLAS handle, its look-ahead side stream); the stream's work must have finished.
+void release_workspaces(hipStream_t stream);
+void release_lu_stream(hipStream_t stream);
 
 // A stream with a hardware queue of its own: created with an all-CUs mask (streams created
 // without a mask share a small pool of queues, and two streams on one queue run one after the

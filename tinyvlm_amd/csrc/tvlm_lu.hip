@@ -888,13 +888,14 @@ __global__ __launch_bounds__(256) void k_update_mfma(T* __restrict__ C, int64_t 
 // device workspace, so GEMMs of plans running concurrently on different streams must not share one.
 constexpr int kBlasUpdateRows = 512;
 
+std::mutex g_blas_mu;
+std::map<std::pair<int, hipStream_t>, rocblas_handle> g_blas_handles;
+
 rocblas_handle blas_handle(hipStream_t st) {
-    static std::mutex mu;
-    static std::map<std::pair<int, hipStream_t>, rocblas_handle> handles;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-    std::lock_guard<std::mutex> lock(mu);
-    rocblas_handle& h = handles[std::make_pair(dev, st)];
+    std::lock_guard<std::mutex> lock(g_blas_mu);
+    rocblas_handle& h = g_blas_handles[std::make_pair(dev, st)];
     if (!h) {
         if (rocblas_create_handle(&h) != rocblas_status_success) {
             h = nullptr;
@@ -1076,13 +1077,14 @@ struct LookaheadStreams {
     hipEvent_t narrow_done = nullptr, rest_done = nullptr;
 };
 
+std::mutex g_lookahead_mu;
+std::map<std::pair<int, hipStream_t>, LookaheadStreams> g_lookahead;
+
 int lookahead_streams(hipStream_t caller, LookaheadStreams** out) {
-    static std::mutex mu;
-    static std::map<std::pair<int, hipStream_t>, LookaheadStreams> sets;
     int dev = 0;
     TVLM_HIP(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lock(mu);
-    LookaheadStreams& L = sets[std::make_pair(dev, caller)];
+    std::lock_guard<std::mutex> lock(g_lookahead_mu);
+    LookaheadStreams& L = g_lookahead[std::make_pair(dev, caller)];
     if (!L.side) {
         TVLM_TRY(create_queue_stream(&L.side));
         TVLM_HIP(hipEventCreateWithFlags(&L.narrow_done, hipEventDisableTiming));
@@ -1576,7 +1578,37 @@ int getrs(const tvlm_view* av, const int32_t* ipiv, const tvlm_view* xv, void* s
     return TVLM_OK;
 }
 
+void release_blas_handle(int dev, hipStream_t stream) {
+    std::lock_guard<std::mutex> lock(g_blas_mu);
+    auto it = g_blas_handles.find(std::make_pair(dev, stream));
+    if (it == g_blas_handles.end()) return;
+    if (it->second) (void)rocblas_destroy_handle(it->second);
+    g_blas_handles.erase(it);
+}
+
 }  // namespace
+
+void release_lu_stream(hipStream_t stream) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    release_blas_handle(dev, stream);
+    LookaheadStreams L;
+    {
+        std::lock_guard<std::mutex> lock(g_lookahead_mu);
+        auto it = g_lookahead.find(std::make_pair(dev, stream));
+        if (it == g_lookahead.end()) return;
+        L = it->second;
+        g_lookahead.erase(it);
+    }
+    if (L.side) {
+        (void)hipStreamSynchronize(L.side);
+        release_blas_handle(dev, L.side);
+        (void)hipStreamDestroy(L.side);
+    }
+    if (L.narrow_done) (void)hipEventDestroy(L.narrow_done);
+    if (L.rest_done) (void)hipEventDestroy(L.rest_done);
+}
+
 }  // namespace tvlm
 
 extern "C" {

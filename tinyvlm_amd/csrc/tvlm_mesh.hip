@@ -42,6 +42,18 @@ std::map<std::tuple<int, int, void*>, Workspace> g_ws;
 
 }  // namespace
 
+void release_workspaces(hipStream_t stream) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    for (int slot = 0; slot < kWorkspaceSlots; slot++) {
+        auto it = g_ws.find(std::make_tuple(dev, slot, static_cast<void*>(stream)));
+        if (it == g_ws.end()) continue;
+        if (it->second.ptr) (void)hipFree(it->second.ptr);
+        g_ws.erase(it);
+    }
+}
+
 int create_queue_stream(hipStream_t* out) {
     int dev = 0, cus = 0;
     TVLM_HIP(hipGetDevice(&dev));
@@ -457,6 +469,8 @@ int tvlm_stream_create(void** stream) {
 int tvlm_stream_destroy(void* stream) {
     if (!stream) return TVLM_OK;
     TVLM_HIP(hipStreamSynchronize(as_stream(stream)));
+    release_lu_stream(as_stream(stream));
+    release_workspaces(as_stream(stream));
     TVLM_HIP(hipStreamDestroy(as_stream(stream)));
     return TVLM_OK;
 }
