@@ -96,3 +96,29 @@ def test_hb_zero_motion(root, oracle):
     F = np.ones((3, 3))
     p.run(0.8, 6.0, np.zeros((3, 3)), F)
     assert np.max(np.abs(F)) < 1e-14
+
+
+def test_hb_graph_replay_equals_call_by_call(root):
+    """The Anderson map replayed from a hipGraph (every iteration after the second, with the loop's
+    own launches, copies and host reads between the replays) gives bitwise the loads, coefficients
+    and iteration count of the same solve launched call by call, over a two-point continuation."""
+    from tinyvlm_amd.hb import HBVLM
+
+    paths = [os.path.join(root, "mesh", m) for m in ("3dof_wing_9x5.x", "3dof_flap_3x5.x")]
+    K = 5
+    runs = {}
+    for use_graph in (False, True):
+        print(f"[case] hb graph={use_graph}", flush=True)
+        p = HBVLM("hip", paths)
+        p.use_graph = use_graph
+        p.init(2, SCALE)
+        out = []
+        for U in (0.9, 1.1):
+            F = np.zeros((3, K))
+            p.run(0.8, U, _motion(K), F)
+            out.append((F.copy(), p.iterations, p.gamma_coeffs.cpu().numpy().copy()))
+        runs[use_graph] = out
+    for (Fe, ie, ge), (Fg, ig, gg) in zip(runs[False], runs[True]):
+        assert ie == ig
+        assert np.array_equal(Fe, Fg)
+        assert np.array_equal(ge, gg)

@@ -158,6 +158,10 @@ class HBVLM:
         self.N = sum(ns * nc for ns, nc in self.dims)
         self.iterations = 0
         self.harmonics = None
+        # the solve runs on a stream of its own; the map evaluated in every Anderson iteration is
+        # replayed from a hipGraph (use_graph = False: launched call by call, the same kernels)
+        self.stream = torch.cuda.Stream()
+        self.use_graph = True
 
     # ---- HBVLM::init (lib/libhbvlm3.cpp:168-233) ------------------------------------------------
     def init(self, harmonics: int, scaling: float):
@@ -209,7 +213,11 @@ class HBVLM:
         cycles = 2.0
         t_final = cycles * 2.0 * PI_F / omega
         fs_node, nodes = build_kinematics(dyn_f, omega, v, self.harmonics)
-        return self.solve(t_final, omega, v, fs_node, nodes, force_t)
+        self.torch.cuda.current_stream().synchronize()
+        with self.torch.cuda.stream(self.stream):
+            out = self.solve(t_final, omega, v, fs_node, nodes, force_t)
+        self.stream.synchronize()
+        return out
 
     def _pose(self, nodes, t, verts, colloc, normals, areas):
         """displace_wing to the pose at t and the panel metrics there; returns the dual transform
@@ -304,9 +312,29 @@ class HBVLM:
             H.gemm(1.0, self.rhs, dft_d, 0.0, g_out)
             H.getrs(self.lhs, self.ipiv, g_out)
 
+        # The map's launches are the same at every call of this solve (same buffers, poses and wake
+        # steps): the first call runs them one by one (which also sizes the library's scratch
+        # buffers on this stream), the second captures them into a hipGraph, which every later
+        # call replays.  Calls on other buffers run one by one.
+        state = {"calls": 0, "graph": None, "ptrs": None}
+
+        def hb_map(x_in, x_out):
+            state["calls"] += 1
+            ptrs = (x_in.data_ptr(), x_out.data_ptr())
+            if not self.use_graph or state["calls"] == 1 or (state["ptrs"] not in (None, ptrs)):
+                hb_vlm_iter(x_in, x_out)
+                return
+            if state["graph"] is None:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=self.stream):
+                    hb_vlm_iter(x_in, x_out)
+                state["graph"], state["ptrs"] = g, ptrs
+            state["graph"].replay()
+
         x = self.gamma_coeffs.t().reshape(-1)  # the {N, K} buffer as one vector (a view)
         assert x.data_ptr() == self.gamma_coeffs.data_ptr()
-        self.iterations = self.anderson.run(x, hb_vlm_iter, 100, 1e-12)
+        self.iterations = self.anderson.run(x, hb_map, 100, 1e-12)
+        state["graph"] = None
 
         # forces at the K times of the first period (lib/libhbvlm3.cpp:424-549)
         H.gemm(1.0, self.gamma_coeffs, dft_d, 0.0, self.gamma_wing, trans_b=True)
