@@ -436,7 +436,7 @@ def hb_bench(n_speeds, with_cpu):
         "latency_ms_per_solve": {"p50": round(float(np.percentile(lat, 50)), 2),
                                  "p99": round(float(np.percentile(lat, 99)), 2)},
         "anderson_iterations": iters, "dtype": "f64",
-        "note": "continuation: each airspeed warm-started from the previous solution; launch-bound driver",
+        "note": "continuation: each airspeed warm-started from the previous solution; the Anderson map (replayed from a hipGraph) is bound by its 11 exact-form FP64 wake inductions",
     }
     if with_cpu:
         from oracle import hb as ohb
