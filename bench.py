@@ -192,7 +192,7 @@ def serve_mix_bench(span_hi_s, span_lo_s, sat_requests, lanes):
 
     plain FP64 rate of the mix: one plain solve per (family, n) cell (the cell's first request of
     the trace), each timed after an untimed solve on the same plan; the mix is uniform over the
-    cells, so the rate is 18 / (sum of the cell times).  Served: one Server (`lanes` lanes,
+    cells, so the rate is 18 / (sum of the cell times).  Served: one Server (`lanes` lanes, 16 GiB plans,
     plans kept warm between runs) serves the first requests of the trace with arrivals at
     lambda = 1.5x and 0.5x that plain rate (a prefix spanning about span_hi_s / span_lo_s seconds
     of arrivals), then the first sat_requests requests all at once (capacity).  Latency = completion
@@ -522,7 +522,8 @@ def main():
     ap.add_argument("--mix-span-hi", type=float, default=20.0, help="seconds of arrivals at 1.5x the plain rate")
     ap.add_argument("--mix-span-lo", type=float, default=15.0, help="seconds of arrivals at 0.5x the plain rate")
     ap.add_argument("--mix-saturation", type=int, default=384, help="requests of the saturation run")
-    ap.add_argument("--mix-lanes", type=int, default=4)
+    ap.add_argument("--mix-lanes", type=int, default=3)
+    ap.add_argument("--uvlm-lanes", type=int, default=4, help="lanes of the served_uvlm run")
     ap.add_argument("--no-mix", action="store_true")
     ap.add_argument("--uvlm-requests", type=int, default=32, help="concurrent UVLM requests of the served_uvlm run")
     ap.add_argument("--lu-probe", type=int, default=0, help="only time getrf of one N x N matrix (FP32, FP64)")
@@ -651,7 +652,7 @@ def main():
     plain.close()
     if not args.no_mix:
         out["served_uvlm"] = serve_uvlm_bench(verts, dt, res_fast["cl"], plain_value, args.uvlm_requests,
-                                              args.mix_lanes)
+                                              args.uvlm_lanes)
     if not args.no_steady:
         out["steady"] = steady_bench(args.steady_batch, args.steady_reps, args.steady_plain, not args.no_cpu)
     if not args.no_steady:

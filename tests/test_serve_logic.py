@@ -117,11 +117,13 @@ def test_results_batches_and_lanes(S):
 
 
 def test_batch_cap_follows_mesh_size(S):
-    assert S.batch_cap(256) == 32 and S.batch_cap(4096) == 21
-    assert S.batch_cap(8100) == 4 and S.batch_cap(16384) == 1
+    assert S.batch_cap(256) == 32 and S.batch_cap(4096, plan_bytes=4 << 30) == 21
+    assert S.batch_cap(8100, plan_bytes=4 << 30) == 4 and S.batch_cap(16384, plan_bytes=4 << 30) == 1
+    assert S.batch_cap(4096) == 32 and S.batch_cap(8100) == 21 and S.batch_cap(16384) == 2
     for n in (1500, 2025, 8100, 16384):  # batches under 8 keep the LU's panels cooperative
-        cap = S.batch_cap(n)
-        assert cap >= 8 or cap * ((n + 511) // 512) <= 64
+        for pb in (1 << 30, 4 << 30, 16 << 30):
+            cap = S.batch_cap(n, plan_bytes=pb)
+            assert cap >= 8 or cap * ((n + 511) // 512) <= 64
 
 
 def test_memory_budget_and_eviction(S):

@@ -4,12 +4,13 @@ solve) and UVLM::run (vlm/src/vlm.cpp:358-509; an unsteady request = one whole p
 simulation).
 
 Structure
-- lanes: `lanes` streams, each with a hardware queue of its own (tvlm_stream_create).  A lane runs
-  one batch at a time; batches on different lanes run at the same time on the device.
+- lanes: `lanes` streams (default 3), each with a hardware queue of its own (tvlm_stream_create).
+  A lane runs one batch at a time; batches on different lanes run at the same time on the device.
 - plans: per mesh, a small pool of served-mode plans (tvlm_vlm_create, mode 1: seeded FP64
   matrices, one batched FP32 LU, FP64 refinement), created when first needed and moved to whichever
   lane is free (tvlm_vlm_set_stream).  A plan's batch cap follows from its mesh size (batch_cap):
-  min(max_batch, plan_bytes // (12 N^2)) requests (FP64 + FP32 copy of each request's matrix).
+  min(max_batch, plan_bytes // (12 N^2)) requests (FP64 + FP32 copy of each request's matrix;
+  plan_bytes default 16 GiB).
 - memory: the device bytes of all plans (tvlm_vlm_footprint) stay under `budget_bytes`.  A new plan
   that would exceed it first evicts the least recently used idle plans; if that is not enough, its
   mesh waits for memory while other meshes are served.
@@ -152,7 +153,7 @@ class Request:
         return self.finished - self.arrival
 
 
-def batch_cap(N: int, max_batch: int = 32, plan_bytes: int = 4 << 30) -> int:
+def batch_cap(N: int, max_batch: int = 32, plan_bytes: int = 16 << 30) -> int:
     """Requests per plan for a mesh of N panels: as many as plan_bytes holds (12 N^2 bytes each),
     at most max_batch.  Below 8 requests, tall matrices (N > 1024) keep the batch small enough for
     the LU's cooperative panels (a batch of b matrices with ceil(N / 512) work-groups each, at most
@@ -194,7 +195,7 @@ class ServeStats:
 
 
 class Server:
-    def __init__(self, meshes: dict, lanes: int = 4, max_batch: int = 32, plan_bytes: int = 4 << 30,
+    def __init__(self, meshes: dict, lanes: int = 3, max_batch: int = 32, plan_bytes: int = 16 << 30,
                  budget_bytes: int | None = None, plans_per_mesh: int | None = None, u_inf: float = 1.0,
                  rho: float = 1.0, keep_gamma: bool = False, unsteady: dict | None = None):
         """meshes: name -> list of verts arrays (one per surface, as read_plot3d returns them).
