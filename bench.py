@@ -658,6 +658,13 @@ def main():
     if not args.no_steady:
         out["assembly"] = assembly_bench()
     if not args.no_mix:
+        lu = lu_probe(16384)
+        for name, peak in (("f32", 157.3), ("f64", 78.6)):
+            lu[name]["peak_mfma_tflops"] = peak
+            lu[name]["frac"] = round(lu[name]["tflops"] / peak, 4)
+        lu["note"] = ("single-matrix getrf (LU::factorize) at the mix's largest size; the served N = 16384 "
+                      "requests factor in FP32, the plain path in FP64; panels are latency-bound")
+        out["lu"] = lu
         out["served_mix"] = serve_mix_bench(args.mix_span_hi, args.mix_span_lo, args.mix_saturation, args.mix_lanes)
     if not args.no_hb:
         out["hb"] = hb_bench(args.hb_speeds, not args.no_cpu)
