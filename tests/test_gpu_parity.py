@@ -435,7 +435,7 @@ def test_error_status_no_launch(H, torch):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("n,batch", [(130, 5), (1500, 3), (200, 9), (1100, 8), (2200, 8),
+@pytest.mark.parametrize("n,batch", [(130, 5), (1500, 3), (200, 9), (1100, 8), (2200, 8), (6400, 2),
                                      pytest.param(5000, 7, marks=pytest.mark.skip(
                                          reason="faulted the GPU through a host bug (cooperative panels started part-way "
                                                 "with no exchange buffers); that sequence is switched off and this case "
@@ -445,8 +445,9 @@ def test_lu_batched(H, torch, prec, n, batch):
     (LAPACK pivots for FP64, per-matrix info, solutions to the single-matrix bar).  Batches of 8 or
     more take the look-ahead factorisation (64-column panels, split trailing updates).  Smaller
     batches of tall matrices take cooperative panels (several work-groups per matrix) when the
-    first panel's groups fit (1500 x 3); n = 5000, batch 7 has too many groups for that and keeps
-    single-group panels."""
+    first panel's groups fit (1500 x 3); while the trailing matrix is taller than 6144 rows they
+    are taken two at a time with one rank-128 trailing update per pair (6400 x 2: two pairs, then
+    single panels); n = 5000, batch 7 has too many groups for that and keeps single-group panels."""
     import scipy.linalg as sl
 
     print(f"[case] lu batched n={n} batch={batch} {prec}", flush=True)

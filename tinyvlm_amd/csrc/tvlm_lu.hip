@@ -738,9 +738,11 @@ __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t ld
 // U12 = L11^{-1} A12 (L11 unit lower).  A work-group of 256 threads takes 64 columns: the swap pairs
 // are gathered for all columns at once, then scattered; U12's 64 x 64 block and L11 sit in LDS and
 // the forward substitution runs one row per step over all 64 columns.
+// mode: 0 = interchanges on both sides and U12 on the right; 1 = interchanges only; 2 = U12 only
+// (the right blocks, no interchanges).
 template <typename T>
 __global__ __launch_bounds__(256) void k_swap_trsm(T* __restrict__ a, int64_t lda, int n, int k0, int kb, int nleft,
-                                                   const int32_t* __restrict__ perm, int64_t sb) {
+                                                   const int32_t* __restrict__ perm, int64_t sb, int mode = 0) {
     a += (int64_t)blockIdx.z * sb;
     perm += (int64_t)blockIdx.z * kPermInts;
     constexpr int kPer = 2 * NB * 64 / 256;  // gathered elements per thread
@@ -751,7 +753,7 @@ __global__ __launch_bounds__(256) void k_swap_trsm(T* __restrict__ a, int64_t ld
     const bool left = (int)blockIdx.x < nleft;
     const int col0 = left ? blockIdx.x * 64 : k0 + kb + (blockIdx.x - nleft) * 64;
     const int colend = left ? k0 : n;
-    const int cnt = perm[0];
+    const int cnt = mode == 2 ? 0 : perm[0];
     for (int e = t; e < cnt; e += 256) {
         s_pd[e] = perm[1 + 2 * e];
         s_ps[e] = perm[2 + 2 * e];
@@ -769,7 +771,7 @@ __global__ __launch_bounds__(256) void k_swap_trsm(T* __restrict__ a, int64_t ld
         const int e = t + 256 * u, pr = e >> 6, col = col0 + (e & 63);
         if (pr < cnt && col < colend) a[s_pd[pr] + (int64_t)col * lda] = v[u];
     }
-    if (left) return;
+    if (left || mode == 1) return;
     __syncthreads();
     for (int e = t; e < NB * 64; e += 256) {
         const int r = e % NB, c = e / NB;
@@ -927,8 +929,9 @@ template <typename T>
 void launch_update(T* C, int64_t ldc, const T* A, int64_t lda, const T* B, int64_t ldb, int m, int n, int kb,
                    int64_t sb, int batch, hipStream_t st) {
     if (m >= kBlasUpdateRows && update_blas<T>(C, ldc, A, lda, B, ldb, m, n, kb, sb, batch, st)) return;
-    hipLaunchKernelGGL(k_update_mfma<T>, dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, st, C, ldc, A, lda,
-                       B, ldb, m, n, kb, sb);
+    for (int k = 0; k < kb; k += NB)  // k_update_mfma takes at most NB columns of A at a time
+        hipLaunchKernelGGL(k_update_mfma<T>, dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, st, C, ldc,
+                           A + (int64_t)k * lda, lda, B + k, ldb, m, n, kb - k < NB ? kb - k : NB, sb);
 }
 
 template <typename T, int NT, int Q, int IB>
@@ -1145,6 +1148,65 @@ int getrf_lookahead(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* ip
     return TVLM_OK;
 }
 
+// Factorisations whose tall panels are cooperative (all panels NB wide), taken two panels at a
+// time so that the trailing matrix is read and written once per pair: panel A at kp, its
+// interchanges and U12 (k_swap_trsm, mode 0); panel B's columns updated by panel A, panel B; panel
+// B's interchanges on both sides (mode 1); the rows of panel B right of it updated by panel A (one
+// NB-row GEMM); panel B's U12 (mode 2); one rank-2NB update of the trailing matrix.  The same
+// products as one panel at a time, summed in a different order.
+constexpr int kPairMinRows = 6144;  // trailing matrices at least this tall are updated once per pair
+
+template <typename T>
+int getrf_coop_pairs(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* ipiv, int32_t* info, int32_t* perm,
+                     hipStream_t st, int32_t* ctr, int panels, T* cval, int32_t* crow, T* xrow) {
+    int ci = 0;  // cooperative panels so far: counter set of the next one
+    auto panel = [&](int kp, int kb) -> int {
+        const int G = coop_groups<T>(n - kp, batch);
+        if (G == 0) return panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb);
+        TVLM_REQUIRE(ci < panels, "cooperative panel %d of at most %d", ci, panels);
+        const int rc = panel_coop<T>(a, lda, n, kp, kb, G, ipiv, info, perm, st, batch, sb,
+                                     ctr + (size_t)ci * batch * NB, cval, crow, xrow);
+        ci++;
+        return rc;
+    };
+    auto swap_trsm = [&](int kp, int kb, int mode) -> int {
+        const int j1 = kp + kb;
+        const int nleft = mode == 2 ? 0 : (kp + 63) / 64, nright = (n - j1 + 63) / 64;
+        if (nleft + nright > 0) {
+            hipLaunchKernelGGL(k_swap_trsm<T>, dim3(nleft + nright, 1, batch), dim3(256), 0, st, a, lda, n, kp, kb,
+                               nleft, perm, sb, mode);
+            TVLM_LAUNCHED();
+        }
+        return TVLM_OK;
+    };
+    auto at = [&](int r, int c) { return a + r + (int64_t)c * lda; };
+    for (int kp = 0; kp < n;) {
+        const int kbA = n - kp < NB ? n - kp : NB, jA = kp + kbA;
+        TVLM_TRY(panel(kp, kbA));
+        TVLM_TRY(swap_trsm(kp, kbA, 0));
+        if (jA >= n) break;
+        if (n - jA < kPairMinRows) {  // the trailing matrix stays in the last-level cache: one panel at a time
+            launch_update<T>(at(jA, jA), lda, at(jA, kp), lda, at(kp, jA), lda, n - jA, n - jA, kbA, sb, batch, st);
+            TVLM_LAUNCHED();
+            kp = jA;
+            continue;
+        }
+        const int kbB = n - jA < NB ? n - jA : NB, jB = jA + kbB;
+        launch_update<T>(at(jA, jA), lda, at(jA, kp), lda, at(kp, jA), lda, n - jA, kbB, kbA, sb, batch, st);
+        TVLM_LAUNCHED();
+        TVLM_TRY(panel(jA, kbB));
+        TVLM_TRY(swap_trsm(jA, kbB, 1));
+        if (jB >= n) break;
+        launch_update<T>(at(jA, jB), lda, at(jA, kp), lda, at(kp, jB), lda, kbB, n - jB, kbA, sb, batch, st);
+        TVLM_LAUNCHED();
+        TVLM_TRY(swap_trsm(jA, kbB, 2));
+        launch_update<T>(at(jB, jB), lda, at(jB, kp), lda, at(kp, jB), lda, n - jB, n - jB, kbA + kbB, sb, batch, st);
+        TVLM_LAUNCHED();
+        kp = jB;
+    }
+    return TVLM_OK;
+}
+
 template <typename T>
 int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     TVLM_REQUIRE(av && av->ptr && ipiv && info, "null argument");
@@ -1187,29 +1249,19 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
         xrow = cval + ncand;
         hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nctr + 255) / 256)), dim3(256), 0, st, ctr, (int64_t)nctr);
         TVLM_LAUNCHED();
+        return getrf_coop_pairs<T>(a, lda, n, batch, sb, ipiv, info, perm, st, ctr, panels, cval, crow, xrow);
     }
-    for (int k0 = 0, ci = 0; k0 < n;) {
+    for (int k0 = 0; k0 < n;) {
         // tall panels of one matrix are narrower: the in-panel left-looking traffic grows with width
         // squared.  A batch is bound by the trailing updates instead, whose traffic (the whole
-        // trailing matrix read and written per panel) halves with the full width.  Cooperative
-        // panels have no in-panel traffic and take the full width.
-        const int G = any_coop ? coop_groups<T>(n - k0, batch) : 0;
-        const int width = (G == 0 && n - k0 > 2048 && batch < kWidePanelBatch) ? NB / 2 : NB;
+        // trailing matrix read and written per panel) halves with the full width.
+        const int width = (n - k0 > 2048 && batch < kWidePanelBatch) ? NB / 2 : NB;
         const int kb = (n - k0 < width) ? n - k0 : width;
         const int j1 = k0 + kb;
         const int kp = k0;  // this panel's first column
         k0 = j1;
-        if (G > 0 || n - kp <= kFastPanelRows) {
-            if (G > 0) {
-                // cooperative panels are NB wide (the last one may be narrower): at most `panels`
-                // of them, whatever narrower panels came first
-                TVLM_REQUIRE(ci < panels, "cooperative panel %d of at most %d", ci, panels);
-                TVLM_TRY(panel_coop<T>(a, lda, n, kp, kb, G, ipiv, info, perm, st, batch, sb,
-                                       ctr + (size_t)ci * batch * NB, cval, crow, xrow));
-                ci++;
-            }
-            else
-                TVLM_TRY(panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb));
+        if (n - kp <= kFastPanelRows) {
+            TVLM_TRY(panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb));
             const int nleft = (kp + 63) / 64, nright = (n - j1 + 63) / 64;
             if (nleft + nright > 0) {
                 hipLaunchKernelGGL(k_swap_trsm<T>, dim3(nleft + nright, 1, batch), dim3(256), 0, st, a, lda, n, kp, kb,
