@@ -739,10 +739,12 @@ __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t ld
 // are gathered for all columns at once, then scattered; U12's 64 x 64 block and L11 sit in LDS and
 // the forward substitution runs one row per step over all 64 columns.
 // mode: 0 = interchanges on both sides and U12 on the right; 1 = interchanges only; 2 = U12 only
-// (the right blocks, no interchanges).
+// (the right blocks, no interchanges).  With gtab, the left blocks before column left0 take the
+// interchanges into their gather lists instead of their columns (deferred interchanges, below).
 template <typename T>
 __global__ __launch_bounds__(256) void k_swap_trsm(T* __restrict__ a, int64_t lda, int n, int k0, int kb, int nleft,
-                                                   const int32_t* __restrict__ perm, int64_t sb, int mode = 0) {
+                                                   const int32_t* __restrict__ perm, int64_t sb, int mode = 0,
+                                                   int left0 = 0, int32_t* __restrict__ gtab = nullptr) {
     a += (int64_t)blockIdx.z * sb;
     perm += (int64_t)blockIdx.z * kPermInts;
     constexpr int kPer = 2 * NB * 64 / 256;  // gathered elements per thread
@@ -754,6 +756,14 @@ __global__ __launch_bounds__(256) void k_swap_trsm(T* __restrict__ a, int64_t ld
     const int col0 = left ? blockIdx.x * 64 : k0 + kb + (blockIdx.x - nleft) * 64;
     const int colend = left ? k0 : n;
     const int cnt = mode == 2 ? 0 : perm[0];
+    if (left && gtab && col0 < left0) {
+        // g'[dst] = g[src] for every pair of the panel's composed interchanges (cnt <= 2 NB)
+        int32_t* g = gtab + ((int64_t)blockIdx.z * ((n + NB - 1) / NB) + col0 / NB) * n;
+        const int v = t < cnt ? g[perm[2 + 2 * t]] : 0;
+        __syncthreads();
+        if (t < cnt) g[perm[1 + 2 * t]] = v;
+        return;
+    }
     for (int e = t; e < cnt; e += 256) {
         s_pd[e] = perm[1 + 2 * e];
         s_ps[e] = perm[2 + 2 * e];
@@ -1148,6 +1158,60 @@ int getrf_lookahead(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* ip
     return TVLM_OK;
 }
 
+// ---- deferred interchanges of the L columns (cooperative factorisations) ------------------------
+//
+// LAPACK's getrf applies every panel's row interchanges to all columns left of the panel.  The
+// cooperative factorisation applies them only to the columns that a later step still reads (the
+// first panel of a pair takes the second's).  Every other column block P (columns [NB P, NB P + NB))
+// keeps a gather list g_P of n rows instead, identity at the start: each later panel's composed
+// interchanges (dst <- src pairs) update the list, g[dst] = g[src] (a few hundred integers, in
+// k_swap_trsm), instead of two scattered rows per pair in each of the block's columns.  At the end
+// k_defer_gather (one work-group per column) stages rows [NB (P + 1), n) of its column in LDS and
+// writes row r back from row g_P[r]: one contiguous pass over L.  The lists only move rows at or
+// below NB (P + 1), so g_P[r] stays in [NB (P + 1), n) there.
+constexpr size_t kDeferLdsBytes = 128 * 1024;  // rows [NB (P + 1), n) of one column
+
+__global__ void k_iota_lists(int32_t* __restrict__ g, int n, int64_t total) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) g[i] = (int32_t)(i % n);
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_defer_gather(T* __restrict__ a, int64_t lda, int n,
+                                                       const int32_t* __restrict__ gtab, int nblocks, int64_t sb) {
+    extern __shared__ unsigned char s_raw[];
+    T* s_col = reinterpret_cast<T*>(s_raw);
+    const int c = blockIdx.x, P = c / NB, t = threadIdx.x;
+    const int from = NB * (P + 1);
+    if (from >= n) return;
+    a += (int64_t)blockIdx.z * sb + (int64_t)c * lda;
+    gtab += ((int64_t)blockIdx.z * nblocks + P) * n;
+    const int m = n - from;
+#pragma unroll 4
+    for (int r = t; r < m; r += 1024) s_col[r] = a[from + r];
+    __syncthreads();
+#pragma unroll 4
+    for (int r = t; r < m; r += 1024) {
+        const int src = gtab[from + r] - from;
+        a[from + r] = s_col[(unsigned)src < (unsigned)m ? src : r];
+    }
+}
+
+template <typename T>
+int defer_gather(T* a, int64_t lda, int n, int batch, int64_t sb, const int32_t* gtab, hipStream_t st) {
+    const int nblocks = (n + NB - 1) / NB;
+    if (nblocks < 2) return TVLM_OK;
+    TVLM_REQUIRE((size_t)n * sizeof(T) <= kDeferLdsBytes, "deferred interchanges: n = %d too large", n);
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_defer_gather<T>),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)kDeferLdsBytes);
+    TVLM_HIP(attr);
+    hipLaunchKernelGGL(k_defer_gather<T>, dim3(n, 1, batch), dim3(1024), sizeof(T) * (size_t)n, st, a, lda, n, gtab,
+                       nblocks, sb);
+    TVLM_LAUNCHED();
+    return TVLM_OK;
+}
+
 // Factorisations whose tall panels are cooperative (all panels NB wide), taken two panels at a
 // time so that the trailing matrix is read and written once per pair: panel A at kp, its
 // interchanges and U12 (k_swap_trsm, mode 0); panel B's columns updated by panel A, panel B; panel
@@ -1158,8 +1222,13 @@ constexpr int kPairMinRows = 6144;  // trailing matrices at least this tall are 
 
 template <typename T>
 int getrf_coop_pairs(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* ipiv, int32_t* info, int32_t* perm,
-                     hipStream_t st, int32_t* ctr, int panels, T* cval, int32_t* crow, T* xrow) {
+                     hipStream_t st, int32_t* ctr, int panels, T* cval, int32_t* crow, T* xrow, int32_t* gtab) {
     int ci = 0;  // cooperative panels so far: counter set of the next one
+    if (n > NB) {
+        const int64_t total = (int64_t)batch * ((n + NB - 1) / NB) * n;
+        hipLaunchKernelGGL(k_iota_lists, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, gtab, n, total);
+        TVLM_LAUNCHED();
+    }
     auto panel = [&](int kp, int kb) -> int {
         const int G = coop_groups<T>(n - kp, batch);
         if (G == 0) return panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb);
@@ -1169,12 +1238,13 @@ int getrf_coop_pairs(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* i
         ci++;
         return rc;
     };
-    auto swap_trsm = [&](int kp, int kb, int mode) -> int {
+    // the left columns before left0 take the interchanges into their gather lists
+    auto swap_trsm = [&](int kp, int kb, int mode, int left0) -> int {
         const int j1 = kp + kb;
         const int nleft = mode == 2 ? 0 : (kp + 63) / 64, nright = (n - j1 + 63) / 64;
         if (nleft + nright > 0) {
             hipLaunchKernelGGL(k_swap_trsm<T>, dim3(nleft + nright, 1, batch), dim3(256), 0, st, a, lda, n, kp, kb,
-                               nleft, perm, sb, mode);
+                               nleft, perm, sb, mode, left0, gtab);
             TVLM_LAUNCHED();
         }
         return TVLM_OK;
@@ -1183,7 +1253,7 @@ int getrf_coop_pairs(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* i
     for (int kp = 0; kp < n;) {
         const int kbA = n - kp < NB ? n - kp : NB, jA = kp + kbA;
         TVLM_TRY(panel(kp, kbA));
-        TVLM_TRY(swap_trsm(kp, kbA, 0));
+        TVLM_TRY(swap_trsm(kp, kbA, 0, kp));
         if (jA >= n) break;
         if (n - jA < kPairMinRows) {  // the trailing matrix stays in the last-level cache: one panel at a time
             launch_update<T>(at(jA, jA), lda, at(jA, kp), lda, at(kp, jA), lda, n - jA, n - jA, kbA, sb, batch, st);
@@ -1195,16 +1265,16 @@ int getrf_coop_pairs(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* i
         launch_update<T>(at(jA, jA), lda, at(jA, kp), lda, at(kp, jA), lda, n - jA, kbB, kbA, sb, batch, st);
         TVLM_LAUNCHED();
         TVLM_TRY(panel(jA, kbB));
-        TVLM_TRY(swap_trsm(jA, kbB, 1));
+        TVLM_TRY(swap_trsm(jA, kbB, 1, kp));
         if (jB >= n) break;
         launch_update<T>(at(jA, jB), lda, at(jA, kp), lda, at(kp, jB), lda, kbB, n - jB, kbA, sb, batch, st);
         TVLM_LAUNCHED();
-        TVLM_TRY(swap_trsm(jA, kbB, 2));
+        TVLM_TRY(swap_trsm(jA, kbB, 2, jA));
         launch_update<T>(at(jB, jB), lda, at(jB, kp), lda, at(kp, jB), lda, n - jB, n - jB, kbA + kbB, sb, batch, st);
         TVLM_LAUNCHED();
         kp = jB;
     }
-    return TVLM_OK;
+    return defer_gather<T>(a, lda, n, batch, sb, gtab, st);
 }
 
 template <typename T>
@@ -1240,16 +1310,18 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     if (any_coop) {
         const size_t nctr = (size_t)panels * batch * NB, ncand = (size_t)batch * NB * kCoopMaxGroups;
         const size_t nrow = (size_t)batch * NB * (kCoopMaxGroups + 1) * NB;
+        const size_t ngat = (size_t)batch * panels * n;  // gather lists of the deferred interchanges
         void* cws = nullptr;
-        TVLM_TRY(device_workspace(sizeof(int32_t) * (nctr + ncand) + sizeof(T) * (ncand + nrow) + 256, &cws, kWsLuCoop,
-                                  st));
+        TVLM_TRY(device_workspace(sizeof(int32_t) * (nctr + ncand + ngat) + sizeof(T) * (ncand + nrow) + 512, &cws,
+                                  kWsLuCoop, st));
         ctr = static_cast<int32_t*>(cws);
         crow = ctr + nctr;
         cval = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(crow + ncand) + 255) & ~(uintptr_t)255);
         xrow = cval + ncand;
+        int32_t* gtab = reinterpret_cast<int32_t*>((reinterpret_cast<uintptr_t>(xrow + nrow) + 255) & ~(uintptr_t)255);
         hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nctr + 255) / 256)), dim3(256), 0, st, ctr, (int64_t)nctr);
         TVLM_LAUNCHED();
-        return getrf_coop_pairs<T>(a, lda, n, batch, sb, ipiv, info, perm, st, ctr, panels, cval, crow, xrow);
+        return getrf_coop_pairs<T>(a, lda, n, batch, sb, ipiv, info, perm, st, ctr, panels, cval, crow, xrow, gtab);
     }
     for (int k0 = 0; k0 < n;) {
         // tall panels of one matrix are narrower: the in-panel left-looking traffic grows with width
