@@ -480,3 +480,54 @@ def test_lu_batched(H, torch, prec, n, batch):
             assert res < 4 * n * np.finfo(np.float64).eps
         else:
             assert res < 4 * n * np.finfo(np.float32).eps
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("n,batch", [(2200, 1), (3000, 2), (6400, 2)])
+def test_lu_coop_single_exchange_attempt(H, torch, prec, n, batch):
+    """Cooperative panels first try one exchange per panel (the first work-group pivots among its
+    own rows, the others eliminate against its U11 and check that no row of theirs would have been
+    the pivot) and fall back to the column-by-column exchange when the check fails.  Matrix 0 is
+    column diagonally dominant (LAPACK never interchanges: every attempt stands); matrix 1 has rows
+    interchanged so that LAPACK pivots once inside the first group's rows (300 <-> 400: the attempt
+    stands with a row interchange) and twice far below them in consecutive panels (1000 <-> 1900,
+    1070 <-> 1870: two failed attempts in a row, after which the rest of that matrix takes the
+    exchange loop).  Pivots equal LAPACK's, factors within 1e-11 (FP64), solutions to the
+    single-matrix bar."""
+    import scipy.linalg as sl
+
+    print(f"[case] lu coop attempt n={n} batch={batch} {prec}", flush=True)
+    dt = DT(torch, prec)
+    rng = np.random.default_rng(77 + n)
+    As = []
+    for b in range(batch):
+        A = rng.standard_normal((n, n)) + n * np.eye(n)
+        if b == 1:
+            for r, s in ((300, 400), (1000, 1900), (1070, 1870)):
+                A[[r, s]] = A[[s, r]]
+        As.append(A)
+    Bs = [rng.standard_normal((n, 1)) for _ in range(batch)]
+    a = H.colmajor((n, n, batch), dt)
+    x = H.colmajor((n, 1, batch), dt)
+    for b in range(batch):
+        a[:, :, b] = H.to_device(As[b], dt)
+        x[:, :, b] = H.to_device(Bs[b], dt)
+    ipiv = torch.zeros(n * batch, dtype=torch.int32, device="cuda")
+    info = torch.full((batch,), -1, dtype=torch.int32, device="cuda")
+    H.getrf(a, ipiv, info)
+    H.getrs(a, ipiv, x)
+    assert np.all(info.cpu().numpy() == 0)
+    piv = ipiv.cpu().numpy().reshape(batch, n)
+    for b in range(batch):
+        lu_ref, piv_ref = sl.lu_factor(As[b])
+        assert np.array_equal(piv[b] - 1, piv_ref)
+        if b == 1:
+            assert piv_ref[300] == 400 and piv_ref[1000] == 1900 and piv_ref[1070] == 1870
+        X = H.to_host(x[:, :, b])
+        res = np.linalg.norm(As[b] @ X - Bs[b]) / (np.linalg.norm(As[b]) * np.linalg.norm(X))
+        if prec == "f64":
+            assert rel(H.to_host(a[:, :, b]), lu_ref) < 1e-11
+            assert res < 4 * n * np.finfo(np.float64).eps
+        else:
+            assert rel(H.to_host(a[:, :, b]), lu_ref) < 1e-4
+            assert res < 4 * n * np.finfo(np.float32).eps

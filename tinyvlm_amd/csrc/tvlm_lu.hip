@@ -23,6 +23,7 @@
 // (k_solve_perm) and each 64-row block step runs on all trailing row blocks at once
 // (k_trsv_step); larger n uses block-row kernels (k_diag_solve / k_off_update).
 #include <map>
+#include <utility>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -573,10 +574,25 @@ __global__ __launch_bounds__(NT) void k_panel_reg(T* __restrict__ a, int64_t lda
 // out (the groups were not all resident), or a pivot row outside [c, n) read back from the exchange,
 // leaves info = kCoopStalled instead of waiting for ever or indexing out of bounds.
 // Group 0 writes ipiv and the panel's composed interchanges for k_swap_trsm, as k_panel_reg does.
+//
+// Before the column-by-column exchange, an attempt with two exchanges in all: group 0 factors its
+// own NT rows with partial pivoting among them and publishes U11 (its first kb rows); every other
+// group eliminates its rows against U11 (the same operations, in the same order, as the exchange
+// loop with those pivots) and checks |a(r, c)| <= |U(c, c)| for every row and column before
+// scaling.  When that holds everywhere, partial pivoting over all rows picks exactly group 0's
+// pivots (on a tie the first row, which is group 0's), so the result is the exchange loop's; the
+// groups vote, and on any failure every group reloads its rows and runs the exchange loop.
+// Influence matrices of rectangular and swept planforms never pivot outside the diagonal block and
+// pass throughout; after kCoopOptFails failed attempts in a row the matrix (ostate, per matrix,
+// zeroed with the counters) skips them.  Counters per panel and matrix: ctr[0, NB) the columns,
+// ctr[NB] U11 published, ctr[NB + 1] the votes, ctr[NB + 2] a failed vote.
 constexpr int kCoopMaxGroups = 64;   // work-groups of one launch; all of them must be resident at once
 constexpr int kCoopMinRows = 1024;   // shorter panels stay in one work-group (k_panel_reg)
 constexpr unsigned kCoopSpinLimit = 1u << 22;
 constexpr int kCoopStalled = -999;
+constexpr int kCoopCtr = NB + 4;                                                      // counters per panel and matrix
+constexpr int64_t kCoopXrow = (int64_t)NB * (kCoopMaxGroups + 1) * NB + NB * NB;  // xrow per matrix (+ U11)
+constexpr int kCoopOptFails = 2;
 
 template <typename T>
 __device__ __forceinline__ void st_agent(T* p, T v) {
@@ -593,21 +609,96 @@ constexpr int coop_threads() {
     return sizeof(T) == 4 ? 512 : 256;
 }
 
+// Column J of the first group's local factorisation (k_panel_coop's single-exchange attempt); cols()
+// runs all NB of them in order, one instantiation per column, so the row stays in registers.
+template <typename T, int NT>
+struct CoopLocal {
+    T* r;
+    int k0, n, i;
+    bool mine;
+    int t, lane, w, wu;
+    T* s_val;
+    int *s_idx;
+    T *s_prow, *s_crow;
+    int *s_arg, *s_piv, *s_zero;
+
+    template <int J>
+    __device__ __forceinline__ void col() {
+        constexpr int NW = NT / 64;
+        const int c = k0 + J;
+        T best = T(-1);
+        int bi = n;
+        if (mine && i >= J) {
+            best = (T)fabs(r[J]);
+            bi = k0 + i;
+        }
+        wave_argmax(best, bi);
+        if (lane == 0) {
+            s_val[w] = best;
+            s_idx[w] = bi;
+        }
+        __syncthreads();
+        best = s_val[0];
+        bi = s_idx[0];
+#pragma unroll
+        for (int ww = 1; ww < NW; ww++) argmax_merge(best, bi, s_val[ww], s_idx[ww]);
+        const int pl = (bi >= k0 + J && bi < k0 + NT && bi < n) ? bi - k0 : J;
+        if (wu == (pl >> 6) && lane == (pl & 63)) {
+#pragma unroll
+            for (int jj = 0; jj < NB; jj++) s_prow[jj] = r[jj];
+        }
+        if (pl != J && wu == 0 && lane == J) {
+#pragma unroll
+            for (int jj = 0; jj < NB; jj++) s_crow[jj] = r[jj];
+        }
+        __syncthreads();
+        const T piv = s_prow[J];
+        const bool nz = piv != T(0);
+        if (t == 0) {
+            s_arg[J] = k0 + pl;
+            s_piv[J] = nz ? k0 + pl : c;
+            if (!nz && *s_zero == 0) *s_zero = c + 1;
+        }
+        if (nz && pl != J) {
+            if (wu == 0 && lane == J) {
+#pragma unroll
+                for (int jj = 0; jj < NB; jj++) r[jj] = s_prow[jj];
+            }
+            if (wu == (pl >> 6) && lane == (pl & 63)) {
+#pragma unroll
+                for (int jj = 0; jj < NB; jj++) r[jj] = s_crow[jj];
+            }
+        }
+        if (nz && mine && i > J) {
+            const T l = r[J] * (T(1) / piv);
+            r[J] = l;
+#pragma unroll
+            for (int jj = J + 1; jj < NB; jj++) r[jj] -= l * s_prow[jj];
+        }
+        __syncthreads();  // s_val, s_prow and s_crow are rewritten for the next column
+    }
+
+    template <int... J>
+    __device__ __forceinline__ void cols(std::integer_sequence<int, J...>) {
+        (col<J>(), ...);
+    }
+};
+
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t lda, int n, int k0, int kb, int G,
                                                    int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
                                                    int32_t* __restrict__ perm, int64_t sb, int32_t* ctr, T* cval,
-                                                   int32_t* crow, T* xrow) {
+                                                   int32_t* crow, T* xrow, int32_t* ostate) {
     constexpr int NW = NT / 64, MG = kCoopMaxGroups;
     const int z = blockIdx.z, g = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
     a += (int64_t)z * sb;
     ipiv += (int64_t)z * n;
     info += z;
     perm += (int64_t)z * kPermInts;
-    ctr += (int64_t)z * NB;
+    ctr += (int64_t)z * kCoopCtr;
     cval += (int64_t)z * NB * MG;
     crow += (int64_t)z * NB * MG;
-    xrow += (int64_t)z * NB * (MG + 1) * NB;
+    xrow += (int64_t)z * kCoopXrow;
     const int m = n - k0, i = g * NT + t;  // this thread's row: k0 + i
     const bool mine = i < m;
     extern __shared__ unsigned char coop_lds[];
@@ -616,14 +707,105 @@ __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t ld
     __shared__ int s_idx[NW];
     __shared__ T s_prow[NB], s_crow[NB];
     __shared__ int s_p, s_fail, s_zero, s_arg[NB], s_piv[NB];
-    __shared__ int s_pd[2 * NB], s_ps[2 * NB], s_pcnt;
+    __shared__ int s_pd[2 * NB], s_ps[2 * NB], s_pcnt, s_opt;
+    __shared__ T s_u[NB * (NB + 1) / 2];  // U11 of the attempt, row r from column r on, rows packed
     if (t == 0) {
         s_fail = 0;
         s_zero = 0;
     }
     for (int jj = 0; jj < kb; jj++) xs[jj * NT + t] = mine ? a[k0 + i + (int64_t)(k0 + jj) * lda] : T(0);
     __syncthreads();
-    for (int j = 0; j < kb; j++) {
+    bool settled = false;
+    const int fails = ld_agent(&ostate[z]);
+    if (kb == NB && fails < kCoopOptFails) {
+        int32_t* octr = ctr + NB;
+        T* xu = xrow + (int64_t)NB * (MG + 1) * NB;
+        bool ok = true;
+        if (g == 0) {
+            // partial pivoting among this group's rows (all NT of them hold rows: m > kCoopMinRows),
+            // each row in its thread's registers (one instantiation per column: every index is a
+            // constant); the pivot row is broadcast through s_prow, the row it displaces through s_crow
+            T r[NB];
+#pragma unroll
+            for (int jj = 0; jj < NB; jj++) r[jj] = xs[jj * NT + t];
+            CoopLocal<T, NT> L{r, k0, n, i, mine, t, lane, w, __builtin_amdgcn_readfirstlane(w), s_val, s_idx, s_prow,
+                               s_crow, s_arg, s_piv, &s_zero};
+            L.cols(std::make_integer_sequence<int, NB>{});
+#pragma unroll
+            for (int jj = 0; jj < NB; jj++) xs[jj * NT + t] = r[jj];
+            __syncthreads();
+            for (int e = t; e < kb * kb; e += NT) {
+                const int r = e / kb, cc = e % kb;
+                if (cc >= r) st_agent(&xu[r * NB + cc], xs[cc * NT + r]);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            if (t == 0) __hip_atomic_fetch_add(&octr[0], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (t == 0) {
+                unsigned spins = 0;
+                while (__hip_atomic_load(&octr[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < 1) {
+                    if (++spins > kCoopSpinLimit) {
+                        s_fail = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __syncthreads();
+            for (int e = t; e < kb * kb; e += NT) {
+                const int r = e / kb, cc = e % kb;
+                if (cc >= r) s_u[r * NB - r * (r - 1) / 2 + (cc - r)] = ld_agent(&xu[r * NB + cc]);
+            }
+            __syncthreads();
+            bool okr = s_fail == 0;
+            if (mine) {  // the row in registers (kb == NB here: every index below is a constant)
+                T r[NB];
+#pragma unroll
+                for (int jj = 0; jj < NB; jj++) r[jj] = xs[jj * NT + t];
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const int b = j * NB - j * (j - 1) / 2;
+                    const T piv = s_u[b];
+                    okr = okr && ((T)fabs(r[j]) <= (T)fabs(piv));
+                    if (piv != T(0)) {
+                        const T l = r[j] * (T(1) / piv);
+                        r[j] = l;
+#pragma unroll
+                        for (int jj = j + 1; jj < NB; jj++) r[jj] -= l * s_u[b + (jj - j)];
+                    }
+                }
+#pragma unroll
+                for (int jj = 0; jj < NB; jj++) xs[jj * NT + t] = r[jj];
+            }
+            ok = __syncthreads_and(okr) != 0;
+        }
+        // the vote: every group counts in, a failing one first raises the flag
+        if (t == 0) {
+            if (!ok) st_agent(&octr[2], 1);
+            __hip_atomic_fetch_add(&octr[1], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned spins = 0;
+            bool stalled = false;
+            while (__hip_atomic_load(&octr[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < G) {
+                if (++spins > kCoopSpinLimit) {
+                    stalled = true;
+                    s_fail = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            s_opt = (!stalled && ld_agent(&octr[2]) == 0) ? 1 : 0;
+        }
+        __syncthreads();
+        settled = s_opt != 0;
+        if (g == 0 && t == 0) st_agent(&ostate[z], settled ? 0 : fails + 1);
+        if (!settled) {  // back to the rows as loaded
+            if (t == 0) s_zero = 0;
+            for (int jj = 0; jj < kb; jj++) xs[jj * NT + t] = mine ? a[k0 + i + (int64_t)(k0 + jj) * lda] : T(0);
+            __syncthreads();
+        }
+    }
+    for (int j = 0; j < kb && !settled; j++) {
         const int c = k0 + j;
         // this group's candidate: largest |a(r, c)| over its rows r >= c, first row on ties
         T best = T(-1);
@@ -706,6 +888,7 @@ __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t ld
             if (mine && i > j) {
                 const T l = xs[j * NT + t] * (T(1) / piv);
                 xs[j * NT + t] = l;
+#pragma unroll 8
                 for (int jj = j + 1; jj < kb; jj++) xs[jj * NT + t] -= l * s_prow[jj];
             }
         }
@@ -1062,7 +1245,7 @@ constexpr size_t kCoopLdsBytes = 128 * 1024;  // NT rows x NB columns (coop_thre
 
 template <typename T>
 int panel_coop(T* a, int64_t lda, int n, int k0, int kb, int G, int32_t* ipiv, int32_t* info, int32_t* perm,
-               hipStream_t st, int batch, int64_t sb, int32_t* ctr, T* cval, int32_t* crow, T* xrow) {
+               hipStream_t st, int batch, int64_t sb, int32_t* ctr, T* cval, int32_t* crow, T* xrow, int32_t* ostate) {
     constexpr int NT = coop_threads<T>();
     static const hipError_t lds_attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_panel_coop<T, NT>),
                                                            hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1070,9 +1253,9 @@ int panel_coop(T* a, int64_t lda, int n, int k0, int kb, int G, int32_t* ipiv, i
     TVLM_HIP(lds_attr);
     TVLM_REQUIRE(G >= 2 && G * batch <= kCoopMaxGroups && kb <= NB && (int64_t)G * NT >= (int64_t)(n - k0),
                  "cooperative panel: %d groups for %d rows", G, n - k0);
-    TVLM_REQUIRE(ctr && cval && crow && xrow, "cooperative panel without exchange buffers");
+    TVLM_REQUIRE(ctr && cval && crow && xrow && ostate, "cooperative panel without exchange buffers");
     hipLaunchKernelGGL((k_panel_coop<T, NT>), dim3(G, 1, batch), dim3(NT), kCoopLdsBytes, st, a, lda, n, k0, kb, G,
-                       ipiv, info, perm, sb, ctr, cval, crow, xrow);
+                       ipiv, info, perm, sb, ctr, cval, crow, xrow, ostate);
     TVLM_LAUNCHED();
     return TVLM_OK;
 }
@@ -1222,7 +1405,8 @@ constexpr int kPairMinRows = 6144;  // trailing matrices at least this tall are 
 
 template <typename T>
 int getrf_coop_pairs(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* ipiv, int32_t* info, int32_t* perm,
-                     hipStream_t st, int32_t* ctr, int panels, T* cval, int32_t* crow, T* xrow, int32_t* gtab) {
+                     hipStream_t st, int32_t* ctr, int panels, T* cval, int32_t* crow, T* xrow, int32_t* gtab,
+                     int32_t* ostate) {
     int ci = 0;  // cooperative panels so far: counter set of the next one
     if (n > NB) {
         const int64_t total = (int64_t)batch * ((n + NB - 1) / NB) * n;
@@ -1234,7 +1418,7 @@ int getrf_coop_pairs(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* i
         if (G == 0) return panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb);
         TVLM_REQUIRE(ci < panels, "cooperative panel %d of at most %d", ci, panels);
         const int rc = panel_coop<T>(a, lda, n, kp, kb, G, ipiv, info, perm, st, batch, sb,
-                                     ctr + (size_t)ci * batch * NB, cval, crow, xrow);
+                                     ctr + (size_t)ci * batch * kCoopCtr, cval, crow, xrow, ostate);
         ci++;
         return rc;
     };
@@ -1308,8 +1492,8 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     T *cval = nullptr, *xrow = nullptr;
     int32_t* crow = nullptr;
     if (any_coop) {
-        const size_t nctr = (size_t)panels * batch * NB, ncand = (size_t)batch * NB * kCoopMaxGroups;
-        const size_t nrow = (size_t)batch * NB * (kCoopMaxGroups + 1) * NB;
+        const size_t nctr = (size_t)panels * batch * kCoopCtr + batch, ncand = (size_t)batch * NB * kCoopMaxGroups;
+        const size_t nrow = (size_t)batch * kCoopXrow;
         const size_t ngat = (size_t)batch * panels * n;  // gather lists of the deferred interchanges
         void* cws = nullptr;
         TVLM_TRY(device_workspace(sizeof(int32_t) * (nctr + ncand + ngat) + sizeof(T) * (ncand + nrow) + 512, &cws,
@@ -1321,7 +1505,9 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
         int32_t* gtab = reinterpret_cast<int32_t*>((reinterpret_cast<uintptr_t>(xrow + nrow) + 255) & ~(uintptr_t)255);
         hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nctr + 255) / 256)), dim3(256), 0, st, ctr, (int64_t)nctr);
         TVLM_LAUNCHED();
-        return getrf_coop_pairs<T>(a, lda, n, batch, sb, ipiv, info, perm, st, ctr, panels, cval, crow, xrow, gtab);
+        int32_t* ostate = ctr + (size_t)panels * batch * kCoopCtr;  // per matrix: failed attempts in a row
+        return getrf_coop_pairs<T>(a, lda, n, batch, sb, ipiv, info, perm, st, ctr, panels, cval, crow, xrow, gtab,
+                                   ostate);
     }
     for (int k0 = 0; k0 < n;) {
         // tall panels of one matrix are narrower: the in-panel left-looking traffic grows with width
