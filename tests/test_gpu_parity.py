@@ -435,7 +435,7 @@ def test_error_status_no_launch(H, torch):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("n,batch", [(130, 5), (1500, 3), (200, 9), (1100, 8), (2200, 8), (6400, 2),
+@pytest.mark.parametrize("n,batch", [(130, 5), (1500, 3), (200, 9), (1100, 8), (2200, 8), (3000, 8), (6400, 2),
                                      pytest.param(5000, 7, marks=pytest.mark.skip(
                                          reason="faulted the GPU through a host bug (cooperative panels started part-way "
                                                 "with no exchange buffers); that sequence is switched off and this case "
@@ -443,7 +443,9 @@ def test_error_status_no_launch(H, torch):
 def test_lu_batched(H, torch, prec, n, batch):
     """A batch of independent matrices in one getrf/getrs pass equals factoring each alone
     (LAPACK pivots for FP64, per-matrix info, solutions to the single-matrix bar).  Batches of 8 or
-    more take the look-ahead factorisation (64-column panels, split trailing updates).  Smaller
+    more take the look-ahead factorisation (64-column panels, split trailing updates; while the
+    batch's trailing matrices exceed 256 MB, two panels per rank-128 update: 2200 x 8 FP64 and
+    3000 x 8 start that way).  Smaller
     batches of tall matrices take cooperative panels (several work-groups per matrix) when the
     first panel's groups fit (1500 x 3); while the trailing matrix is taller than 6144 rows they
     are taken two at a time with one rank-128 trailing update per pair (6400 x 2: two pairs, then

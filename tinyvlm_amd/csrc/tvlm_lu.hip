@@ -923,11 +923,13 @@ __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t ld
 // the forward substitution runs one row per step over all 64 columns.
 // mode: 0 = interchanges on both sides and U12 on the right; 1 = interchanges only; 2 = U12 only
 // (the right blocks, no interchanges).  With gtab, the left blocks before column left0 take the
-// interchanges into their gather lists instead of their columns (deferred interchanges, below).
+// interchanges into their gather lists instead of their columns (deferred interchanges, below).  The
+// left blocks start at column lbase, the right ones at rbase (default: right after the panel).
 template <typename T>
 __global__ __launch_bounds__(256) void k_swap_trsm(T* __restrict__ a, int64_t lda, int n, int k0, int kb, int nleft,
                                                    const int32_t* __restrict__ perm, int64_t sb, int mode = 0,
-                                                   int left0 = 0, int32_t* __restrict__ gtab = nullptr) {
+                                                   int left0 = 0, int32_t* __restrict__ gtab = nullptr, int lbase = 0,
+                                                   int rbase = -1) {
     a += (int64_t)blockIdx.z * sb;
     perm += (int64_t)blockIdx.z * kPermInts;
     constexpr int kPer = 2 * NB * 64 / 256;  // gathered elements per thread
@@ -936,7 +938,7 @@ __global__ __launch_bounds__(256) void k_swap_trsm(T* __restrict__ a, int64_t ld
     __shared__ T s_l[NB][NB + 1];
     const int t = threadIdx.x;
     const bool left = (int)blockIdx.x < nleft;
-    const int col0 = left ? blockIdx.x * 64 : k0 + kb + (blockIdx.x - nleft) * 64;
+    const int col0 = left ? lbase + blockIdx.x * 64 : (rbase < 0 ? k0 + kb : rbase) + (blockIdx.x - nleft) * 64;
     const int colend = left ? k0 : n;
     const int cnt = mode == 2 ? 0 : perm[0];
     if (left && gtab && col0 < left0) {
@@ -1296,47 +1298,81 @@ int lookahead_streams(hipStream_t caller, LookaheadStreams** out) {
 // panel k+1 at once, while the rest of the trailing update runs on a side stream; the next
 // k_swap_trsm waits for both.  The panels (one work-group per matrix) thus run beside the
 // bandwidth-bound updates instead of between them.  Same arithmetic as the plain loop.
+// Batched look-ahead factorisation.  Each step starts from a factored block [k0, j1) (one or two
+// panels) whose U12 is known on every column right of it, and the trailing matrix not yet updated by
+// that block.  The caller's stream updates the step's own columns first and factors them at once,
+// while the rest of the trailing update runs on a side stream; what touches the rest (interchanges,
+// U12) waits for it.  While the batch's trailing matrices exceed the last-level cache
+// (kLookaheadPairBytes) a step takes two panels A and B: A, A's interchanges and U12 on B's columns
+// only, B's columns updated by A, B, B's interchanges on A's columns -- all inside the step's 128
+// columns, beside the side stream -- then, after it, A's and B's interchanges on the columns left
+// and right of the step (in that order), A's U12 on the right, B's rows right of the step updated by
+// A, B's U12.  The next trailing update is then one of rank 2 NB: the trailing matrices are read and
+// written once per two panels.  perm holds two panels' composed interchanges (A's, then B's).
+constexpr double kLookaheadPairBytes = 256.0 * 1024 * 1024;
+
 template <typename T>
 int getrf_lookahead(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* ipiv, int32_t* info, int32_t* perm,
                     hipStream_t st) {
     LookaheadStreams* L = nullptr;
     TVLM_TRY(lookahead_streams(st, &L));
-    auto swap_trsm = [&](int kp, int kb) -> int {
-        const int j1 = kp + kb;
-        const int nleft = (kp + 63) / 64, nright = (n - j1 + 63) / 64;
+    int32_t* perm2 = perm + (size_t)batch * kPermInts;
+    // interchanges (mode 0/1) and U12 (mode 0/2) of the panel at kp on nleft blocks of 64 columns
+    // from lbase and nright blocks from rbase
+    auto swaps = [&](int kp, int kb, const int32_t* pm, int mode, int lbase, int nleft, int rbase, int nright) -> int {
         if (nleft + nright > 0) {
             hipLaunchKernelGGL(k_swap_trsm<T>, dim3(nleft + nright, 1, batch), dim3(256), 0, st, a, lda, n, kp, kb,
-                               nleft, perm, sb);
+                               nleft, pm, sb, mode, 0, nullptr, lbase, rbase);
             TVLM_LAUNCHED();
         }
         return TVLM_OK;
     };
-    int kp = 0, kb = n < NB ? n : NB;
-    TVLM_TRY(panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb));
-    TVLM_TRY(swap_trsm(kp, kb));
-    while (kp + kb < n) {
-        const int j1 = kp + kb, m = n - j1;
-        const int nb = m < NB ? m : NB;
-        const T* L21 = a + j1 + (int64_t)kp * lda;
-        const T* U12 = a + kp + (int64_t)j1 * lda;
-        launch_update<T>(a + j1 + (int64_t)j1 * lda, lda, L21, lda, U12, lda, m, nb, kb, sb, batch, st);
+    auto blocks = [](int cols) { return cols > 0 ? (cols + 63) / 64 : 0; };
+    auto at = [&](int r, int c) { return a + r + (int64_t)c * lda; };
+    int k0 = 0, j1 = n < NB ? n : NB;
+    TVLM_TRY(panel_fast<T>(a, lda, n, k0, j1, ipiv, info, perm, st, batch, sb));
+    TVLM_TRY(swaps(k0, j1, perm, 0, 0, 0, j1, blocks(n - j1)));
+    while (j1 < n) {
+        const int kr = j1 - k0, m = n - j1;  // rank of the pending update; trailing size
+        const double tail = (double)batch * (double)m * (double)m * sizeof(T);
+        const bool pair = m > 2 * NB && tail > kLookaheadPairBytes;
+        const int w = pair ? 2 * NB : (m < NB ? m : NB), jw = j1 + w;  // the step's columns [j1, jw)
+        launch_update<T>(at(j1, j1), lda, at(j1, k0), lda, at(k0, j1), lda, m, w, kr, sb, batch, st);
         TVLM_LAUNCHED();
-        const bool rest = m > nb;
+        const bool rest = m > w;
         if (rest) {
             TVLM_HIP(hipEventRecord(L->narrow_done, st));
             TVLM_HIP(hipStreamWaitEvent(L->side, L->narrow_done, 0));
-        }
-        TVLM_TRY(panel_fast<T>(a, lda, n, j1, nb, ipiv, info, perm, st, batch, sb));
-        if (rest) {
-            launch_update<T>(a + j1 + (int64_t)(j1 + nb) * lda, lda, L21, lda, U12 + (int64_t)nb * lda, lda, m, m - nb,
-                             kb, sb, batch, L->side);
+            launch_update<T>(at(j1, jw), lda, at(j1, k0), lda, at(k0, jw), lda, m, m - w, kr, sb, batch, L->side);
             TVLM_LAUNCHED();
             TVLM_HIP(hipEventRecord(L->rest_done, L->side));
-            TVLM_HIP(hipStreamWaitEvent(st, L->rest_done, 0));
         }
-        TVLM_TRY(swap_trsm(j1, nb));
-        kp = j1;
-        kb = nb;
+        if (!pair) {
+            TVLM_TRY(panel_fast<T>(a, lda, n, j1, w, ipiv, info, perm, st, batch, sb));
+            if (rest) TVLM_HIP(hipStreamWaitEvent(st, L->rest_done, 0));
+            TVLM_TRY(swaps(j1, w, perm, 0, 0, blocks(j1), jw, blocks(n - jw)));
+            k0 = j1;
+            j1 = jw;
+            continue;
+        }
+        const int jA = j1 + NB;  // panel A = [j1, jA), panel B = [jA, jw)
+        TVLM_TRY(panel_fast<T>(a, lda, n, j1, NB, ipiv, info, perm, st, batch, sb));
+        TVLM_TRY(swaps(j1, NB, perm, 0, 0, 0, jA, 1));
+        launch_update<T>(at(jA, jA), lda, at(jA, j1), lda, at(j1, jA), lda, n - jA, NB, NB, sb, batch, st);
+        TVLM_LAUNCHED();
+        TVLM_TRY(panel_fast<T>(a, lda, n, jA, NB, ipiv, info, perm2, st, batch, sb));
+        TVLM_TRY(swaps(jA, NB, perm2, 1, j1, 1, 0, 0));
+        if (rest) TVLM_HIP(hipStreamWaitEvent(st, L->rest_done, 0));
+        TVLM_TRY(swaps(j1, NB, perm, 1, 0, blocks(j1), jw, blocks(n - jw)));
+        TVLM_TRY(swaps(jA, NB, perm2, 1, 0, blocks(j1), jw, blocks(n - jw)));
+        if (jw < n) {
+            TVLM_TRY(swaps(j1, NB, perm, 2, 0, 0, jw, blocks(n - jw)));
+            launch_update<T>(at(jA, jw), lda, at(jA, j1), lda, at(j1, jw), lda, NB, n - jw, NB, sb, batch, st);
+            TVLM_LAUNCHED();
+            TVLM_TRY(swaps(jA, NB, perm2, 2, 0, 0, jw, blocks(n - jw)));
+        }
+        k0 = j1;
+        j1 = jw;
     }
     return TVLM_OK;
 }
@@ -1478,7 +1514,7 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     hipLaunchKernelGGL(k_zero_info_batch<T>, dim3((batch + 255) / 256), dim3(256), 0, st, info, batch);
     TVLM_LAUNCHED();
     void* ws = nullptr;
-    TVLM_TRY(device_workspace(sizeof(int32_t) * kPermInts * (size_t)batch, &ws, kWsLuPerm, st));
+    TVLM_TRY(device_workspace(sizeof(int32_t) * kPermInts * 2 * (size_t)batch, &ws, kWsLuPerm, st));
     int32_t* perm = static_cast<int32_t*>(ws);
     if (batch >= kWidePanelBatch && n <= kFastPanelRows) return getrf_lookahead<T>(a, lda, n, batch, sb, ipiv, info, perm, st);
     // Cooperative tall panels only when the first panel already is one; then every panel of more
