@@ -435,11 +435,7 @@ def test_error_status_no_launch(H, torch):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("n,batch", [(130, 5), (1500, 3), (200, 9), (1100, 8), (2200, 8), (3000, 8), (6400, 2),
-                                     pytest.param(5000, 7, marks=pytest.mark.skip(
-                                         reason="faulted the GPU through a host bug (cooperative panels started part-way "
-                                                "with no exchange buffers); that sequence is switched off and this case "
-                                                "is not run again until the next review"))])
+@pytest.mark.parametrize("n,batch", [(130, 5), (1500, 3), (200, 9), (1100, 8), (2200, 8), (3000, 8), (6400, 2), (5000, 7)])
 def test_lu_batched(H, torch, prec, n, batch):
     """A batch of independent matrices in one getrf/getrs pass equals factoring each alone
     (LAPACK pivots for FP64, per-matrix info, solutions to the single-matrix bar).  Batches of 8 or
