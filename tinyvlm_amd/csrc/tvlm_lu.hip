@@ -1517,11 +1517,12 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     TVLM_TRY(device_workspace(sizeof(int32_t) * kPermInts * 2 * (size_t)batch, &ws, kWsLuPerm, st));
     int32_t* perm = static_cast<int32_t*>(ws);
     if (batch >= kWidePanelBatch && n <= kFastPanelRows) return getrf_lookahead<T>(a, lda, n, batch, sb, ipiv, info, perm, st);
-    // Cooperative tall panels only when the first panel already is one; then every panel of more
-    // than kCoopMinRows rows is (the group count falls with the rows), all NB wide.  A batch whose
-    // first panels are too tall for them keeps single-group panels throughout.  Exchange buffers:
-    // per cooperative panel and matrix NB counters (zeroed here, once for the whole
-    // factorisation), per matrix the candidates and published rows.
+    // Cooperative tall panels (getrf_coop_pairs) only when the first panel already is one; then
+    // every panel of more than kCoopMinRows rows is (the group count falls with the rows), all NB
+    // wide.  A batch whose first panels are too tall for them keeps single-group panels throughout
+    // (the loop below).  Buffers: per cooperative panel and matrix kCoopCtr counters, per matrix
+    // the attempt counter (all zeroed here, once for the whole factorisation), the candidates, the
+    // published rows and U11, and the deferred interchanges' gather lists.
     const int panels = (n + NB - 1) / NB;
     const bool any_coop = coop_groups<T>(n, batch) > 0;
     int32_t* ctr = nullptr;
