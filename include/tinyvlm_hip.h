@@ -166,6 +166,21 @@ int tvlm_fill_f64(int32_t location, double* ptr, int64_t stride, double value, i
 TVLM_DECLARE_BACKEND(double, f64)
 TVLM_DECLARE_BACKEND(float, f32)
 
+/* Allocation-free form of tvlm_rhs_assemble_wake_influence_f64 (same arguments, same kernels, same
+ * result bit for bit).  No counterpart in the reference: it is for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  It calls no hipMalloc/hipFree and no allocating library routine: the tile partials go
+ * to a fixed 32 MiB device array of the library (one per device, held until the process ends).
+ * A call whose partials would exceed it (tiles x receivers x 8 bytes > 32 MiB) is rejected with
+ * TVLM_ERR_INVALID before any work is queued.  Every call on a device shares that device's array:
+ * calls that overlap on one device (different streams, or host threads) race and must be ordered
+ * by the caller.  The array's address is looked up by tvlm_init, or else by the first call on a
+ * device (outside any capture). */
+int tvlm_rhs_assemble_wake_influence_noalloc_f64(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_wake,
+                                                 const tvlm_view* colloc, const tvlm_view* normals,
+                                                 const tvlm_view* verts_wake, const uint8_t* lifting, int32_t iteration,
+                                                 double sigma, void* stream);
+
 /* ---- device-resident UVLM time loop ---------------------------------------------------------
  * The caller of the hot path, UVLM::run (vlm/src/vlm.cpp:358-509), kept on the device: the host
  * evaluates the prescribed kinematics once for all steps (tvlm_uvlm_set_motion) and the whole

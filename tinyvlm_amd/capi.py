@@ -194,6 +194,21 @@ def rhs_assemble_wake_influence(rhs, gamma_wake, colloc, normals, verts_wake, li
           "rhs_assemble_wake_influence")
 
 
+def rhs_assemble_wake_influence_noalloc(rhs, gamma_wake, colloc, normals, verts_wake, lifting, iteration, sigma=0.0,
+                                        stream=None):
+    """tvlm_rhs_assemble_wake_influence_noalloc_f64: the FP64 wake induction with its tile partials in
+    the library's fixed device scratch (no device allocation); float64 tensors only."""
+    torch = _torch()
+    if rhs.dtype != torch.float64:
+        raise TvlmError("rhs_assemble_wake_influence_noalloc: float64 only")
+    n = len(colloc)
+    lift = (ctypes.c_uint8 * n)(*[1 if x else 0 for x in lifting])
+    check(lib().tvlm_rhs_assemble_wake_influence_noalloc_f64(
+        ctypes.byref(view(rhs)), ctypes.c_int32(n), views(gamma_wake), views(colloc), views(normals),
+        views(verts_wake), lift, ctypes.c_int32(iteration), ctypes.c_double(sigma), stream_ptr(stream)),
+        "rhs_assemble_wake_influence_noalloc")
+
+
 def mesh_metrics(alpha, verts_wing, colloc, normals, areas, stream=None):
     check(_fn("mesh_metrics", colloc[0])(_real(colloc[0])(alpha), ctypes.c_int32(len(colloc)), views(verts_wing),
                                         views(colloc), views(normals), views(areas), stream_ptr(stream)),

@@ -11,6 +11,8 @@
 // rows and circulations in LDS and reduces its 4 waves in LDS to one partial per receiver.  A
 // second kernel sums the partials of all tiles in a fixed order, so results are bitwise
 // reproducible (the CUDA reference uses float atomics, vlm_backend_cuda_kernels.cuh:370).
+#include <mutex>
+
 #include "tvlm_kernels.hpp"
 
 namespace tvlm {
@@ -99,10 +101,13 @@ int lhs_assemble(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, 
     return TVLM_OK;
 }
 
+// fixed_ws: when non-null, the tile partials go to this caller-provided device buffer of
+// fixed_bytes (nothing is allocated; a launch that needs more is rejected before any work is
+// queued); when null, to the library's per-stream workspace (grown on demand).
 template <typename T>
 int wake_influence(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_wake, const tvlm_view* colloc,
                    const tvlm_view* normals, const tvlm_view* verts_wake, const uint8_t* lifting, int32_t iteration,
-                   T sigma, void* stream) {
+                   T sigma, void* stream, void* fixed_ws = nullptr, size_t fixed_bytes = 0) {
     TVLM_REQUIRE(rhs && gamma_wake && colloc && normals && verts_wake && lifting, "null argument");
     TVLM_REQUIRE(n_surf >= 1 && n_surf <= TVLM_MAX_SURFACES, "n_surf %d out of range", n_surf);
     TVLM_REQUIRE(iteration >= 0, "negative iteration");
@@ -153,7 +158,16 @@ int wake_influence(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_
     if (tiles == 0 || total == 0) return TVLM_OK;
     void* ws = nullptr;
     hipStream_t st = as_stream(stream);
-    TVLM_TRY(device_workspace(sizeof(T) * (size_t)tiles * (size_t)total, &ws, kWsWakePartials, st));
+    const size_t need = sizeof(T) * (size_t)tiles * (size_t)total;
+    if (fixed_ws) {
+        TVLM_REQUIRE(need <= fixed_bytes,
+                     "wake induction needs %zu bytes of tile partials (%d tiles x %d receivers), above the %zu-byte "
+                     "fixed scratch of the allocation-free entry point",
+                     need, tiles, total, fixed_bytes);
+        ws = fixed_ws;
+    } else {
+        TVLM_TRY(device_workspace(need, &ws, kWsWakePartials, st));
+    }
     dim3 grid((total + kRecv - 1) / kRecv, tiles);
     if constexpr (sizeof(T) == 4) {
         if (sigma != T(0))
@@ -172,7 +186,41 @@ int wake_influence(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_
     return TVLM_OK;
 }
 
+// ---- allocation-free wake induction ------------------------------------------------------------
+// The tile partials of tvlm_rhs_assemble_wake_influence_noalloc_f64 live in this fixed device array
+// (each device that runs the library holds its own copy until the process ends).  Host code never
+// passes the array's name to a kernel: its device address is looked up once per device with
+// hipGetSymbolAddress and kept in g_scratch_dev.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// device address of `scratch` on the current device, looked up on the first call per device
+int scratch_address(void** out) {
+    int dev = 0;
+    TVLM_HIP(hipGetDevice(&dev));
+    TVLM_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d devices the library tracks", dev,
+                 kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_dev[dev]) {
+        void* ptr = nullptr;
+        TVLM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        TVLM_REQUIRE(ptr != nullptr, "hipGetSymbolAddress returned a null address for the scratch array");
+        g_scratch_dev[dev] = ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return TVLM_OK;
+}
+
 }  // namespace
+
+int noalloc_scratch_setup() {
+    void* ptr = nullptr;
+    return scratch_address(&ptr);
+}
 
 int lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, const tvlm_view* normals,
                             const tvlm_view* verts_wing, const tvlm_view* verts_wake, int32_t iteration, void* stream) {
@@ -198,6 +246,15 @@ int tvlm_rhs_assemble_wake_influence_f64(const tvlm_view* rhs, int32_t n_surf, c
                                          const uint8_t* lifting, int32_t iteration, double sigma, void* stream) {
     return tvlm::wake_influence<double>(rhs, n_surf, gamma_wake, colloc, normals, verts_wake, lifting, iteration, sigma,
                                         stream);
+}
+int tvlm_rhs_assemble_wake_influence_noalloc_f64(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_wake,
+                                                 const tvlm_view* colloc, const tvlm_view* normals,
+                                                 const tvlm_view* verts_wake, const uint8_t* lifting, int32_t iteration,
+                                                 double sigma, void* stream) {
+    void* ws = nullptr;
+    TVLM_TRY(tvlm::scratch_address(&ws));
+    return tvlm::wake_influence<double>(rhs, n_surf, gamma_wake, colloc, normals, verts_wake, lifting, iteration, sigma,
+                                        stream, ws, tvlm::kNoAllocScratchBytes);
 }
 int tvlm_rhs_assemble_wake_influence_f32(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_wake,
                                          const tvlm_view* colloc, const tvlm_view* normals, const tvlm_view* verts_wake,

@@ -65,6 +65,10 @@ void release_lu_stream(hipStream_t stream);
 // other).  Default priority.
 int create_queue_stream(hipStream_t* out);
 
+// Looks up the device address of the allocation-free wake induction's fixed scratch array on the
+// current device (once per device; tvlm_init calls it so that later calls find it set up).
+int noalloc_scratch_setup();
+
 // lhs_assemble in the FP64 seeded form (not part of the C ABI; the served steady solve uses it)
 int lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, const tvlm_view* normals,
                             const tvlm_view* verts_wing, const tvlm_view* verts_wake, int32_t iteration, void* stream);

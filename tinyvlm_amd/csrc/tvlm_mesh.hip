@@ -455,6 +455,7 @@ int tvlm_init(int32_t device) {
     TVLM_HIP(hipGetDeviceCount(&n));
     TVLM_REQUIRE(device >= 0 && device < n, "device %d not present (%d devices)", device, n);
     TVLM_HIP(hipSetDevice(device));
+    TVLM_TRY(noalloc_scratch_setup());
     return TVLM_OK;
 }
 
