@@ -244,6 +244,20 @@ def coeff_cm(forces, verts, ref_pt, freestream, rho, area, mac):
     return out
 
 
+def coeff_cm_multibody(forces_list, surfs: list[Surface], ref_pt, freestream, rho):
+    """coeff_cm_multibody (vlm/src/vlm_backend.cpp:102-139) over surfaces whose metrics are set."""
+    dt = forces_list[0].dtype
+    rp = np.ascontiguousarray(ref_pt, dtype=dt)
+    fs = np.ascontiguousarray(freestream, dtype=dt)
+    out = np.zeros(3, dtype=dt)
+    forces_list = [np.asfortranarray(f) for f in forces_list]
+    getattr(lib(), f"orc_coeff_cm_multibody_{_sfx(dt)}")(
+        _I32(len(surfs)), _i64s([s.ns for s in surfs]), _i64s([s.nc for s in surfs]), _ptrs(forces_list),
+        _ptrs([s.verts for s in surfs]), _ptrs([s.areas for s in surfs]), _P(_ptr(rp)), _P(_ptr(fs)), _real(dt)(rho),
+        _P(_ptr(out)))
+    return out
+
+
 def sum_(x):
     x = np.asfortranarray(x)
     return getattr(lib(), f"orc_sum_{_sfx(x.dtype)}")(_I64(x.size), _P(_ptr(x)))
