@@ -166,6 +166,21 @@ int tvlm_fill_f64(int32_t location, double* ptr, int64_t stride, double value, i
 TVLM_DECLARE_BACKEND(double, f64)
 TVLM_DECLARE_BACKEND(float, f32)
 
+/* LU factorisation plan (no counterpart in the reference; host only, no device needed): the path
+ * tvlm_getrf_{f32,f64} takes for n x n matrices in batches of `batch` (elem_bytes 4 or 8):
+ * *path = 0 batched look-ahead (one-group panels), 1 cooperative panels (several work-groups per
+ * panel, waiting on each other through bounded spins), 2 one-group panels throughout.  For path 1,
+ * *n_steps panels of 64 columns: groups[k] work-groups of panel k (0 = one-group panel) and the
+ * exchange counter set ctr_set[k] it uses (-1 for one-group panels); *coop_ws_bytes the exchange
+ * buffers getrf allocates.  groups/ctr_set may be null; otherwise they hold max_steps entries. */
+int tvlm_getrf_plan(int64_t n, int32_t batch, int32_t elem_bytes, int32_t* path, int32_t* n_steps, int32_t* groups,
+                    int32_t* ctr_set, int32_t max_steps, int64_t* coop_ws_bytes);
+/* Bound of every wait of the cooperative LU panels, in spins (default 4194304; 0 restores it).  No
+ * counterpart in the reference: a diagnostic knob with which tests make the bounded waits give up,
+ * so that the recovery of a stalled factorisation (info = -999, see tvlm_vlm_refinement) runs.  A
+ * wait that gives up never hangs or faults; it leaves that matrix's factors invalid and info -999. */
+int tvlm_lu_set_coop_spin_limit(uint32_t limit);
+
 /* Allocation-free form of tvlm_rhs_assemble_wake_influence_f64 (same arguments, same kernels, same
  * result bit for bit).  No counterpart in the reference: it is for callers that cannot allocate
  * device memory -- code under HIP graph capture, tight serving loops, harnesses that forbid
@@ -189,7 +204,8 @@ int tvlm_rhs_assemble_wake_influence_noalloc_f64(const tvlm_view* rhs, int32_t n
  * mode 0: plain FP64 (exact FP64 Biot-Savart, LU forward/backward substitution every step).
  * mode 1: wake induction in the FP64 seeded form (FP64 arithmetic, 1/sqrt and 1/x from the FP64
  *         hardware estimates plus one Newton step), gamma = A^{-1} rhs with the FP64 inverse formed
- *         once from the LU; use_graph replays the loop as a hipGraph (needs a non-default stream).
+ *         once from the LU; use_graph replays the loop as a hipGraph (needs a non-default stream;
+ *         mode 1 only: plain mode runs call by call, use_graph != 0 is TVLM_ERR_INVALID).
  * All surfaces are lifting; LHS assembly uses iteration 0 as the reference does (vlm.cpp:402). */
 typedef struct tvlm_uvlm_plan tvlm_uvlm_plan;
 /* verts_init[m]: host {ns+1, nc+1, 4} column-major double (w = 1); t_steps as UVLM::run's
@@ -254,6 +270,17 @@ int tvlm_vlm_solve_n(tvlm_vlm_plan* plan, int32_t n_active, const double* alpha,
 /* waits for the plan's last solve; cl[n], cm[n][3], gamma[n][N], info[n] (getrf status per request)
  * for the n requests of the last solve; null pointers are skipped */
 int tvlm_vlm_results(tvlm_vlm_plan* plan, double* cl, double* cm, double* gamma, int32_t* info);
+/* What the served refinement (mode 1) took for each of the n requests of the last solve; null
+ * pointers are skipped.  The returned x of every request is tested (||b - A x||_inf <= sqrt(N) eps
+ * ||x||_inf ||A||_inf against the FP64 matrix, LAPACK dsgesv's stopping test) before results are
+ * returned; status bits: 0 = passed after the fixed 4 corrections, 1 = needed more corrections (at
+ * most 8 in all), 2 = failed them and was solved by the FP64 LU of its FP64 matrix (the plain
+ * path's solve), 4 = a cooperative panel of its LU gave up waiting (info -999) and the matrix was
+ * re-factored with one-group panels, 8 = unresolved (the FP64 matrix is singular: info > 0).
+ * corrections[b]: FP32 corrections applied; backward_error[b]: ||b - A x||_inf / (||x||_inf
+ * ||A||_inf) of the returned x (NaN after the FP64 fallback).  Mode 0: all zero.  The test, and the
+ * extra work of requests that fail it, run inside tvlm_vlm_results (or here, whichever comes first). */
+int tvlm_vlm_refinement(tvlm_vlm_plan* plan, int32_t* status, int32_t* corrections, double* backward_error);
 /* moves the plan to another stream: its later solves queue there after its own latest work (a
  * device-side wait; the host does not block).  A serving loop keeps plans per mesh and runs each
  * batch on whichever stream is free.  Not while a solve of the plan is being queued elsewhere. */

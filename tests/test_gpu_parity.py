@@ -242,6 +242,45 @@ def test_lu_singular_info(H, torch):
     assert int(info.item()) == 2
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("n,batch", [(300, 1), (2048, 1), (200, 8), (1500, 3)])
+def test_lu_nan_columns_keep_pivots_in_range(H, torch, prec, n, batch):
+    """A column of NaNs wins no pivot comparison.  Every pivot must still be a row of the matrix
+    (1 <= ipiv <= n; a NaN column takes its diagonal row, as LAPACK's idamax stays on the first
+    entry), so that the interchanges and the solve stay inside the matrix; the other matrices of a
+    batch are factored exactly as without the NaN member.  Paths: one-group panels (300), cooperative
+    panels (2048), the batched look-ahead (200 x 8), cooperative panels over a batch (1500 x 3)."""
+    import scipy.linalg as sl
+
+    print(f"[case] lu NaN column n={n} batch={batch} {prec}", flush=True)
+    dt = DT(torch, prec)
+    rng = np.random.default_rng(77 + n)
+    As = [rng.standard_normal((n, n)) for _ in range(batch)]
+    As[0][:, n // 3] = np.nan  # member 0: NaN from column n/3 on, through the elimination
+    a = H.colmajor((n, n, batch), dt)
+    x = H.colmajor((n, 1, batch), dt)
+    for b in range(batch):
+        a[:, :, b] = H.to_device(As[b], dt)
+        x[:, :, b] = H.to_device(np.ones((n, 1)), dt)
+    ipiv = torch.zeros(n * batch, dtype=torch.int32, device="cuda")
+    info = torch.full((batch,), -1, dtype=torch.int32, device="cuda")
+    H.getrf(a, ipiv, info)
+    H.getrs(a, ipiv, x)
+    torch.cuda.synchronize()
+    piv = ipiv.cpu().numpy().reshape(batch, n)
+    assert piv.min() >= 1 and piv.max() <= n
+    # pivots never point above their own row: row k is exchanged with a row k' >= k
+    assert np.all(piv - 1 >= np.arange(n)[None, :])
+    assert not np.all(np.isfinite(H.to_host(x[:, :, 0])))
+    for b in range(1, batch):
+        X = H.to_host(x[:, :, b])
+        res = np.linalg.norm(As[b] @ X - 1.0) / (np.linalg.norm(As[b]) * np.linalg.norm(X))
+        eps = np.finfo(np.float64 if prec == "f64" else np.float32).eps
+        assert res < 4 * n * eps
+        if prec == "f64":
+            assert np.array_equal(piv[b] - 1, sl.lu_factor(As[b])[1])
+
+
 def test_getrs_many_rhs(H, torch):
     n, k = 600, 11  # harmonic-balance shape: one factor, 2H+1 right-hand sides
     rng = np.random.default_rng(11)

@@ -176,3 +176,37 @@ def test_unsteady_requests_in_the_mix(mix, oracle):
     for r in requests:
         if r.kind == "steady":
             _check_same(r, _alone(meshes, r))
+
+
+def test_large_mix_meshes_through_the_server(tmp_path):
+    """The mix's large meshes (N = 4096, 8100, 16384: rectangular 64 x 64, swept 90 x 90, the
+    pivot-heavy elliptic 128 x 128) served beside small ones through Server: every request equals
+    the same request solved alone, and the plain FP64 solve within 1e-10 (circulation, relative;
+    CL absolute); the refinement test passed on the fixed corrections for all of them."""
+    from tinyvlm_amd import capi as H
+    from tinyvlm_amd import serve as S
+    from tinyvlm_amd.steady_plan import MODE_PLAIN_FP64, SteadyPlan
+
+    H.init(0)
+    d = str(tmp_path)
+    keys = [("rectangular", 64), ("swept25", 90), ("elliptic", 128), ("elliptic", 16), ("swept25", 32)]
+    meshes = {f"{f}_{n}": [S.mix_mesh(f, n, d)] for f, n in keys}
+    spec = [("elliptic_128", 4.0), ("rectangular_64", -3.0), ("swept25_90", 7.5), ("elliptic_16", 1.0),
+            ("elliptic_128", -1.5), ("swept25_32", 9.0), ("rectangular_64", 2.5), ("swept25_90", 0.5)]
+    requests = [S.Request(i, k, float(np.deg2rad(a)), arrival=1e-3 * i) for i, (k, a) in enumerate(spec)]
+    print(f"[case] serve large mix meshes: {len(requests)} requests", flush=True)
+    with S.Server(meshes, lanes=3, keep_gamma=True) as srv:
+        st = srv.run(requests)
+    assert st.max_running >= 2
+    for r in requests:
+        assert r.refine == 0, (r.rid, r.mesh, r.refine)
+        alone = _alone(meshes, r)
+        _check_same(r, alone)
+        p = SteadyPlan(meshes[r.mesh], 1, mode=MODE_PLAIN_FP64)
+        p.solve([r.alpha])
+        plain = p.results()
+        p.close()
+        g = plain["gamma"][0]
+        print(f"[case]   {r.mesh} rid {r.rid}: served vs plain |dCL| {abs(r.cl - plain['cl'][0]):.2e}", flush=True)
+        assert np.max(np.abs(r.gamma - g)) <= TOL * np.max(np.abs(g)), r.rid
+        assert abs(r.cl - plain["cl"][0]) <= TOL, r.rid

@@ -41,6 +41,12 @@ class FakePlan:
             out["gamma"] = np.repeat(a[:, None], self.N, axis=1)
         return out
 
+    def refinement(self):
+        # a request with alpha < -0.05 pretends its refinement needed the FP64 fallback
+        a = self.alphas
+        return {"status": np.where(a < -0.05, 2, 0).astype(np.int32), "corrections": np.full(len(a), 4, np.int32),
+                "backward_error": np.zeros(len(a))}
+
     def set_stream(self, s):
         assert not self.busy
         self.stream = s
@@ -54,10 +60,14 @@ class FakePlan:
 
 
 class FakeUVLM:
+    live = []
+
     def __init__(self, verts_list, t_steps, dt, mode=1, stream=None):
         self.stream, self.t_steps = stream, t_steps
         self.motion = None
         self.polls = 0
+        self.closed = False
+        FakeUVLM.live.append(self)
 
     def set_pitch_heave(self, **m):
         self.motion = m
@@ -75,7 +85,7 @@ class FakeUVLM:
         return {"cl": cl, "cm": np.zeros((n, 3)), "gamma": np.zeros(4)}
 
     def close(self):
-        pass
+        self.closed = True
 
 
 @pytest.fixture
@@ -88,6 +98,7 @@ def S(monkeypatch):
     monkeypatch.setattr(serve, "SteadyPlan", FakePlan)
     monkeypatch.setattr(serve, "UVLMPlan", FakeUVLM)
     FakePlan.live = []
+    FakeUVLM.live = []
     return serve
 
 
@@ -182,3 +193,46 @@ def test_request_errors(S):
             srv.run([S.Request(0, "w", kind="other")], realtime=False)
     with pytest.raises(KeyError):
         S.Server(meshes, lanes=1, budget_bytes=1 << 40, unsteady={"x": (11, 0.1)})
+
+
+def test_refinement_status_is_recorded_and_counted(S):
+    meshes = {"a": _mesh(16)}
+    alphas = [-0.08, 0.02, -0.06, 0.1, 0.0]
+    reqs = _requests(S, [("a", x) for x in alphas])
+    with S.Server(meshes, lanes=1, max_batch=2, budget_bytes=1 << 40) as srv:
+        st = srv.run(reqs, realtime=False)
+    assert [r.refine for r in reqs] == [2 if x < -0.05 else 0 for x in alphas]
+    assert st.fallbacks == 2 and st.more_corrections == 0 and st.refactored == 0 and st.unresolved == 0
+
+
+def test_unsteady_plan_larger_than_budget_raises(S):
+    meshes = {"w": _mesh(8)}
+    with S.Server(meshes, lanes=1, budget_bytes=1000, unsteady={"w": (11, 0.1)}) as srv:
+        with pytest.raises(RuntimeError, match="exceeds the memory budget"):
+            srv.run([S.Request(0, "w", kind="unsteady", motion=dict(u=1.0, h_amp=0.0))], realtime=False)
+
+
+def test_idle_unsteady_plans_are_evicted_for_steady_plans(S):
+    """An idle unsteady plan holding most of the budget is evicted when a steady plan needs room
+    (before the fix, steady requests waited for ever behind it)."""
+    meshes = {"w": _mesh(40), "s": _mesh(40)}
+    with S.Server(meshes, lanes=1, max_batch=1, budget_bytes=1 << 40, unsteady={"w": (11, 0.1)}) as probe:
+        u_need, s_need = probe._uestimate("w"), probe._estimate("s")
+    budget = max(u_need, s_need) + min(u_need, s_need) // 2  # one plan of either kind fits, not both
+    reqs = [S.Request(0, "w", kind="unsteady", motion=dict(u=1.0, h_amp=0.5)),
+            S.Request(1, "s", 0.05, arrival=0.001)]
+    with S.Server(meshes, lanes=1, max_batch=1, budget_bytes=budget, unsteady={"w": (11, 0.1)}) as srv:
+        st = srv.run(reqs, realtime=False)
+        assert srv.used_bytes <= budget
+    assert st.plans_evicted == 1 and FakeUVLM.live[0].closed
+    assert np.all(reqs[0].cl_t == 0.5) and reqs[1].cl == pytest.approx(2 * np.pi * 0.05)
+
+
+def test_stalled_admission_raises_instead_of_spinning(S, monkeypatch):
+    """If nothing runs and no waiting queue can be placed, nothing will ever free a lane or memory:
+    the loop must raise, not sleep for ever."""
+    meshes = {"a": _mesh(8)}
+    with S.Server(meshes, lanes=2, budget_bytes=1 << 40) as srv:
+        monkeypatch.setattr(srv, "_slot_for", lambda key, lane, now: None)
+        with pytest.raises(RuntimeError, match="serving stalled"):
+            srv.run(_requests(S, [("a", 0.0), ("a", 0.1)]), realtime=False)

@@ -69,6 +69,12 @@ int create_queue_stream(hipStream_t* out);
 // current device (once per device; tvlm_init calls it so that later calls find it set up).
 int noalloc_scratch_setup();
 
+// getrf with every panel in one work-group (no cooperative panels, hence no waits between
+// work-groups): the recovery path after a cooperative panel's bounded wait gave up
+int getrf_single_group_f64(const tvlm_view* a, int32_t* ipiv, int32_t* info, void* stream);
+int getrf_single_group_f32(const tvlm_view* a, int32_t* ipiv, int32_t* info, void* stream);
+constexpr int32_t kGetrfStalled = -999;  // info of a matrix whose cooperative panel gave up waiting
+
 // lhs_assemble in the FP64 seeded form (not part of the C ABI; the served steady solve uses it)
 int lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, const tvlm_view* normals,
                             const tvlm_view* verts_wing, const tvlm_view* verts_wake, int32_t iteration, void* stream);

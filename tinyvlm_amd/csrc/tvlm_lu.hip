@@ -22,6 +22,7 @@
 // (k_solve_blocked); from n = 1024 with few right-hand sides, the interchanges are applied alone
 // (k_solve_perm) and each 64-row block step runs on all trailing row blocks at once
 // (k_trsv_step); larger n uses block-row kernels (k_diag_solve / k_off_update).
+#include <atomic>
 #include <map>
 #include <utility>
 #include <mutex>
@@ -421,7 +422,9 @@ __global__ __launch_bounds__(NT) void k_panel_reg(T* __restrict__ a, int64_t lda
                 bi = s_idx[0];
 #pragma unroll
                 for (int ww = 1; ww < NW; ww++) argmax_merge(best, bi, s_val[ww], s_idx[ww]);
-                const int p = bi;
+                // a column of NaNs wins no comparison and leaves bi = n: take row c then (LAPACK's
+                // idamax also stays on the first entry), so every row index below stays in [c, n)
+                const int p = (bi >= c && bi < n) ? bi : c;
                 const int tp = (p - k0) % NT, qp = (p - k0) / NT;
                 const int tc = (c - k0) % NT, qc = (c - k0) / NT;
                 if (t == tp) {
@@ -588,7 +591,7 @@ __global__ __launch_bounds__(NT) void k_panel_reg(T* __restrict__ a, int64_t lda
 // ctr[NB] U11 published, ctr[NB + 1] the votes, ctr[NB + 2] a failed vote.
 constexpr int kCoopMaxGroups = 64;   // work-groups of one launch; all of them must be resident at once
 constexpr int kCoopMinRows = 1024;   // shorter panels stay in one work-group (k_panel_reg)
-constexpr unsigned kCoopSpinLimit = 1u << 22;
+constexpr unsigned kCoopSpinLimit = 1u << 22;  // default bound of every wait (tvlm_lu_set_coop_spin_limit)
 constexpr int kCoopStalled = -999;
 constexpr int kCoopCtr = NB + 4;                                                      // counters per panel and matrix
 constexpr int64_t kCoopXrow = (int64_t)NB * (kCoopMaxGroups + 1) * NB + NB * NB;  // xrow per matrix (+ U11)
@@ -688,7 +691,8 @@ template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t lda, int n, int k0, int kb, int G,
                                                    int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
                                                    int32_t* __restrict__ perm, int64_t sb, int32_t* ctr, T* cval,
-                                                   int32_t* crow, T* xrow, int32_t* ostate) {
+                                                   int32_t* crow, T* xrow, int32_t* ostate, unsigned spin_limit,
+                                                   int32_t* fail) {
     constexpr int NW = NT / 64, MG = kCoopMaxGroups;
     const int z = blockIdx.z, g = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
     a += (int64_t)z * sb;
@@ -745,7 +749,7 @@ __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t ld
             if (t == 0) {
                 unsigned spins = 0;
                 while (__hip_atomic_load(&octr[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < 1) {
-                    if (++spins > kCoopSpinLimit) {
+                    if (++spins > spin_limit) {
                         s_fail = 1;
                         break;
                     }
@@ -787,7 +791,7 @@ __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t ld
             unsigned spins = 0;
             bool stalled = false;
             while (__hip_atomic_load(&octr[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < G) {
-                if (++spins > kCoopSpinLimit) {
+                if (++spins > spin_limit) {
                     stalled = true;
                     s_fail = 1;
                     break;
@@ -840,7 +844,7 @@ __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t ld
                 __hip_atomic_fetch_add(&ctr[j], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 unsigned spins = 0;
                 while (__hip_atomic_load(&ctr[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < G) {
-                    if (++spins > kCoopSpinLimit) {
+                    if (++spins > spin_limit) {
                         s_fail = 1;
                         break;
                     }
@@ -859,11 +863,12 @@ __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t ld
             }
             wave_argmax(v, ri);
             if (lane == 0) {
-                // a pivot row outside [c, n) can only come from a failed exchange: keep every
+                // ri == n: no group had a candidate (a column of NaNs): row c, as LAPACK's idamax;
+                // any other pivot row outside [c, n) can only come from a failed exchange: keep every
                 // index below in bounds and report it
                 const bool ok = ri >= c && ri < n;
                 s_p = ok ? ri : c;
-                if (!ok) s_fail = 1;
+                if (!ok && ri != n) s_fail = 1;
             }
         }
         __syncthreads();
@@ -896,6 +901,8 @@ __global__ __launch_bounds__(NT) void k_panel_coop(T* __restrict__ a, int64_t ld
     }
     if (mine)
         for (int jj = 0; jj < kb; jj++) a[k0 + i + (int64_t)(k0 + jj) * lda] = xs[jj * NT + t];
+    // any group whose wait ran out marks the matrix (k_coop_fail_info turns the mark into info)
+    if (t == 0 && s_fail) __hip_atomic_fetch_or(&fail[z], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (g != 0) return;
     if (t < kb) ipiv[k0 + t] = s_arg[t] + 1;
     if (t == 0) {
@@ -1245,9 +1252,18 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n) {
 
 constexpr size_t kCoopLdsBytes = 128 * 1024;  // NT rows x NB columns (coop_threads)
 
+std::atomic<unsigned> g_coop_spin_limit{kCoopSpinLimit};
+
+// info[z] = kCoopStalled for every matrix a cooperative panel's wait gave up on (by any group)
+__global__ void k_coop_fail_info(const int32_t* __restrict__ fail, int32_t* __restrict__ info, int batch) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z < batch && fail[z] != 0) info[z] = kCoopStalled;
+}
+
 template <typename T>
 int panel_coop(T* a, int64_t lda, int n, int k0, int kb, int G, int32_t* ipiv, int32_t* info, int32_t* perm,
-               hipStream_t st, int batch, int64_t sb, int32_t* ctr, T* cval, int32_t* crow, T* xrow, int32_t* ostate) {
+               hipStream_t st, int batch, int64_t sb, int32_t* ctr, T* cval, int32_t* crow, T* xrow, int32_t* ostate,
+               int32_t* fail) {
     constexpr int NT = coop_threads<T>();
     static const hipError_t lds_attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_panel_coop<T, NT>),
                                                            hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1255,9 +1271,9 @@ int panel_coop(T* a, int64_t lda, int n, int k0, int kb, int G, int32_t* ipiv, i
     TVLM_HIP(lds_attr);
     TVLM_REQUIRE(G >= 2 && G * batch <= kCoopMaxGroups && kb <= NB && (int64_t)G * NT >= (int64_t)(n - k0),
                  "cooperative panel: %d groups for %d rows", G, n - k0);
-    TVLM_REQUIRE(ctr && cval && crow && xrow && ostate, "cooperative panel without exchange buffers");
+    TVLM_REQUIRE(ctr && cval && crow && xrow && ostate && fail, "cooperative panel without exchange buffers");
     hipLaunchKernelGGL((k_panel_coop<T, NT>), dim3(G, 1, batch), dim3(NT), kCoopLdsBytes, st, a, lda, n, k0, kb, G,
-                       ipiv, info, perm, sb, ctr, cval, crow, xrow, ostate);
+                       ipiv, info, perm, sb, ctr, cval, crow, xrow, ostate, g_coop_spin_limit.load(), fail);
     TVLM_LAUNCHED();
     return TVLM_OK;
 }
@@ -1439,24 +1455,82 @@ int defer_gather(T* a, int64_t lda, int n, int batch, int64_t sb, const int32_t*
 // products as one panel at a time, summed in a different order.
 constexpr int kPairMinRows = 6144;  // trailing matrices at least this tall are updated once per pair
 
+// ---- the factorisation plan -------------------------------------------------------------------
+// Which path a getrf call takes and, for the cooperative path, every panel's form and exchange
+// buffers; computed on the host before anything is queued (getrf_plan), executed by getrf, and
+// exported as tvlm_getrf_plan so that the schedule and the buffer sizing can be checked without a GPU.
+//   kPathLookahead : batches of >= kWidePanelBatch matrices of <= kFastPanelRows rows (one-group panels
+//                    beside the trailing updates on a side stream);
+//   kPathCoopPairs : the first panel is cooperative; then every panel of more than kCoopMinRows rows is
+//                    (coop_groups falls with the rows) and takes its own counter set;
+//   kPathPanels    : one work-group per panel throughout (k_panel_reg, or the column-by-column k_panel
+//                    above kFastPanelRows rows) -- also every factorisation with cooperative panels
+//                    switched off.
+enum GetrfPath { kPathLookahead = 0, kPathCoopPairs = 1, kPathPanels = 2 };
+
+struct GetrfStep {
+    int kp, kb;   // first column and width of the panel
+    int groups;   // work-groups of a cooperative panel, 0 for a one-group panel
+    int ctr_set;  // counter set of a cooperative panel, -1 otherwise
+};
+
+struct GetrfPlan {
+    int path = kPathPanels;
+    std::vector<GetrfStep> steps;  // kPathCoopPairs only
+    int ctr_sets = 0;              // counter sets allocated (one per cooperative panel)
+    // exchange buffers (kPathCoopPairs): ints (counters, per-matrix attempt state and failure word,
+    // candidate rows, gather lists) and reals (candidate values, published rows and U11)
+    size_t nctr = 0, ncand = 0, nrow = 0, ngat = 0;
+    size_t coop_bytes(size_t real_bytes) const {
+        return nctr || ncand ? sizeof(int32_t) * (nctr + ncand + ngat) + real_bytes * (ncand + nrow) + 512 : 0;
+    }
+};
+
+template <typename T>
+GetrfPlan getrf_plan(int n, int batch, bool allow_coop) {
+    GetrfPlan P;
+    if (batch >= kWidePanelBatch && n <= kFastPanelRows) {
+        P.path = kPathLookahead;
+        return P;
+    }
+    if (!allow_coop || coop_groups<T>(n, batch) == 0) {
+        P.path = kPathPanels;
+        return P;
+    }
+    P.path = kPathCoopPairs;
+    for (int kp = 0; kp < n; kp += NB) {
+        GetrfStep st{kp, n - kp < NB ? n - kp : NB, coop_groups<T>(n - kp, batch), -1};
+        if (st.groups > 0) st.ctr_set = P.ctr_sets++;
+        P.steps.push_back(st);
+    }
+    const int panels = (n + NB - 1) / NB;
+    P.nctr = (size_t)P.ctr_sets * batch * kCoopCtr + 2 * (size_t)batch;
+    P.ncand = (size_t)batch * NB * kCoopMaxGroups;
+    P.nrow = (size_t)batch * kCoopXrow;
+    P.ngat = (size_t)batch * panels * n;
+    return P;
+}
+
 template <typename T>
 int getrf_coop_pairs(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* ipiv, int32_t* info, int32_t* perm,
-                     hipStream_t st, int32_t* ctr, int panels, T* cval, int32_t* crow, T* xrow, int32_t* gtab,
-                     int32_t* ostate) {
-    int ci = 0;  // cooperative panels so far: counter set of the next one
+                     hipStream_t st, const GetrfPlan& plan, int32_t* ctr, T* cval, int32_t* crow, T* xrow,
+                     int32_t* gtab, int32_t* ostate, int32_t* fail) {
     if (n > NB) {
         const int64_t total = (int64_t)batch * ((n + NB - 1) / NB) * n;
         hipLaunchKernelGGL(k_iota_lists, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, gtab, n, total);
         TVLM_LAUNCHED();
     }
     auto panel = [&](int kp, int kb) -> int {
-        const int G = coop_groups<T>(n - kp, batch);
-        if (G == 0) return panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb);
-        TVLM_REQUIRE(ci < panels, "cooperative panel %d of at most %d", ci, panels);
-        const int rc = panel_coop<T>(a, lda, n, kp, kb, G, ipiv, info, perm, st, batch, sb,
-                                     ctr + (size_t)ci * batch * kCoopCtr, cval, crow, xrow, ostate);
-        ci++;
-        return rc;
+        // the planned panel at kp: its width, group count and exchange counter set
+        const int s = kp / NB;
+        TVLM_REQUIRE(kp % NB == 0 && s < (int)plan.steps.size() && plan.steps[s].kp == kp && plan.steps[s].kb == kb,
+                     "panel at column %d (width %d) is not in the factorisation plan", kp, kb);
+        const GetrfStep& p = plan.steps[s];
+        if (p.groups == 0) return panel_fast<T>(a, lda, n, kp, kb, ipiv, info, perm, st, batch, sb);
+        TVLM_REQUIRE(p.ctr_set >= 0 && p.ctr_set < plan.ctr_sets, "cooperative panel %d: counter set %d of %d", s,
+                     p.ctr_set, plan.ctr_sets);
+        return panel_coop<T>(a, lda, n, kp, kb, p.groups, ipiv, info, perm, st, batch, sb,
+                             ctr + (size_t)p.ctr_set * batch * kCoopCtr, cval, crow, xrow, ostate, fail);
     };
     // the left columns before left0 take the interchanges into their gather lists
     auto swap_trsm = [&](int kp, int kb, int mode, int left0) -> int {
@@ -1494,11 +1568,14 @@ int getrf_coop_pairs(T* a, int64_t lda, int n, int batch, int64_t sb, int32_t* i
         TVLM_LAUNCHED();
         kp = jB;
     }
-    return defer_gather<T>(a, lda, n, batch, sb, gtab, st);
+    TVLM_TRY(defer_gather<T>(a, lda, n, batch, sb, gtab, st));
+    hipLaunchKernelGGL(k_coop_fail_info, dim3((batch + 255) / 256), dim3(256), 0, st, fail, info, batch);
+    TVLM_LAUNCHED();
+    return TVLM_OK;
 }
 
 template <typename T>
-int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
+int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream, bool allow_coop = true) {
     TVLM_REQUIRE(av && av->ptr && ipiv && info, "null argument");
     TVLM_REQUIRE(av->stride[0] == 1, "A must be column-major");
     TVLM_REQUIRE(av->shape[0] == av->shape[1], "A must be square (got %lld x %lld)", (long long)av->shape[0],
@@ -1516,35 +1593,28 @@ int getrf(const tvlm_view* av, int32_t* ipiv, int32_t* info, void* stream) {
     void* ws = nullptr;
     TVLM_TRY(device_workspace(sizeof(int32_t) * kPermInts * 2 * (size_t)batch, &ws, kWsLuPerm, st));
     int32_t* perm = static_cast<int32_t*>(ws);
-    if (batch >= kWidePanelBatch && n <= kFastPanelRows) return getrf_lookahead<T>(a, lda, n, batch, sb, ipiv, info, perm, st);
-    // Cooperative tall panels (getrf_coop_pairs) only when the first panel already is one; then
-    // every panel of more than kCoopMinRows rows is (the group count falls with the rows), all NB
-    // wide.  A batch whose first panels are too tall for them keeps single-group panels throughout
-    // (the loop below).  Buffers: per cooperative panel and matrix kCoopCtr counters, per matrix
-    // the attempt counter (all zeroed here, once for the whole factorisation), the candidates, the
-    // published rows and U11, and the deferred interchanges' gather lists.
-    const int panels = (n + NB - 1) / NB;
-    const bool any_coop = coop_groups<T>(n, batch) > 0;
-    int32_t* ctr = nullptr;
-    T *cval = nullptr, *xrow = nullptr;
-    int32_t* crow = nullptr;
-    if (any_coop) {
-        const size_t nctr = (size_t)panels * batch * kCoopCtr + batch, ncand = (size_t)batch * NB * kCoopMaxGroups;
-        const size_t nrow = (size_t)batch * kCoopXrow;
-        const size_t ngat = (size_t)batch * panels * n;  // gather lists of the deferred interchanges
+    const GetrfPlan plan = getrf_plan<T>(n, batch, allow_coop);
+    if (plan.path == kPathLookahead) return getrf_lookahead<T>(a, lda, n, batch, sb, ipiv, info, perm, st);
+    if (plan.path == kPathCoopPairs) {
+        // Buffers (sized by the plan): per cooperative panel and matrix kCoopCtr counters, per matrix
+        // the attempt counter and the failure word (all zeroed here, once for the whole
+        // factorisation), the candidates, the published rows and U11, and the deferred
+        // interchanges' gather lists.
         void* cws = nullptr;
-        TVLM_TRY(device_workspace(sizeof(int32_t) * (nctr + ncand + ngat) + sizeof(T) * (ncand + nrow) + 512, &cws,
-                                  kWsLuCoop, st));
-        ctr = static_cast<int32_t*>(cws);
-        crow = ctr + nctr;
-        cval = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(crow + ncand) + 255) & ~(uintptr_t)255);
-        xrow = cval + ncand;
-        int32_t* gtab = reinterpret_cast<int32_t*>((reinterpret_cast<uintptr_t>(xrow + nrow) + 255) & ~(uintptr_t)255);
-        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nctr + 255) / 256)), dim3(256), 0, st, ctr, (int64_t)nctr);
+        TVLM_TRY(device_workspace(plan.coop_bytes(sizeof(T)), &cws, kWsLuCoop, st));
+        int32_t* ctr = static_cast<int32_t*>(cws);
+        int32_t* crow = ctr + plan.nctr;
+        T* cval = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(crow + plan.ncand) + 255) & ~(uintptr_t)255);
+        T* xrow = cval + plan.ncand;
+        int32_t* gtab =
+            reinterpret_cast<int32_t*>((reinterpret_cast<uintptr_t>(xrow + plan.nrow) + 255) & ~(uintptr_t)255);
+        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((plan.nctr + 255) / 256)), dim3(256), 0, st, ctr,
+                           (int64_t)plan.nctr);
         TVLM_LAUNCHED();
-        int32_t* ostate = ctr + (size_t)panels * batch * kCoopCtr;  // per matrix: failed attempts in a row
-        return getrf_coop_pairs<T>(a, lda, n, batch, sb, ipiv, info, perm, st, ctr, panels, cval, crow, xrow, gtab,
-                                   ostate);
+        int32_t* ostate = ctr + (size_t)plan.ctr_sets * batch * kCoopCtr;  // per matrix: failed attempts in a row
+        int32_t* fail = ostate + batch;                                     // per matrix: a wait gave up
+        return getrf_coop_pairs<T>(a, lda, n, batch, sb, ipiv, info, perm, st, plan, ctr, cval, crow, xrow, gtab,
+                                   ostate, fail);
     }
     for (int k0 = 0; k0 < n;) {
         // tall panels of one matrix are narrower: the in-panel left-looking traffic grows with width
@@ -1956,9 +2026,40 @@ void release_lu_stream(hipStream_t stream) {
     if (L.rest_done) (void)hipEventDestroy(L.rest_done);
 }
 
+int getrf_single_group_f64(const tvlm_view* a, int32_t* ipiv, int32_t* info, void* stream) {
+    return getrf<double>(a, ipiv, info, stream, false);
+}
+int getrf_single_group_f32(const tvlm_view* a, int32_t* ipiv, int32_t* info, void* stream) {
+    return getrf<float>(a, ipiv, info, stream, false);
+}
+
 }  // namespace tvlm
 
 extern "C" {
+int tvlm_getrf_plan(int64_t n, int32_t batch, int32_t elem_bytes, int32_t* path, int32_t* n_steps, int32_t* groups,
+                    int32_t* ctr_set, int32_t max_steps, int64_t* coop_ws_bytes) {
+    TVLM_REQUIRE(path && n_steps, "null argument");
+    TVLM_REQUIRE(n >= 0 && n <= (int64_t)1 << 30 && batch >= 1 && batch <= 65535, "n %lld / batch %d out of range",
+                 (long long)n, batch);
+    TVLM_REQUIRE(elem_bytes == 4 || elem_bytes == 8, "elem_bytes must be 4 or 8");
+    const tvlm::GetrfPlan P = elem_bytes == 8 ? tvlm::getrf_plan<double>((int)n, batch, true)
+                                              : tvlm::getrf_plan<float>((int)n, batch, true);
+    *path = P.path;
+    *n_steps = (int32_t)P.steps.size();
+    if (coop_ws_bytes) *coop_ws_bytes = (int64_t)P.coop_bytes((size_t)elem_bytes);
+    if (groups || ctr_set) {
+        TVLM_REQUIRE((int64_t)P.steps.size() <= (int64_t)max_steps, "%zu panels, room for %d", P.steps.size(), max_steps);
+        for (size_t k = 0; k < P.steps.size(); k++) {
+            if (groups) groups[k] = P.steps[k].groups;
+            if (ctr_set) ctr_set[k] = P.steps[k].ctr_set;
+        }
+    }
+    return TVLM_OK;
+}
+int tvlm_lu_set_coop_spin_limit(uint32_t limit) {
+    tvlm::g_coop_spin_limit.store(limit == 0 ? tvlm::kCoopSpinLimit : limit);
+    return TVLM_OK;
+}
 int tvlm_getrf_f64(const tvlm_view* a, int32_t* ipiv, int32_t* info, void* stream) {
     return tvlm::getrf<double>(a, ipiv, info, stream);
 }

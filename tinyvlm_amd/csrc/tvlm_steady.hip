@@ -10,8 +10,15 @@
 //                        this is the reference's arithmetic, operation for operation.
 //   mode 1 (served)    : every request's FP64 influence matrix in the seeded form, copied to FP32,
 //                        one batched FP32 LU for all requests, then FP64 iterative refinement
-//                        x += A32^{-1} (b - A64 x) (kRefine steps, residual in FP64 against the FP64
-//                        matrix), so the answer carries the FP64 matrix's accuracy.
+//                        x += A32^{-1} (b - A64 x) (kRefine + 1 corrections, residual in FP64 against
+//                        the FP64 matrix), so the answer carries the FP64 matrix's accuracy.  The
+//                        returned x of every request is checked: ||b - A64 x||_inf <= sqrt(N) eps
+//                        ||x||_inf ||A64||_inf (the stopping test of LAPACK's dsgesv).  A request that
+//                        fails it gets more corrections (up to kMaxCorrections in all), then, if it
+//                        still fails, the FP64 LU of its FP64 matrix (the plain path's solve); a request
+//                        whose FP32 LU had a cooperative panel give up waiting (info -999) is first
+//                        re-factored with one-group panels.  tvlm_vlm_refinement reports what each
+//                        request took.
 // Per-surface coefficients are combined on the host (area-weighted CL, area*MAC-weighted CM,
 // vlm/src/vlm_backend.cpp:71-139).
 #include <cmath>
@@ -23,6 +30,9 @@
 namespace {
 constexpr int kMaxS = TVLM_MAX_SURFACES;
 constexpr int kRefine = 3;
+constexpr int kMaxCorrections = 8;  // corrections of one request before the FP64 fallback
+// tvlm_vlm_refinement status bits
+constexpr int32_t kStMoreCorrections = 1, kStFallback64 = 2, kStRefactored = 4, kStUnresolved = 8;
 constexpr int kResidRows = 64;     // rows per work-group of the residual
 constexpr int kResidWaves = 8;     // waves splitting its columns
 }  // namespace
@@ -42,6 +52,11 @@ struct tvlm_vlm_plan {
     double *lhs = nullptr, *rhs = nullptr, *x = nullptr, *res = nullptr, *tr = nullptr, *coef = nullptr;
     float *lhs32 = nullptr, *r32 = nullptr;
     int32_t *ipiv = nullptr, *info = nullptr;
+    double *rowabs = nullptr, *berr = nullptr;  // served mode: row sums of |A64|, backward error per request
+    int32_t* conv = nullptr;                     // served mode: 1 = the request's x passed the test
+    std::vector<int32_t> st_status, st_corr;     // per request of the last solve (tvlm_vlm_refinement)
+    std::vector<double> st_berr;
+    bool checked = false;                        // the last solve's requests were checked (and rescued)
     double* tr_host = nullptr;  // pinned: transforms of all requests and surfaces
     std::vector<double> alpha, beta, ref;  // per request; ref: CM reference point in the moved frame
     double u_inf = 1.0, rho = 1.0;
@@ -63,11 +78,15 @@ namespace {
 // column (512 contiguous bytes) and N / 8 work-groups' worth of waves keep enough loads in flight
 // to stream the FP64 matrix at HBM rate; x[j] is the same for the whole wave; the 8 partial sums
 // are added in LDS in a fixed order.
+// With ROWABS the same pass also stores the row sums of |A| (for ||A||_inf of the convergence test).
+template <bool ROWABS>
 __global__ __launch_bounds__(kResidRows * kResidWaves) void k_residual(const double* __restrict__ A, int n,
                                                                       const double* __restrict__ rhs,
                                                                       const double* __restrict__ x,
-                                                                      double* __restrict__ r) {
+                                                                      double* __restrict__ r,
+                                                                      double* __restrict__ rowabs) {
     __shared__ double part[kResidWaves][kResidRows];
+    __shared__ double apart[ROWABS ? kResidWaves : 1][kResidRows];
     const int64_t b = blockIdx.y;
     A += b * (int64_t)n * n;
     rhs += b * n;
@@ -75,19 +94,75 @@ __global__ __launch_bounds__(kResidRows * kResidWaves) void k_residual(const dou
     r += b * n;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = blockIdx.x * kResidRows + lane;
-    double acc = 0.0;
+    double acc = 0.0, aacc = 0.0;
     if (i < n) {
         const double* Ai = A + i;
 #pragma unroll 4
-        for (int j = w; j < n; j += kResidWaves) acc += Ai[(int64_t)j * n] * x[j];
+        for (int j = w; j < n; j += kResidWaves) {
+            const double aij = Ai[(int64_t)j * n];
+            acc += aij * x[j];
+            if constexpr (ROWABS) aacc += fabs(aij);
+        }
     }
     part[w][lane] = acc;
+    if constexpr (ROWABS) apart[w][lane] = aacc;
     __syncthreads();
     if (w == 0 && i < n) {
         double s = 0.0;
 #pragma unroll
         for (int q = 0; q < kResidWaves; q++) s += part[q][lane];
         r[i] = rhs[i] - s;
+        if constexpr (ROWABS) {
+            double sa = 0.0;
+#pragma unroll
+            for (int q = 0; q < kResidWaves; q++) sa += apart[q][lane];
+            rowabs[b * n + i] = sa;
+        }
+    }
+}
+
+// Convergence test of request blockIdx.x (LAPACK dsgesv's stopping test): with rnrm = ||r||_inf,
+// xnrm = ||x||_inf, anrm = ||A||_inf (max row sum), converged iff the FP32 LU reported no problem
+// (info == 0), every entry of r and x is finite and rnrm <= sqrt(n) eps xnrm anrm.  berr = rnrm /
+// (xnrm anrm), the normwise backward error.
+__global__ __launch_bounds__(256) void k_refine_check(const double* __restrict__ r, const double* __restrict__ x,
+                                                      const double* __restrict__ rowabs,
+                                                      const int32_t* __restrict__ info, int n,
+                                                      double* __restrict__ berr, int32_t* __restrict__ conv) {
+    __shared__ double sr[256], sx[256], sa[256];
+    __shared__ int sbad[256];
+    const int64_t b = blockIdx.x;
+    r += b * n;
+    x += b * n;
+    rowabs += b * n;
+    double mr = 0.0, mx = 0.0, ma = 0.0;
+    int bad = 0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const double ri = r[i], xi = x[i];
+        bad |= !isfinite(ri) || !isfinite(xi);
+        mr = fmax(mr, fabs(ri));
+        mx = fmax(mx, fabs(xi));
+        ma = fmax(ma, rowabs[i]);
+    }
+    sr[threadIdx.x] = mr;
+    sx[threadIdx.x] = mx;
+    sa[threadIdx.x] = ma;
+    sbad[threadIdx.x] = bad;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + h]);
+            sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + h]);
+            sa[threadIdx.x] = fmax(sa[threadIdx.x], sa[threadIdx.x + h]);
+            sbad[threadIdx.x] |= sbad[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double rn = sr[0], xa = sx[0] * sa[0];
+        const bool ok = info[b] == 0 && sbad[0] == 0 && rn <= sqrt((double)n) * 2.220446049250313e-16 * xa;
+        berr[b] = sbad[0] ? (double)NAN : (xa > 0.0 ? rn / xa : (rn == 0.0 ? 0.0 : (double)INFINITY));
+        conv[b] = ok ? 1 : 0;
     }
 }
 
@@ -191,6 +266,97 @@ int post(tvlm_vlm_plan* P, int b) {
     return TVLM_OK;
 }
 
+// Served mode, after the solve: reads the per-request test results; for every request that failed,
+// (1) if its FP32 LU stalled (info -999), re-factors its FP32 copy with one-group panels and restarts
+// its refinement from x = 0; (2) applies more corrections, one at a time with the test after each,
+// up to kMaxCorrections in all; (3) if it still fails, solves it with the FP64 LU of its FP64 matrix
+// (factored in place; the next solve assembles it again); then redoes its post-processing.  Runs
+// synchronously on the plan's stream; requests that passed are not touched.
+int rescue(tvlm_vlm_plan* P) {
+    const int B = P->active;
+    const int64_t N = P->N, NN = N * N;
+    hipStream_t st = P->stream;
+    std::vector<int32_t> conv(B), info(B);
+    std::vector<double> berr(B);
+    TVLM_HIP(hipMemcpyAsync(conv.data(), P->conv, sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
+    TVLM_HIP(hipMemcpyAsync(info.data(), P->info, sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
+    TVLM_HIP(hipMemcpyAsync(berr.data(), P->berr, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+    TVLM_HIP(hipStreamSynchronize(st));
+    for (int b = 0; b < B; b++) {
+        P->st_berr[b] = berr[b];
+        if (conv[b]) continue;
+        double* A64 = P->lhs + b * NN;
+        float* A32 = P->lhs32 + b * NN;
+        int32_t* ipiv = P->ipiv + b * N;
+        int32_t* inf = P->info + b;
+        double* x = P->x + b * N;
+        double* rhs = P->rhs + b * N;
+        double* res = P->res + b * N;
+        float* r32 = P->r32 + b * N;
+        tvlm_view L32 = {A32, {N, N, 1}, {1, N, NN}};
+        tvlm_view R32 = {r32, {N, 1, 1}, {1, N, N}};
+        const dim3 rg((unsigned)((N + kResidRows - 1) / kResidRows), 1);
+        int32_t status = 0, corr = P->st_corr[b];
+        if (info[b] == kGetrfStalled) {
+            hipLaunchKernelGGL(k_to_f32, dim3(grid_for(NN)), dim3(256), 0, st, A64, A32, NN);
+            TVLM_LAUNCHED();
+            TVLM_TRY(getrf_single_group_f32(&L32, ipiv, inf, st));
+            TVLM_TRY(tvlm_fill_f64(0, x, 1, 0.0, N, st));
+            TVLM_HIP(hipMemcpyAsync(res, rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            status |= kStRefactored;
+            corr = 0;
+        }
+        bool ok = false;
+        while (corr < kMaxCorrections && !ok) {
+            hipLaunchKernelGGL(k_to_f32, dim3(grid_for(N)), dim3(256), 0, st, res, r32, N);
+            TVLM_LAUNCHED();
+            TVLM_TRY(tvlm_getrs_f32(&L32, ipiv, &R32, st));
+            hipLaunchKernelGGL(k_add_f32, dim3(grid_for(N)), dim3(256), 0, st, x, r32, N);
+            TVLM_LAUNCHED();
+            corr++;
+            hipLaunchKernelGGL(k_residual<false>, rg, dim3(kResidRows * kResidWaves), 0, st, A64, (int)N, rhs, x, res,
+                               P->rowabs + b * N);
+            TVLM_LAUNCHED();
+            hipLaunchKernelGGL(k_refine_check, dim3(1), dim3(256), 0, st, res, x, P->rowabs + b * N, inf, (int)N,
+                               P->berr + b, P->conv + b);
+            TVLM_LAUNCHED();
+            int32_t c = 0;
+            TVLM_HIP(hipMemcpyAsync(&c, P->conv + b, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            TVLM_HIP(hipMemcpyAsync(&P->st_berr[b], P->berr + b, sizeof(double), hipMemcpyDeviceToHost, st));
+            TVLM_HIP(hipStreamSynchronize(st));
+            ok = c != 0;
+        }
+        if (ok) {
+            if (corr > kRefine + 1) status |= kStMoreCorrections;
+        } else {
+            // the plain path's solve of this request: FP64 LU of its FP64 matrix, in place
+            tvlm_view L64 = {A64, {N, N, 1}, {1, N, NN}};
+            TVLM_TRY(tvlm_getrf_f64(&L64, ipiv, inf, st));
+            int32_t i64 = 0;
+            TVLM_HIP(hipMemcpyAsync(&i64, inf, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            TVLM_HIP(hipStreamSynchronize(st));
+            if (i64 == kGetrfStalled) {
+                // its factors are invalid and the matrix is gone: assemble it again, one-group panels
+                TVLM_TRY(assemble(P, b, true));
+                TVLM_TRY(getrf_single_group_f64(&L64, ipiv, inf, st));
+                TVLM_HIP(hipMemcpyAsync(&i64, inf, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                TVLM_HIP(hipStreamSynchronize(st));
+                status |= kStRefactored;
+            }
+            TVLM_HIP(hipMemcpyAsync(x, rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            tvlm_view X = {x, {N, 1, 1}, {1, N, N}};
+            TVLM_TRY(tvlm_getrs_f64(&L64, ipiv, &X, st));
+            status |= kStFallback64;
+            if (i64 != 0) status |= kStUnresolved;
+            P->st_berr[b] = NAN;  // the FP64 matrix is factored in place: no residual against it
+        }
+        P->st_status[b] = status;
+        P->st_corr[b] = corr;
+        TVLM_TRY(post(P, b));
+    }
+    return TVLM_OK;
+}
+
 }  // namespace
 }  // namespace tvlm
 
@@ -252,6 +418,9 @@ int tvlm_vlm_create(tvlm_vlm_plan** plan, int32_t n_surf, const int64_t* ns, con
     if (rc == TVLM_OK && mode == 1) rc = P->alloc((void**)&P->lhs32, sizeof(float) * NN * nb);
     if (rc == TVLM_OK && mode == 1) rc = P->alloc((void**)&P->r32, sizeof(float) * N * batch);
     if (rc == TVLM_OK && mode == 1) rc = P->alloc((void**)&P->res, sizeof(double) * N * batch);
+    if (rc == TVLM_OK && mode == 1) rc = P->alloc((void**)&P->rowabs, sizeof(double) * N * batch);
+    if (rc == TVLM_OK && mode == 1) rc = P->alloc((void**)&P->berr, sizeof(double) * batch);
+    if (rc == TVLM_OK && mode == 1) rc = P->alloc((void**)&P->conv, sizeof(int32_t) * batch);
     if (rc == TVLM_OK) rc = P->alloc((void**)&P->rhs, sizeof(double) * N * batch);
     if (rc == TVLM_OK) rc = P->alloc((void**)&P->x, sizeof(double) * N * batch);
     if (rc == TVLM_OK) rc = P->alloc((void**)&P->tr, sizeof(double) * 16 * n_surf * batch);
@@ -341,13 +510,17 @@ int tvlm_vlm_solve_n(tvlm_vlm_plan* P, int32_t n_active, const double* alpha, co
         tvlm_view R32 = {P->r32, {N, 1, B}, {1, N, N}};
         // x = 0, r = b; then kRefine + 1 corrections x += A32^{-1} r, r = b - A64 x
         TVLM_TRY(tvlm_fill_f64(0, P->x, 1, 0.0, N * B, st));
+        const dim3 rg((unsigned)((N + kResidRows - 1) / kResidRows), B);
         for (int it = 0; it <= kRefine; it++) {
             if (it == 0) {
                 hipLaunchKernelGGL(k_to_f32, dim3(grid_for(N * B)), dim3(256), 0, st, P->rhs, P->r32, N * B);
             } else {
-                const dim3 rg((unsigned)((N + kResidRows - 1) / kResidRows), B);
-                hipLaunchKernelGGL(k_residual, rg, dim3(kResidRows * kResidWaves), 0, st, P->lhs, (int)N, P->rhs,
-                                   P->x, P->res);
+                if (it == 1)
+                    hipLaunchKernelGGL(k_residual<true>, rg, dim3(kResidRows * kResidWaves), 0, st, P->lhs, (int)N,
+                                       P->rhs, P->x, P->res, P->rowabs);
+                else
+                    hipLaunchKernelGGL(k_residual<false>, rg, dim3(kResidRows * kResidWaves), 0, st, P->lhs, (int)N,
+                                       P->rhs, P->x, P->res, P->rowabs);
                 TVLM_LAUNCHED();
                 hipLaunchKernelGGL(k_to_f32, dim3(grid_for(N * B)), dim3(256), 0, st, P->res, P->r32, N * B);
             }
@@ -356,10 +529,22 @@ int tvlm_vlm_solve_n(tvlm_vlm_plan* P, int32_t n_active, const double* alpha, co
             hipLaunchKernelGGL(k_add_f32, dim3(grid_for(N * B)), dim3(256), 0, st, P->x, P->r32, N * B);
             TVLM_LAUNCHED();
         }
+        // the returned x of every request: its residual and the convergence test (the rescue of the
+        // requests that fail it runs in tvlm_vlm_results)
+        hipLaunchKernelGGL(k_residual<false>, rg, dim3(kResidRows * kResidWaves), 0, st, P->lhs, (int)N, P->rhs, P->x,
+                           P->res, P->rowabs);
+        TVLM_LAUNCHED();
+        hipLaunchKernelGGL(k_refine_check, dim3(B), dim3(256), 0, st, P->res, P->x, P->rowabs, P->info, (int)N,
+                           P->berr, P->conv);
+        TVLM_LAUNCHED();
         for (int b = 0; b < B; b++) TVLM_TRY(post(P, b));
     }
     TVLM_HIP(hipEventRecord(P->done, st));
     P->solved = true;
+    P->checked = false;
+    P->st_status.assign(B, 0);
+    P->st_corr.assign(B, P->mode == 1 ? kRefine + 1 : 0);
+    P->st_berr.assign(B, 0.0);
     return TVLM_OK;
 }
 
@@ -370,6 +555,10 @@ int tvlm_vlm_results(tvlm_vlm_plan* P, double* cl, double* cm, double* gamma, in
     TVLM_REQUIRE(P && P->solved, "no solve has run on this plan");
     const int n = P->n_surf, B = P->active;
     hipStream_t st = P->stream;
+    if (P->mode == 1 && !P->checked) {
+        TVLM_TRY(rescue(P));
+        P->checked = true;
+    }
     std::vector<double> coef((size_t)8 * n * B);
     TVLM_HIP(hipMemcpyAsync(coef.data(), P->coef, sizeof(double) * coef.size(), hipMemcpyDeviceToHost, st));
     if (gamma) TVLM_HIP(hipMemcpyAsync(gamma, P->x, sizeof(double) * P->N * B, hipMemcpyDeviceToHost, st));
@@ -390,6 +579,22 @@ int tvlm_vlm_results(tvlm_vlm_plan* P, double* cl, double* cm, double* gamma, in
         if (cl) cl[b] = cl_sum / area_sum;
         if (cm)
             for (int k = 0; k < 3; k++) cm[3 * b + k] = cm_sum[k] / (area_sum * mac_sum);
+    }
+    return TVLM_OK;
+}
+
+// What the served refinement took for each request of the last solve (after tvlm_vlm_results).
+int tvlm_vlm_refinement(tvlm_vlm_plan* P, int32_t* status, int32_t* corrections, double* backward_error) {
+    TVLM_REQUIRE(P && P->solved, "no solve has run on this plan");
+    if (P->mode == 1 && !P->checked) {
+        TVLM_TRY(rescue(P));
+        P->checked = true;
+    }
+    const int B = P->active;
+    for (int b = 0; b < B; b++) {
+        if (status) status[b] = P->st_status[b];
+        if (corrections) corrections[b] = P->st_corr[b];
+        if (backward_error) backward_error[b] = P->st_berr[b];
     }
     return TVLM_OK;
 }

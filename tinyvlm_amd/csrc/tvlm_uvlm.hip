@@ -799,6 +799,9 @@ int tvlm_uvlm_run(tvlm_uvlm_plan* P, int32_t use_graph) {
         return TVLM_OK;
     }
     TVLM_REQUIRE(P->stream != nullptr, "hipGraph capture needs a non-default stream");
+    // the plain loop's per-step getrs uses the library's per-stream scratch, which a later, larger
+    // solve on the same stream may reallocate under a captured graph: plain mode is never captured
+    TVLM_REQUIRE(P->mode == 1, "use_graph needs mode 1 (plain mode runs call by call)");
     if (!P->exec) {
         TVLM_HIP(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
         int rc = TVLM_OK;

@@ -147,6 +147,7 @@ class Request:
     info: int = 0
     cl_t: np.ndarray | None = None
     cm_t: np.ndarray | None = None
+    refine: int = 0  # served refinement status bits (SteadyPlan.refinement)
 
     @property
     def latency(self) -> float:
@@ -192,6 +193,10 @@ class ServeStats:
     peak_bytes: int = 0
     max_running: int = 0  # most lanes busy at once
     batch_sizes: list = field(default_factory=list)
+    more_corrections: int = 0  # steady requests whose refinement needed more than the fixed corrections
+    fallbacks: int = 0  # steady requests solved by the FP64 LU after the refinement failed
+    refactored: int = 0  # steady requests re-factored after a stalled cooperative LU panel
+    unresolved: int = 0  # steady requests whose FP64 matrix is singular
 
 
 class Server:
@@ -227,19 +232,28 @@ class Server:
         n, b = self.N[key], self.cap[key]
         return 12 * n * n * b + 400 * n * b
 
+    def _uestimate(self, key: str) -> int:
+        t_steps, _ = self.unsteady[key]
+        n = self.N[key]
+        return 16 * n * n + 8 * n * t_steps + 256 * n + (1 << 20)  # LHS, inverse, rhs of all steps, wake
+
     def _evict_for(self, need: int, now: float) -> bool:
-        idle = sorted((s.last_used, k, i) for k, ss in self.slots.items() for i, s in enumerate(ss) if not s.busy)
+        """Frees the least recently used idle plans (steady and unsteady) until `need` bytes fit the
+        budget; False (nothing evicted) when even all idle plans would not make room."""
+        idle = [(s.last_used, 0, k, i) for k, ss in self.slots.items() for i, s in enumerate(ss) if not s.busy]
+        idle += [(s.last_used, 1, k, 0) for k, s in self.uslots.items() if not s.busy]
+        idle.sort(key=lambda v: (v[0], v[1], str(v[2]), v[3]))
         victims = []
         free = self.budget - self.used_bytes
-        for _, k, i in idle:
+        for _, unst, k, i in idle:
             if free >= need:
                 break
-            victims.append((k, i))
-            free += self.slots[k][i].bytes
+            victims.append((unst, k, i))
+            free += (self.uslots[k] if unst else self.slots[k][i]).bytes
         if free < need:
             return False
-        for k, i in sorted(victims, key=lambda v: -v[1]):
-            s = self.slots[k].pop(i)
+        for unst, k, i in sorted(victims, key=lambda v: -v[2]):
+            s = self.uslots.pop(k) if unst else self.slots[k].pop(i)
             s.plan.close()
             self.used_bytes -= s.bytes
             self.stats.plans_evicted += 1
@@ -269,8 +283,9 @@ class Server:
         if s is not None:
             return None if s.busy else s
         t_steps, dt = self.unsteady[key]
-        n = self.N[key]
-        need = 16 * n * n + 8 * n * t_steps + 256 * n + (1 << 20)  # LHS, inverse, rhs of all steps, wake
+        need = self._uestimate(key)
+        if need > self.budget:
+            raise RuntimeError(f"mesh {key}: one unsteady plan ({need} bytes) exceeds the memory budget {self.budget}")
         if self.used_bytes + need > self.budget and not self._evict_for(need, now):
             return None
         plan = UVLMPlan(self.meshes[key], t_steps, dt, mode=MODE_FAST, stream=self.lanes[lane])
@@ -328,12 +343,18 @@ class Server:
                     progressed = True
                     continue
                 res = slot.plan.results(gamma=self.keep_gamma)
+                ref = slot.plan.refinement()["status"]
                 t = time.perf_counter() - t0
                 for j, r in enumerate(batch):
                     r.finished = t
                     r.cl = float(res["cl"][j])
                     r.cm = res["cm"][j].copy()
                     r.info = int(res["info"][j])
+                    r.refine = int(ref[j])
+                    st.more_corrections += bool(r.refine & 1)
+                    st.fallbacks += bool(r.refine & 2)
+                    st.refactored += bool(r.refine & 4)
+                    st.unresolved += bool(r.refine & 8)
                     if self.keep_gamma:
                         r.gamma = res["gamma"][j].copy()
                 slot.busy = False
@@ -387,6 +408,12 @@ class Server:
                 st.max_running = max(st.max_running, len(running))
                 progressed = True
             if not progressed:
+                if not running and any(waiting.values()):
+                    # nothing runs, so nothing will free a lane or memory: a waiting head that cannot
+                    # be placed now never will be
+                    heads = sorted(k for k, q in waiting.items() if q)
+                    raise RuntimeError(f"serving stalled: no plan can be placed for the waiting queues {heads} "
+                                       f"(budget {self.budget} bytes, {self.used_bytes} held by idle plans)")
                 if not running and nxt < len(order) and not any(waiting.values()):
                     time.sleep(max(0.0, order[nxt].arrival - (time.perf_counter() - t0)))
                 else:

@@ -81,6 +81,19 @@ class SteadyPlan:
             out["gamma"] = g
         return out
 
+    def refinement(self):
+        """What the served refinement took for each request of the last solve (checks and rescues
+        them first if results() has not): dict status[n] (bits: 1 more corrections, 2 FP64
+        fallback, 4 re-factored after a stalled cooperative panel, 8 unresolved), corrections[n],
+        backward_error[n]."""
+        n = self.active
+        st = np.zeros(n, dtype=np.int32)
+        co = np.zeros(n, dtype=np.int32)
+        be = np.zeros(n)
+        H.check(H.lib().tvlm_vlm_refinement(self._plan, ctypes.c_void_p(st.ctypes.data), ctypes.c_void_p(co.ctypes.data),
+                                            ctypes.c_void_p(be.ctypes.data)), "tvlm_vlm_refinement")
+        return {"status": st, "corrections": co, "backward_error": be}
+
     def done(self) -> bool:
         """True once the plan's latest solve has finished on the device (does not block)."""
         d = ctypes.c_int32()

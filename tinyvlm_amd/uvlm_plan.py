@@ -72,7 +72,11 @@ class UVLMPlan:
         H.check(H.lib().tvlm_uvlm_set_motion(self._plan, *[ctypes.c_void_p(a.ctypes.data) for a in arrs]),
                 "tvlm_uvlm_set_motion")
 
-    def run(self, use_graph=True):
+    def run(self, use_graph=None):
+        """One whole simulation, asynchronous.  use_graph (default: the served mode's hipGraph replay;
+        plain mode always runs call by call)."""
+        if use_graph is None:
+            use_graph = self.mode == MODE_FAST
         H.check(H.lib().tvlm_uvlm_run(self._plan, ctypes.c_int32(int(use_graph))), "tvlm_uvlm_run")
 
     def results(self):
