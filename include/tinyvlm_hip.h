@@ -240,6 +240,44 @@ int tvlm_uvlm_destroy(tvlm_uvlm_plan* plan);
  * in the plan. */
 int tvlm_uvlm_time_wake(tvlm_uvlm_plan* plan, double* total_ms, int32_t* launches, double* pairs);
 
+/* ---- coupled pitch/heave flutter sweep ------------------------------------------------------
+ * The coupled aeroelastic driver UVLM_2DOF (tests/uvlm_2dof.cpp:428-865, COUPLED path, with the
+ * Newmark-beta integrator of vlm/src/vlm_integrator.cpp:26-68) for n_airspeeds reduced velocities of
+ * one lifting wing at once, resident on the device.  Per airspeed and time step: the reference's
+ * fixed-point iterations on the structural increment (Newmark step, pose, panel metrics, wake shed,
+ * kinematic velocities, rhs, wake induction, solve, circulation post-processing, unsteady forces,
+ * CL/CM about the elastic axis, loads) until |du_k - du| <= EPS_f or 51 iterations.  The influence
+ * matrix is assembled and factored once at the initial pose (rigid motion) and inverted once; the
+ * wake is a ring buffer of the newest wake_rows ring rows (older rings are dropped, as if their
+ * circulation were zero).  FP64 throughout; wake induction in the FP64 seeded form (hardware
+ * 1/sqrt and 1/x estimates refined by one Newton step).  No counterpart in the reference beyond the
+ * single-airspeed driver it batches. */
+typedef struct tvlm_flutter_params {
+    double b;           /* half chord (the mesh is divided by it: nondimensionalize_verts) */
+    double rho, a_h, omega, zeta_a, zeta_h, x_a, mu, r_a; /* UVLM_2DOF_Vars (tests/uvlm_2dof.cpp:23-39) */
+    double beta[4];     /* torsional spring f(alpha) = beta0 + beta1 a + beta2 a^2 + beta3 a^3 (:285) */
+    double init_pitch;  /* initial pitch, radians (the reference: to_radians(3)) */
+} tvlm_flutter_params;
+typedef struct tvlm_flutter_plan tvlm_flutter_plan;
+/* verts_init: host {ns+1, nc+1, 4} column-major double (one surface, as Simulation reads it: FP32
+ * parse + quarter chord); U_a: host [n_airspeeds] reduced velocities; the time step is the
+ * reference's (last chordwise panel of the dimensionless mesh).  Sets up everything (state at t = 0,
+ * LU and inverse of the influence matrix) and synchronises. */
+int tvlm_flutter_create(tvlm_flutter_plan** plan, int64_t ns, int64_t nc, const double* verts_init,
+                        const tvlm_flutter_params* params, int32_t n_airspeeds, const double* U_a,
+                        int32_t wake_rows, int64_t max_steps, void* stream);
+/* advances every airspeed by `steps` coupled time steps (synchronous: the host reads the per-airspeed
+ * convergence flags after each iteration) */
+int tvlm_flutter_run(tvlm_flutter_plan* plan, int64_t steps);
+/* steps done so far, the time step, states [steps_done + 1][n_airspeeds][8] (h, alpha, h', alpha',
+ * h'', alpha'', F_h, F_alpha) and iterations [steps_done][n_airspeeds]; null pointers are skipped */
+int tvlm_flutter_results(tvlm_flutter_plan* plan, int64_t* steps_done, double* dt, double* states, int32_t* iters);
+/* measurement: the wake-induction launch of the next step (every airspeed), replayed `launches`
+ * times between two HIP events on the plan's stream; total ms and (receiver, wake ring) pairs per
+ * launch */
+int tvlm_flutter_time_wake(tvlm_flutter_plan* plan, int32_t launches, double* total_ms, double* pairs);
+int tvlm_flutter_destroy(tvlm_flutter_plan* plan);
+
 /* ---- plot3d meshes (MeshIO, vlm/src/vlm_mesh.cpp:34-124); host only, no device needed ----------
  * dims: ns spanwise and nc chordwise panels.  read: verts {ns+1, nc+1, 4} column-major double, w = 1,
  * parsed in FP32 (parse_f64 = 0: the VLM/UVLM drivers' buffers) or FP64 (the HB driver's) and, if

@@ -39,6 +39,8 @@ class Vars:
     mu: float = 100.0
     r_a: float = 0.5
     U_a: float = 7.0
+    beta: tuple = (0.0, 0.1, 0.0, 40.0)  # torsional_func's default coefficients (tests/uvlm_2dof.cpp:285)
+    init_pitch_deg: float = 3.0  # to_radians(3.f) initial pitch (tests/uvlm_2dof.cpp:512)
 
 
 def torsional_func(alpha, beta0=0.0, beta1=0.1, beta2=0.0, beta3=40.0):
@@ -97,19 +99,26 @@ class NewmarkBeta:
         return du.astype(T), dv.astype(T), da.astype(T)
 
 
-def run(meshes, vars: Vars, t_final_nd: float, dtype=np.float32, max_steps=None):
-    """UVLM_2DOF::run (tests/uvlm_2dof.cpp:428-865, COUPLED).  meshes: [(path, lifting), ...].
-    Returns dict t, u (2, n), v, a, F, iters (per step) with n = steps + 1 states."""
+def run(meshes, vars: Vars, t_final_nd: float, dtype=np.float32, max_steps=None, wake_rows=None):
+    """UVLM_2DOF::run (tests/uvlm_2dof.cpp:428-865, COUPLED).  meshes: [(path or verts array, lifting),
+    ...] (an array: the mesh as Simulation reads it, FP32 parse + quarter chord).  wake_rows: keep only
+    the newest wake_rows ring rows in the wake induction (truncation: the older rings contribute
+    nothing, as the reference's FP32 kernel does past 5000 rows, vlm_backend_cpu_kernels.ispc:157-162);
+    None keeps the whole wake.  Returns dict t, u (2, n), v, a, F, iters (per step) with n = steps + 1
+    states."""
     T = dtype
     b = T(vars.b)
     surfs = []
     for path, lifting in meshes:
-        v = C.read_mesh(path, dtype)  # FP32 parse + quarter chord (Simulation, vlm.cpp:17-45)
+        if isinstance(path, np.ndarray):
+            v = np.asfortranarray(path.astype(dtype))
+        else:
+            v = C.read_mesh(path, dtype)  # FP32 parse + quarter chord (Simulation, vlm.cpp:17-45)
         v[:, :, :3] = (v[:, :, :3] / b).astype(T)
         surfs.append(C.Surface(verts=np.asfortranarray(v), lifting=lifting))
     verts_init = [s.verts.copy(order="F") for s in surfs]
     a_h = float(T(vars.a_h))
-    init_angle = float(T(3.0) * PI_F / T(180.0))
+    init_angle = float(T(vars.init_pitch_deg) * PI_F / T(180.0))
     T0, _ = _rot_y(1.0 + a_h, init_angle, 0.0)
     for s, vi in zip(surfs, verts_init):
         s.verts = C.displace_wing(T0.astype(T), vi)
@@ -136,7 +145,8 @@ def run(meshes, vars: Vars, t_final_nd: float, dtype=np.float32, max_steps=None)
     u[1, 0] = T(init_angle)
     kh = T((vars.omega / vars.U_a) ** 2)
     ka = T(1.0 / (vars.U_a * vars.U_a))
-    a0 = np.array([-kh * u[0, 0], -ka * T(torsional_func(u[1, 0]))], dtype=T)
+    tf = lambda al: torsional_func(al, *vars.beta)  # noqa: E731
+    a0 = np.array([-kh * u[0, 0], -ka * T(tf(u[1, 0]))], dtype=T)
     a0 = a0 - Cm @ v[:, 0] - K @ u[:, 0]
     a[:, 0] = np.linalg.solve(M.astype(np.float64), a0.astype(np.float64)).astype(T)
     # aerodynamics at t = 0 (tests/uvlm_2dof.cpp:547-559)
@@ -158,7 +168,16 @@ def run(meshes, vars: Vars, t_final_nd: float, dtype=np.float32, max_steps=None)
     def solve_gamma(vel, it):
         rhs = np.zeros(sum(s.ns * s.nc for s in surfs), dtype=T)
         C.rhs_assemble_velocities(rhs, surfs, vel)
-        C.rhs_assemble_wake_influence(rhs, surfs, it)
+        if wake_rows is None or it <= wake_rows:
+            C.rhs_assemble_wake_influence(rhs, surfs, it)
+        else:
+            # the newest wake_rows ring rows: vertex rows [nwv-1-it, nwv-1-it+R] and circulation
+            # columns [nwv-1-it, nwv-1-it+R) as a wake of R+1 rows, all of whose R rings induce
+            r0 = s0 = surfs[0].verts_wake.shape[1] - 1 - it
+            sub = [C.Surface(verts=s.verts, verts_wake=np.asfortranarray(s.verts_wake[:, r0: r0 + wake_rows + 1]),
+                             gamma_wake=np.asfortranarray(s.gamma_wake[:, s0: s0 + wake_rows]), colloc=s.colloc,
+                             normals=s.normals, areas=s.areas, lifting=s.lifting) for s in surfs]
+            C.rhs_assemble_wake_influence(rhs, sub, wake_rows)
         return C.getrs(lu, ipiv, rhs)
 
     def update_gamma(g, it, gamma_prev=None):
@@ -226,7 +245,7 @@ def run(meshes, vars: Vars, t_final_nd: float, dtype=np.float32, max_steps=None)
             F[0, i + 1] = T(-cl / (float(PI_F) * vars.mu))
             F[1, i + 1] = T(2.0 * cmy / (float(PI_F) * vars.mu * vars.r_a ** 2))
             dF = np.array([F[0, i + 1] - F[0, i] - kh * du[0],
-                           F[1, i + 1] - F[1, i] - ka * T(torsional_func(u[1, i + 1]) - torsional_func(u[1, i]))],
+                           F[1, i + 1] - F[1, i] - ka * T(tf(u[1, i + 1]) - tf(u[1, i]))],
                           dtype=T)
             du_k = du_k - du
             it += 1

@@ -86,3 +86,53 @@ def test_product_rejects_unknown_backend():
 
     with pytest.raises(RuntimeError, match="Unsupported backend"):
         UVLM2DOF("cpu", MESHES)
+
+
+def test_wake_truncation_is_zeroed_old_rings(oracle):
+    """Truncating the wake to its newest R ring rows (a wake view of those rows: vertex rows
+    [nwv-1-it, nwv-1-it+R], circulation columns [nwv-1-it, nwv-1-it+R), all R rings inducing) is the
+    full wake with the older rings' circulation set to zero, bit for bit: a zero-strength ring adds
+    exactly 0 to each receiver's sum.  (Row nwv-1-s holds the vertices shed at step s: the newest
+    rings have the smallest row indices.)"""
+    rng = np.random.default_rng(4)
+    ns, nc, rows = 7, 3, 30
+    v = oracle.rectangular_wing(ns, nc, half_span=2.0)
+    wake = np.zeros((ns + 1, rows + 1, 4), order="F")
+    for r in range(rows + 1):
+        wake[:, r] = v[:, -1]
+        wake[:, r, 0] += 0.13 * (rows - r) + 0.01 * rng.standard_normal(ns + 1)
+        wake[:, r, 2] += 0.04 * math.sin(0.4 * r)
+    gw = np.asfortranarray(rng.standard_normal((ns, rows)))
+    s = oracle.Surface(verts=v, verts_wake=np.asfortranarray(wake), gamma_wake=gw)
+    oracle.mesh_metrics(0.0, [s])
+    nwv = rows + 1
+    for it, R in ((25, 10), (30, 1), (12, 11)):
+        r0 = nwv - 1 - it
+        sub = oracle.Surface(verts=v, verts_wake=np.asfortranarray(wake[:, r0: r0 + R + 1]),
+                             gamma_wake=np.asfortranarray(gw[:, r0: r0 + R]), colloc=s.colloc, normals=s.normals,
+                             areas=s.areas)
+        trunc = np.zeros(ns * nc)
+        oracle.rhs_assemble_wake_influence(trunc, [sub], R)
+        zeroed = oracle.Surface(verts=v, verts_wake=s.verts_wake, gamma_wake=gw.copy(order="F"), colloc=s.colloc,
+                                normals=s.normals, areas=s.areas)
+        # ring rows used by iteration it: [nwv - 1 - it, nwv - 2]; the newest R of them (the first R) stay
+        zeroed.gamma_wake[:, r0 + R: nwv - 1] = 0.0
+        full = np.zeros(ns * nc)
+        oracle.rhs_assemble_wake_influence(full, [zeroed], it)
+        assert np.array_equal(trunc, full), (it, R)
+
+
+def test_oracle_two_dof_truncation_limits(o2, oracle):
+    """A ring of R rows changes nothing while the wake is at most R rows long: step i induces with i + 1
+    ring rows, so the first R states are bitwise the full wake's and the first load that differs is
+    that of step R (state R + 1); a ring as long as the run changes nothing at all."""
+    v = oracle.rectangular_wing(6, 4, chord=1.0, half_span=3.0)
+    vars = o2.Vars(mu=20.0, omega=0.5, a_h=-0.3, U_a=0.9)
+    full = o2.run([(v, True)], vars, t_final_nd=40 * 0.5 + 0.2, dtype=np.float64)
+    n = full["F"].shape[1] - 1
+    same = o2.run([(v, True)], vars, t_final_nd=40 * 0.5 + 0.2, dtype=np.float64, wake_rows=n + 5)
+    assert np.array_equal(full["F"], same["F"]) and np.array_equal(full["u"], same["u"])
+    for R in (5, 12):
+        tr = o2.run([(v, True)], vars, t_final_nd=40 * 0.5 + 0.2, dtype=np.float64, wake_rows=R)
+        differ = np.flatnonzero(np.any(tr["F"] != full["F"], axis=0))
+        assert differ.size and differ[0] == R + 1, (R, differ[:3])

@@ -322,6 +322,51 @@ __device__ __forceinline__ void lat_pair(double& ax, double& ay, double& az, con
 
 constexpr int kLatChunkMax = 16;  // spans per wave
 
+// The lattice walk of one wave over its span chunk [sa, sb) of a staged tile (vertex rows 0..rhe,
+// vw = swe + 1 vertex columns, 3 coordinates each in sverts; circulations rhe x swe in sgam) for
+// the receiver (cx, cy, cz): chordwise segments of each vertex column with the circulation
+// difference of the two ring columns beside it, spanwise segments of each vertex row with the
+// difference of the two ring rows, the previous vertex column's terms kept in registers; the tile's
+// right edge only when sb == swe.  Accumulates the induced velocity into (ax, ay, az).
+template <int RH>
+__device__ __forceinline__ void lattice_chunk(const double* sverts, const double* sgam, int swe, int rhe, int vw,
+                                              int sa, int sb, double cx, double cy, double cz, double& ax, double& ay,
+                                              double& az) {
+    LatVertex prev[RH + 1], cur[RH + 1];
+#pragma unroll
+    for (int q = 0; q <= RH; q++)
+        if (q <= rhe) prev[q] = lat_vertex(cx, cy, cz, sverts + 3 * (q * vw + sa));
+    for (int s = sa; s < sb; s++) {
+        // chordwise segments of vertex column s (left edge of ring column s)
+#pragma unroll
+        for (int q = 0; q < RH; q++) {
+            if (q < rhe) {
+                double g = -sgam[q * swe + s];
+                if (s > 0) g += sgam[q * swe + s - 1];
+                lat_pair(ax, ay, az, prev[q], prev[q + 1], g);
+            }
+        }
+        // vertex column s+1 and the spanwise segments of ring column s
+#pragma unroll
+        for (int q = 0; q <= RH; q++) {
+            if (q <= rhe) {
+                cur[q] = lat_vertex(cx, cy, cz, sverts + 3 * (q * vw + s + 1));
+                double g = 0.0;
+                if (q < rhe) g += sgam[q * swe + s];
+                if (q > 0) g -= sgam[(q - 1) * swe + s];
+                lat_pair(ax, ay, az, prev[q], cur[q], g);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q <= RH; q++) prev[q] = cur[q];
+    }
+    if (sb == swe) {  // right edge of the work-group tile: chordwise segments of vertex column swe
+#pragma unroll
+        for (int q = 0; q < RH; q++)
+            if (q < rhe) lat_pair(ax, ay, az, prev[q], prev[q + 1], sgam[q * swe + swe - 1]);
+    }
+}
+
 template <typename TS, int RH>
 __global__ __launch_bounds__(kThreads) void k_wake_lattice(TS* __restrict__ partial, RecvTable<TS> rt,
                                                            WakeTable<TS> wt) {
@@ -363,39 +408,7 @@ __global__ __launch_bounds__(kThreads) void k_wake_lattice(TS* __restrict__ part
         const int li = p - rt.offset[si];
         const double cx = rc.colloc[li], cy = rc.colloc[li + rc.cs2], cz = rc.colloc[li + 2 * rc.cs2];
         double ax = 0.0, ay = 0.0, az = 0.0;
-        LatVertex prev[RH + 1], cur[RH + 1];
-#pragma unroll
-        for (int q = 0; q <= RH; q++)
-            if (q <= rhe) prev[q] = lat_vertex(cx, cy, cz, sverts + 3 * (q * vw + sa));
-        for (int s = sa; s < sb; s++) {
-            // chordwise segments of vertex column s (left edge of ring column s)
-#pragma unroll
-            for (int q = 0; q < RH; q++) {
-                if (q < rhe) {
-                    double g = -sgam[q * swe + s];
-                    if (s > 0) g += sgam[q * swe + s - 1];
-                    lat_pair(ax, ay, az, prev[q], prev[q + 1], g);
-                }
-            }
-            // vertex column s+1 and the spanwise segments of ring column s
-#pragma unroll
-            for (int q = 0; q <= RH; q++) {
-                if (q <= rhe) {
-                    cur[q] = lat_vertex(cx, cy, cz, sverts + 3 * (q * vw + s + 1));
-                    double g = 0.0;
-                    if (q < rhe) g += sgam[q * swe + s];
-                    if (q > 0) g -= sgam[(q - 1) * swe + s];
-                    lat_pair(ax, ay, az, prev[q], cur[q], g);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q <= RH; q++) prev[q] = cur[q];
-        }
-        if (sb == swe) {  // right edge of the work-group tile: chordwise segments of vertex column swe
-#pragma unroll
-            for (int q = 0; q < RH; q++)
-                if (q < rhe) lat_pair(ax, ay, az, prev[q], prev[q + 1], sgam[q * swe + swe - 1]);
-        }
+        lattice_chunk<RH>(sverts, sgam, swe, rhe, vw, sa, sb, cx, cy, cz, ax, ay, az);
         part = ax * (double)rc.normals[li] + ay * (double)rc.normals[li + rc.ns2] + az * (double)rc.normals[li + 2 * rc.ns2];
     }
     sred[wave][lane] = part;

@@ -40,6 +40,8 @@ class Vars:
     mu: float = 100.0       # mass ratio
     r_a: float = 0.5        # radius of gyration about the elastic axis
     U_a: float = 7.0        # reduced velocity
+    beta: tuple = (0.0, 0.1, 0.0, 40.0)  # torsional_func's coefficients (tests/uvlm_2dof.cpp:285)
+    init_pitch_deg: float = 3.0  # initial pitch (tests/uvlm_2dof.cpp:512)
 
 
 def torsional_func(alpha, beta0=0.0, beta1=0.1, beta2=0.0, beta3=40.0):
@@ -105,6 +107,7 @@ class UVLM2DOF:
     Simulation (FP32 parse, quarter-chord shift, vlm/src/vlm.cpp:17-45)."""
 
     def __init__(self, backend_name: str, meshes, dtype=np.float32):
+        """meshes: [(plot3d path or verts array as read_plot3d returns it, lifting), ...]."""
         if backend_name != "hip":
             raise RuntimeError("Unsupported backend: " + str(backend_name))
         import torch
@@ -114,13 +117,15 @@ class UVLM2DOF:
         self.T = np.dtype(dtype).type
         self.tdt = torch.float32 if self.T == np.float32 else torch.float64
         self.lifting = [bool(l) for _, l in meshes]
-        self.verts_file = [read_plot3d(p, quarter_chord=True, parse_f64=False) for p, _ in meshes]
+        self.verts_file = [np.asarray(p, dtype=np.float64) if isinstance(p, np.ndarray)
+                           else read_plot3d(p, quarter_chord=True, parse_f64=False) for p, _ in meshes]
         self.dims = [(v.shape[0] - 1, v.shape[1] - 1) for v in self.verts_file]
         self.N = sum(ns * nc for ns, nc in self.dims)
 
-    def run(self, vars: Vars, t_final_nd: float, max_steps: int | None = None):
+    def run(self, vars: Vars, t_final_nd: float, max_steps: int | None = None, wake_rows: int | None = None):
         """UVLM_2DOF::run (tests/uvlm_2dof.cpp:428-865).  Returns dict t, u (2, n), v, a, F (loads
-        [-CL/(pi mu), 2 CM_y/(pi mu r_a^2)]), iters (Newton iterations per step)."""
+        [-CL/(pi mu), 2 CM_y/(pi mu r_a^2)]), iters (Newton iterations per step).  wake_rows: only the
+        newest wake_rows ring rows induce (truncated wake; None: the whole wake)."""
         torch, T, tdt = self.torch, self.T, self.tdt
         z = lambda *s: H.colmajor(s, tdt)  # noqa: E731
         b = T(vars.b)
@@ -142,7 +147,7 @@ class UVLM2DOF:
         gprev = [z(ns, nc) for ns, nc in dims]
         scal = torch.zeros(6 * len(dims), dtype=tdt, device="cuda")
         a_h = float(T(vars.a_h))
-        init_angle = float(T(3.0) * PI_F / T(180.0))  # to_radians(3.f)
+        init_angle = float(T(vars.init_pitch_deg) * PI_F / T(180.0))  # to_radians(3.f)
         T0, _ = _rotation_y(1.0 + a_h, init_angle, 0.0)
         T0d = H.to_device(np.asfortranarray(T0), tdt)
         H.displace_wing([T0d] * len(dims), verts, verts_init)
@@ -170,7 +175,8 @@ class UVLM2DOF:
         u[1, 0] = T(init_angle)
         kh = T((vars.omega / vars.U_a) ** 2)
         ka = T(1.0 / (vars.U_a * vars.U_a))
-        a0 = np.array([-kh * u[0, 0], -ka * T(torsional_func(u[1, 0]))], dtype=T) - Cm @ v[:, 0] - K @ u[:, 0]
+        tf = lambda al: torsional_func(al, *vars.beta)  # noqa: E731
+        a0 = np.array([-kh * u[0, 0], -ka * T(tf(u[1, 0]))], dtype=T) - Cm @ v[:, 0] - K @ u[:, 0]
         acc[:, 0] = np.linalg.solve(M.astype(np.float64), a0.astype(np.float64)).astype(T)
         # rigid motion of all surfaces together leaves the influence matrix unchanged: factor once
         lhs = z(N, N)
@@ -189,7 +195,13 @@ class UVLM2DOF:
             H.kinematic_velocities([dT] * len(dims), colloc_h, vel)
             rhs.zero_()
             H.rhs_assemble_velocities(rhs, normals, vel)
-            H.rhs_assemble_wake_influence(rhs, gamma_wake, colloc, normals, wake, self.lifting, it)
+            if wake_rows is None or it <= wake_rows:
+                H.rhs_assemble_wake_influence(rhs, gamma_wake, colloc, normals, wake, self.lifting, it)
+            else:
+                # the newest wake_rows ring rows, as views of R+1 vertex rows and R circulation columns
+                r0 = t_steps - 1 - it
+                H.rhs_assemble_wake_influence(rhs, [g[:, r0: r0 + wake_rows] for g in gamma_wake], colloc, normals,
+                                              [w[:, r0: r0 + wake_rows + 1] for w in wake], self.lifting, wake_rows)
             H.getrs(lhs, ipiv, rhs)
 
         def gamma_post(it, with_dt):
@@ -246,7 +258,7 @@ class UVLM2DOF:
                 F[0, i + 1] = T(-cl / (float(PI_F) * vars.mu))
                 F[1, i + 1] = T(2.0 * cmy / (float(PI_F) * vars.mu * vars.r_a ** 2))
                 dF = np.array([F[0, i + 1] - F[0, i] - kh * du[0],
-                               F[1, i + 1] - F[1, i] - ka * T(torsional_func(u[1, i + 1]) - torsional_func(u[1, i]))],
+                               F[1, i + 1] - F[1, i] - ka * T(tf(u[1, i + 1]) - tf(u[1, i]))],
                               dtype=T)
                 du_k = du_k - du
                 it += 1
