@@ -531,6 +531,121 @@ def two_dof_bench(steps, with_cpu, large=False):
     return out
 
 
+FLUTTER_PARAMS = dict(mu=20.0, omega=0.5, a_h=-0.3, x_a=0.25, r_a=0.5)  # EA at 0.35 c
+
+
+def flutter_bench(n_airspeeds, steps, window, plain_steps, with_cpu):
+    """The flutter sweep: rectangular wing 80 chordwise x 100 spanwise (chord 1, half span 5: 8000
+    panels), wake ring of 200 rows (20000 rings once full), mass ratio 20, heave/pitch frequency ratio
+    0.5, elastic axis 0.35 c, x_a 0.25, r_a 0.5, cubic pitch spring of the reference, 3-degree initial
+    pitch; n_airspeeds reduced velocities over [0.6, 1.2] (onset between 0.7 and 1.0) as one resident
+    batch, `steps` coupled time steps each (every step: the fixed-point iterations of the reference
+    driver).  Each step is timed on its own (the host reads the convergence flags after each
+    iteration).  value = airspeed-steps / s over the whole sweep; `full_ring` = the same over the steps
+    at the full 200-row ring; plain FP64 = the drop-in driver (tinyvlm_amd.two_dof over the Backend
+    entry points, exact Biot-Savart, LU substitution) on the first airspeed, per-step timed, compared
+    over the same full-ring steps; accuracy = |dCL| of that airspeed against it."""
+    from tinyvlm_amd import flutter_plan as FP
+    from tinyvlm_amd.two_dof import UVLM2DOF, Vars
+
+    R = 200
+    v = FP.rectangular_wing(100, 80, 1.0, 5.0)
+    speeds = FP.sweep_airspeeds(n_airspeeds)
+    t0 = time.perf_counter()
+    plan = FP.FlutterPlan(v, speeds, FP.params(**FLUTTER_PARAMS), wake_rows=R, max_steps=steps)
+    setup_s = time.perf_counter() - t0
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        plan.run(1)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    wall = time.perf_counter() - t0
+    res = plan.results()
+    ms_wake, pairs = plan.time_wake(5)
+    plan.close()
+    lat = np.asarray(lat)
+    full = lat[R:R + window] if steps >= R + window else lat[-window:]
+    value = n_airspeeds * steps / wall
+    full_value = n_airspeeds * len(full) / (full.sum() / 1e3)
+    achieved = FLOPS_PER_PAIR * pairs / (ms_wake * 1e-3) / 1e12
+    nplain = min(plain_steps, steps)
+    dt = res["dt"]
+    drop = UVLM2DOF("hip", [(v, True)], dtype=np.float64)
+    plat = []
+    t0 = time.perf_counter()
+    pr = drop.run(Vars(U_a=float(speeds[0]), **FLUTTER_PARAMS), t_final_nd=(nplain + 2) * dt + 0.5 * dt, wake_rows=R,
+                  step_times=plat)
+    plain_wall = time.perf_counter() - t0
+    plat = np.asarray(plat) * 1e3
+    pfull = plat[R:R + window] if nplain >= R + window else plat[-window:]
+    plain_full = len(pfull) / (pfull.sum() / 1e3)
+    cl = -res["F"][: nplain + 1, 0, 0] * (FP.PI_F * FLUTTER_PARAMS["mu"])
+    cl_p = -pr["F"][0] * (FP.PI_F * FLUTTER_PARAMS["mu"])
+    out = {
+        "workload": "uvlm_2dof_flutter_sweep_rect_80c_x_100s_R200",
+        "panels": 8000, "wake_rows": R, "airspeeds": n_airspeeds, "U_a": [round(float(speeds[0]), 4),
+                                                                         round(float(speeds[-1]), 4)],
+        "steps": steps, "dt": dt, "setup_s": round(setup_s, 3),
+        "value": round(value, 2), "unit": "airspeed coupled UVLM steps/s (whole sweep)",
+        "full_ring": {"value": round(full_value, 2), "unit": "airspeed steps/s", "steps": int(len(full)),
+                      "speedup_vs_plain_fp64": round(full_value / plain_full, 3)},
+        "latency_ms_per_step": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                "p99": round(float(np.percentile(lat, 99)), 3),
+                                "full_ring_p50": round(float(np.percentile(full, 50)), 3)},
+        "iterations_mean": round(float(res["iters"].mean()), 3),
+        "plain_fp64": {"value": round(plain_full, 3), "unit": "airspeed steps/s at the full ring",
+                       "steps": nplain, "wall_s": round(plain_wall, 2),
+                       "note": "drop-in driver (Backend entry points), first airspeed"},
+        "accuracy": {"max_abs_dCL_vs_plain_fp64": float(np.max(np.abs(cl - cl_p))), "bar": 1e-6,
+                     "iterations_equal": bool(np.array_equal(res["iters"][:nplain, 0], pr["iters"])),
+                     "accepted": bool(np.max(np.abs(cl - cl_p)) <= 1e-6)},
+        "roofline": {"bound": "valu", "kernel": "k_fl_wake<4> (FP64 seeded lattice, ring buffer, all airspeeds)",
+                     "achieved": round(achieved, 3), "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64_VALU_TFLOPS, 4), "avg_launch_us": round(ms_wake * 1e3, 1),
+                     "pairs_per_launch": pairs, "flops_per_pair": FLOPS_PER_PAIR,
+                     "wake_share_of_step": round(2 * ms_wake / float(np.percentile(full, 50)), 3)},
+    }
+    if with_cpu:
+        out["cpu_baseline"] = flutter_cpu_sample(v, R, dt, float(res["iters"].mean()))
+    return out
+
+
+def flutter_cpu_sample(v, R, dt, iters_mean):
+    """CPU cost of one airspeed-step of the flutter sweep at the full ring: the oracle's wake induction
+    over the 200-row ring (8000 receivers x 20000 rings) and one getrs of order 8000 (timed on unit
+    triangular factors: the substitution's cost does not depend on the values), times the sweep's mean
+    iterations per step.  The one-time LU of the influence matrix is not part of the sample."""
+    from oracle import cpuref as C
+
+    w = np.asfortranarray(v.copy())
+    w[:, :, :3] /= 0.5
+    s = C.Surface(verts=w)
+    C.mesh_metrics(0.0, [s])
+    ns = w.shape[0] - 1
+    wake = np.zeros((ns + 1, R + 1, 4), order="F")
+    for r in range(R + 1):
+        wake[:, r] = w[:, -1]
+        wake[:, r, 0] += dt * r
+    s.verts_wake = wake
+    s.gamma_wake = np.asfortranarray(np.random.default_rng(0).standard_normal((ns, R)))
+    n = ns * (w.shape[1] - 1)
+    lu = np.asfortranarray(np.eye(n))
+    piv = np.arange(1, n + 1, dtype=np.int32)  # 1-based, as LAPACK's getrs reads them
+    rhs = np.zeros(n)
+    C.rhs_assemble_wake_influence(np.zeros(n), [s], 2)  # warm-up (thread pool)
+    t0 = time.perf_counter()
+    C.rhs_assemble_wake_influence(rhs, [s], R)
+    C.getrs(lu, piv, rhs)
+    it_s = time.perf_counter() - t0
+    step_s = it_s * iters_mean
+    return {"value": round(1.0 / step_s, 4), "unit": "airspeed steps/s at the full ring",
+            "cores": C.lib().orc_num_threads(), "kind": "port",
+            "sample": f"one fixed-point iteration at the full {R}-row ring (oracle wake induction {n} x {ns * R} "
+                      f"pairs + getrs of order {n}, {it_s:.2f} s) x {iters_mean:.2f} iterations per step; "
+                      f"one-time LU excluded"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -559,6 +674,11 @@ def main():
     ap.add_argument("--two-dof-steps", type=int, default=400, help="timed coupled pitch/heave steps")
     ap.add_argument("--two-dof-large-steps", type=int, default=100, help="timed steps of the 600-panel section")
     ap.add_argument("--no-two-dof", action="store_true")
+    ap.add_argument("--flutter-airspeeds", type=int, default=32, help="airspeeds of the flutter sweep")
+    ap.add_argument("--flutter-steps", type=int, default=1000, help="coupled steps of the flutter sweep")
+    ap.add_argument("--flutter-plain-steps", type=int, default=220, help="steps of the plain FP64 comparison")
+    ap.add_argument("--no-flutter", action="store_true")
+    ap.add_argument("--flutter-only", action="store_true", help="only the flutter section")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -578,6 +698,10 @@ def main():
         return
     if args.mix_only:
         print(json.dumps(serve_mix_bench(args.mix_span_hi, args.mix_span_lo, args.mix_saturation, args.mix_lanes)))
+        return
+    if args.flutter_only:
+        print(json.dumps(flutter_bench(args.flutter_airspeeds, args.flutter_steps, 20, args.flutter_plain_steps,
+                                       not args.no_cpu)))
         return
     verts = rectangular_wing_f32(NS, NC, CHORD, HALF_SPAN)
     dt = time_step(verts, MOTION["u"])
@@ -698,6 +822,9 @@ def main():
     if not args.no_two_dof:
         out["two_dof"] = two_dof_bench(args.two_dof_steps, not args.no_cpu)
         out["two_dof_600"] = two_dof_bench(args.two_dof_large_steps, not args.no_cpu, large=True)
+    if not args.no_flutter:
+        out["flutter"] = flutter_bench(args.flutter_airspeeds, args.flutter_steps, 20, args.flutter_plain_steps,
+                                       not args.no_cpu)
     print(json.dumps(out))
 
 

@@ -14,6 +14,7 @@ the kinematics are evaluated on the host.
 from __future__ import annotations
 
 import math
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -122,10 +123,12 @@ class UVLM2DOF:
         self.dims = [(v.shape[0] - 1, v.shape[1] - 1) for v in self.verts_file]
         self.N = sum(ns * nc for ns, nc in self.dims)
 
-    def run(self, vars: Vars, t_final_nd: float, max_steps: int | None = None, wake_rows: int | None = None):
+    def run(self, vars: Vars, t_final_nd: float, max_steps: int | None = None, wake_rows: int | None = None,
+            step_times: list | None = None):
         """UVLM_2DOF::run (tests/uvlm_2dof.cpp:428-865).  Returns dict t, u (2, n), v, a, F (loads
         [-CL/(pi mu), 2 CM_y/(pi mu r_a^2)]), iters (Newton iterations per step).  wake_rows: only the
-        newest wake_rows ring rows induce (truncated wake; None: the whole wake)."""
+        newest wake_rows ring rows induce (truncated wake; None: the whole wake).  step_times: if a list,
+        the wall time of every step (seconds) is appended to it."""
         torch, T, tdt = self.torch, self.T, self.tdt
         z = lambda *s: H.colmajor(s, tdt)  # noqa: E731
         b = T(vars.b)
@@ -229,6 +232,7 @@ class UVLM2DOF:
         la = (0.0, 0.0, 1.0)
         iters = []
         for i in range(n_loop):
+            t_step = time.perf_counter()
             t_nd = T(i) * dt_nd
             for m in range(len(dims)):
                 gprev[m].copy_(gam[m])
@@ -265,6 +269,8 @@ class UVLM2DOF:
                 if it > 50:
                     break
             iters.append(it)
+            if step_times is not None:
+                step_times.append(time.perf_counter() - t_step)  # each iteration ends with a host read
         return {"t": np.arange(n_loop + 1) * float(dt_nd), "u": u[:, : n_loop + 1], "v": v[:, : n_loop + 1],
                 "a": acc[:, : n_loop + 1], "F": F[:, : n_loop + 1], "iters": np.array(iters), "dt": float(dt_nd),
                 "t_steps": t_steps}
