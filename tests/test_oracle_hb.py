@@ -193,3 +193,25 @@ def test_product_rejects_unknown_backend():
 
     with pytest.raises(RuntimeError, match="Unsupported backend"):
         HBVLM("cpu", [WING, FLAP])
+
+
+def test_libhbvlm3_module_name_resolves_to_the_hip_driver(root):
+    """`from libhbvlm3 import HBVLM` (scripts/dof3_hbvlm.py:23) works unchanged with the module
+    directory on sys.path, and gives the HIP driver; a backend name this build lacks is refused
+    before any device work, as create_backend refuses one (vlm/src/vlm_backend.cpp:26)."""
+    import importlib
+    import os
+    import sys
+
+    path = os.path.join(root, "tinyvlm_amd", "pymod")
+    sys.path.insert(0, path)
+    try:
+        mod = importlib.import_module("libhbvlm3")
+        from tinyvlm_amd.hb import HBVLM
+
+        assert mod.HBVLM is HBVLM
+        with pytest.raises(RuntimeError, match="Unsupported backend: cpu"):
+            mod.HBVLM("cpu", ["mesh/3dof_wing_45x1.x", "mesh/3dof_flap_15x1.x"])
+    finally:
+        sys.path.remove(path)
+        sys.modules.pop("libhbvlm3", None)
