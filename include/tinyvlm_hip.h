@@ -175,6 +175,14 @@ TVLM_DECLARE_BACKEND(float, f32)
  * buffers getrf allocates.  groups/ctr_set may be null; otherwise they hold max_steps entries. */
 int tvlm_getrf_plan(int64_t n, int32_t batch, int32_t elem_bytes, int32_t* path, int32_t* n_steps, int32_t* groups,
                     int32_t* ctr_set, int32_t max_steps, int64_t* coop_ws_bytes);
+/* C -= A B for `batch` matrices (matrix z at c/a/b + z * stride elements; elem_bytes 4 or 8,
+ * column-major, C m x n, A m x k, B k x n): the trailing update inside tvlm_getrf_{f32,f64}, on the
+ * matrix cores.  No counterpart in the reference, where the update runs inside LAPACKE_?getrf
+ * (vlm/backends/cpu/src/vlm_backend_cpu_lapack.cpp:40-50) and cuSOLVER ?getrf
+ * (vlm/backends/cuda/src/vlm_backend_cuda_solvers.cu:71-102); exported so that tests can check
+ * it against a vendor GEMM at the shapes getrf uses. */
+int tvlm_lu_update(int32_t elem_bytes, void* c, int64_t ldc, const void* a, int64_t lda, const void* b, int64_t ldb,
+                   int64_t m, int64_t n, int64_t k, int32_t batch, int64_t stride, void* stream);
 /* Bound of every wait of the cooperative LU panels, in spins (default 4194304; 0 restores it).  No
  * counterpart in the reference: a diagnostic knob with which tests make the bounded waits give up,
  * so that the recovery of a stalled factorisation (info = -999, see tvlm_vlm_refinement) runs.  A
