@@ -247,6 +247,16 @@ int tvlm_uvlm_destroy(tvlm_uvlm_plan* plan);
  * count and the (receiver, wake ring) pairs they evaluate.  Leaves the results of that simulation
  * in the plan. */
 int tvlm_uvlm_time_wake(tvlm_uvlm_plan* plan, double* total_ms, int32_t* launches, double* pairs);
+/* Far-wake aggregation of the fast mode (mode 1) wake induction; no counterpart in the reference,
+ * whose UVLM evaluates every wake ring (vlm/src/vlm.cpp:439).  n_levels (0: exact wake, the
+ * default) triples {age A, row group G, span group S}, A increasing, each G a multiple of the one
+ * before.  Counted from the oldest ring row, level l's blocks are G consecutive rows; the
+ * floor((it - A) / G) oldest blocks are older than A steps; the coarsest level takes its blocks
+ * first, each finer level those of its own count that remain, the newest rows stay exact.  Each
+ * (block, group of S spans) becomes one vortex ring through the wake vertices at its corners with
+ * the mean circulation of the rings it replaces.  An approximation: accept it only where |dCL|
+ * against the exact wake stays within the caller's bar.  Drops a captured graph. */
+int tvlm_uvlm_set_far_wake(tvlm_uvlm_plan* plan, int32_t n_levels, const int32_t* levels);
 
 /* ---- coupled pitch/heave flutter sweep ------------------------------------------------------
  * The coupled aeroelastic driver UVLM_2DOF (tests/uvlm_2dof.cpp:428-865, COUPLED path, with the

@@ -195,13 +195,18 @@ def rhs_assemble_velocities(rhs, surfs: list[Surface], velocities):
     return rhs
 
 
-def rhs_assemble_wake_influence(rhs, surfs: list[Surface], iteration: int, sigma=0.0):
+def rhs_assemble_wake_influence(rhs, surfs: list[Surface], iteration: int, sigma=0.0, far_wake=None):
+    """far_wake: far-wake aggregation levels as in uvlm_run (None: the reference's exact wake)."""
     dt = rhs.dtype
-    getattr(lib(), f"orc_rhs_assemble_wake_influence_{_sfx(dt)}")(
-        _P(_ptr(rhs)), _I32(len(surfs)), _i64s([s.ns for s in surfs]), _i64s([s.nc for s in surfs]),
-        _i64s([s.nwv for s in surfs]), _ptrs([s.gamma_wake for s in surfs]), _ptrs([s.colloc for s in surfs]),
-        _ptrs([s.normals for s in surfs]), _ptrs([s.verts_wake for s in surfs]),
-        (ctypes.c_int32 * len(surfs))(*[int(s.lifting) for s in surfs]), _I32(iteration), _real(dt)(sigma))
+    args = (_P(_ptr(rhs)), _I32(len(surfs)), _i64s([s.ns for s in surfs]), _i64s([s.nc for s in surfs]),
+            _i64s([s.nwv for s in surfs]), _ptrs([s.gamma_wake for s in surfs]), _ptrs([s.colloc for s in surfs]),
+            _ptrs([s.normals for s in surfs]), _ptrs([s.verts_wake for s in surfs]),
+            (ctypes.c_int32 * len(surfs))(*[int(s.lifting) for s in surfs]), _I32(iteration), _real(dt)(sigma))
+    if far_wake is None:
+        getattr(lib(), f"orc_rhs_assemble_wake_influence_{_sfx(dt)}")(*args)
+    else:
+        far = far_levels_array(far_wake)
+        getattr(lib(), f"orc_rhs_assemble_wake_influence_far_{_sfx(dt)}")(*args, _P(_ptr(far)))
     return rhs
 
 
@@ -376,8 +381,25 @@ def motion_transform(motion, t):
     return T.reshape(4, 4, order="F"), dT.reshape(4, 4, order="F")
 
 
-def uvlm_run(verts_list, motion, t_steps=0, t_final=0.0):
-    """UVLM::run restatement for the prescribed motion; returns (cl[t], cm_y[t], gamma_last, t_steps)."""
+def far_levels_array(far_wake):
+    """[(A, G, S), ...] (or one (A, G, S)) -> int32 {L, A_1, G_1, S_1, ...}, checked: A increasing,
+    G_l >= 1 a multiple of G_{l-1}, S_l >= 1, at most 16 levels."""
+    lv = [tuple(int(x) for x in far_wake)] if np.ndim(far_wake) == 1 else [tuple(int(x) for x in t) for t in far_wake]
+    assert 1 <= len(lv) <= 16
+    for k, (A, G, S) in enumerate(lv):
+        assert A >= 0 and G >= 1 and S >= 1
+        if k:
+            assert A > lv[k - 1][0] and G % lv[k - 1][1] == 0
+    return np.ascontiguousarray([len(lv)] + [x for t in lv for x in t], dtype=np.int32)
+
+
+def uvlm_run(verts_list, motion, t_steps=0, t_final=0.0, far_wake=None):
+    """UVLM::run restatement for the prescribed motion; returns (cl[t], cm_y[t], gamma_last, t_steps).
+
+    far_wake = [(A_1, G_1, S_1), ...] evaluates the wake with far-wake aggregation: levels of age
+    threshold A_l, row group G_l (each a multiple of the one before) and span group S_l; ring rows older
+    than A_l steps are merged into G_l x S_l rings of mean circulation (one tuple: one level).  None is
+    the exact wake of the reference."""
     dt = verts_list[0].dtype
     N = sum((v.shape[0] - 1) * (v.shape[1] - 1) for v in verts_list)
     cap = t_steps if t_steps > 0 else 1_000_000
@@ -387,10 +409,14 @@ def uvlm_run(verts_list, motion, t_steps=0, t_final=0.0):
     mot = np.ascontiguousarray(motion, dtype=dt)
     ts = ctypes.c_int64(0)
     R = _real(dt)
-    info = getattr(lib(), f"orc_uvlm_run_{_sfx(dt)}")(
-        _I32(len(verts_list)), _i64s([v.shape[0] - 1 for v in verts_list]), _i64s([v.shape[1] - 1 for v in verts_list]),
-        _ptrs(verts_list), _P(_ptr(mot)), _I64(t_steps), R(t_final), _P(_ptr(cl)), _P(_ptr(cm)), _P(_ptr(gamma)),
-        ctypes.byref(ts))
+    args = (_I32(len(verts_list)), _i64s([v.shape[0] - 1 for v in verts_list]),
+            _i64s([v.shape[1] - 1 for v in verts_list]), _ptrs(verts_list), _P(_ptr(mot)), _I64(t_steps), R(t_final),
+            _P(_ptr(cl)), _P(_ptr(cm)), _P(_ptr(gamma)), ctypes.byref(ts))
+    if far_wake is None:
+        info = getattr(lib(), f"orc_uvlm_run_{_sfx(dt)}")(*args)
+    else:
+        far = far_levels_array(far_wake)
+        info = getattr(lib(), f"orc_uvlm_run_far_{_sfx(dt)}")(*args, _P(_ptr(far)))
     if info != 0:
         raise RuntimeError(f"singular LHS (getrf info {info})")
     n = ts.value - 1

@@ -47,6 +47,12 @@ extern "C" {
                                              REAL* const* gamma_wake, REAL* const* colloc,            \
                                              REAL* const* normals, REAL* const* verts_wake,           \
                                              const int32_t* lifting, int32_t iteration, REAL sigma);  \
+    void orc_rhs_assemble_wake_influence_far_##S(REAL* rhs, int32_t nsurf, const int64_t* ns,         \
+                                                 const int64_t* nc, const int64_t* nwv,               \
+                                                 REAL* const* gamma_wake, REAL* const* colloc,        \
+                                                 REAL* const* normals, REAL* const* verts_wake,       \
+                                                 const int32_t* lifting, int32_t iteration,           \
+                                                 REAL sigma, const int32_t* far);                     \
     void orc_mesh_metrics_##S(REAL alpha_rad, int32_t nsurf, const int64_t* ns, const int64_t* nc,    \
                               REAL* const* verts_wing, REAL* const* colloc, REAL* const* normals,     \
                               REAL* const* areas);                                                    \
@@ -88,7 +94,11 @@ extern "C" {
     int orc_uvlm_run_##S(int32_t nsurf, const int64_t* ns, const int64_t* nc,                         \
                          REAL* const* verts_init, const REAL* motion, int64_t t_steps_override,       \
                          REAL t_final, REAL* cl_out, REAL* cm_out, REAL* gamma_out,                   \
-                         int64_t* t_steps_out);
+                         int64_t* t_steps_out);                                                       \
+    int orc_uvlm_run_far_##S(int32_t nsurf, const int64_t* ns, const int64_t* nc,                     \
+                             REAL* const* verts_init, const REAL* motion, int64_t t_steps_override,   \
+                             REAL t_final, REAL* cl_out, REAL* cm_out, REAL* gamma_out,               \
+                             int64_t* t_steps_out, const int32_t* far);
 
 ORC_DECLARE(double, d)
 ORC_DECLARE(float, f)

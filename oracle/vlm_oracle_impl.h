@@ -176,12 +176,85 @@ static void FN(wake_influence_one)(const REAL* c, const REAL* n, const REAL* v, 
     *rhs -= FN(dot3)(ind_tot, n);
 }
 
+/* Far-wake aggregation (no counterpart in the reference; the served UVLM loop's approximation, accepted
+ * only at |dCL| <= 1e-6 against the exact wake).  far = {L, A_1, G_1, S_1, ..., A_L, G_L, S_L}: L levels
+ * with age thresholds A_1 < ... < A_L, row groups G_l (each a multiple of the one before) and span
+ * groups S_l.  At iteration it, level l's blocks are G_l consecutive ring rows counted from the oldest
+ * row (block b: rows [end-(b+1)G_l, end-bG_l), end = v_m-1); the C_l = floor((it - A_l) / G_l) oldest
+ * blocks are older than A_l.  From the oldest end, level L takes its C_L blocks, each finer level the
+ * blocks of its own count that the coarser levels left, and the remaining (newest) rows stay exact.
+ * Within a level, each block is cut into groups of S_l spans (group j: spans [jS, min((j+1)S, ns))) and
+ * each (block, group) is replaced by ONE vortex ring through the lattice vertices at its corners,
+ * carrying the mean circulation of the G_l x S rings it replaces (summed rows-then-spans in order). */
+static void FN(far_levels)(int32_t iteration, int64_t end, const int32_t* far, int64_t* lo, int64_t* nb) {
+    const int32_t L = far[0];
+    int64_t hi = end;
+    for (int32_t l = L - 1; l >= 0; l--) {
+        const int64_t A = far[1 + 3 * l], G = far[2 + 3 * l];
+        const int64_t C = iteration >= A ? (iteration - A) / G : 0;
+        const int64_t cov = (end - hi) / G;
+        nb[l] = C > cov ? C - cov : 0;
+        lo[l] = hi - nb[l] * G;
+        hi = lo[l];
+    }
+}
+
+static void FN(wake_influence_far_one)(const REAL* c, const REAL* n, const REAL* v, int64_t v_ld, int64_t v_m,
+                                       int64_t v_n, const REAL* gamma, REAL* rhs, REAL sigma, int32_t iteration,
+                                       const int32_t* far) {
+    const int64_t ns = v_n - 1, end = v_m - 1;
+    const int32_t L = far[0];
+    int64_t lo[16], nb[16];
+    FN(far_levels)(iteration, end, far, lo, nb);
+    const int64_t near_end = L > 0 ? lo[0] : end;
+    REAL ind_tot[3] = {0, 0, 0};
+    for (int64_t i = v_m - iteration - 1; i < near_end; i++) {
+        for (int64_t j = 0; j < ns; j++) {
+            const int64_t i0 = i * v_n + j, i1 = i0 + 1, i2 = (i + 1) * v_n + j + 1, i3 = (i + 1) * v_n + j;
+            const REAL v0[3] = {v[i0], v[v_ld + i0], v[2 * v_ld + i0]};
+            const REAL v1[3] = {v[i1], v[v_ld + i1], v[2 * v_ld + i1]};
+            const REAL v2[3] = {v[i2], v[v_ld + i2], v[2 * v_ld + i2]};
+            const REAL v3[3] = {v[i3], v[v_ld + i3], v[2 * v_ld + i3]};
+            REAL ind[3];
+            FN(ring)(ind, c, v0, v1, v2, v3, sigma);
+            const REAL g = gamma[i * ns + j];
+            ind_tot[0] += ind[0] * g;
+            ind_tot[1] += ind[1] * g;
+            ind_tot[2] += ind[2] * g;
+        }
+    }
+    for (int32_t l = 0; l < L; l++) {
+        const int64_t G = far[2 + 3 * l], S = far[3 + 3 * l];
+        for (int64_t q = 0; q < nb[l]; q++) {
+            const int64_t rt = lo[l] + q * G, rb = rt + G; /* vertex rows of the block's edges */
+            for (int64_t j0 = 0; j0 < ns; j0 += S) {
+                const int64_t j1 = j0 + S < ns ? j0 + S : ns;
+                REAL gs = 0;
+                for (int64_t i = rt; i < rb; i++)
+                    for (int64_t j = j0; j < j1; j++) gs += gamma[i * ns + j];
+                const REAL g = gs / (REAL)((rb - rt) * (j1 - j0));
+                const int64_t i0 = rt * v_n + j0, i1 = rt * v_n + j1, i2 = rb * v_n + j1, i3 = rb * v_n + j0;
+                const REAL v0[3] = {v[i0], v[v_ld + i0], v[2 * v_ld + i0]};
+                const REAL v1[3] = {v[i1], v[v_ld + i1], v[2 * v_ld + i1]};
+                const REAL v2[3] = {v[i2], v[v_ld + i2], v[2 * v_ld + i2]};
+                const REAL v3[3] = {v[i3], v[v_ld + i3], v[2 * v_ld + i3]};
+                REAL ind[3];
+                FN(ring)(ind, c, v0, v1, v2, v3, sigma);
+                ind_tot[0] += ind[0] * g;
+                ind_tot[1] += ind[1] * g;
+                ind_tot[2] += ind[2] * g;
+            }
+        }
+    }
+    *rhs -= FN(dot3)(ind_tot, n);
+}
+
 /* Backend::rhs_assemble_wake_influence (vlm_backend_cpu.cpp:181-224, vlm_backend_cpud.cpp:124-167):
  * every receiver of every surface sees the wake of every lifting surface. */
-void FN(orc_rhs_assemble_wake_influence)(REAL* rhs, int32_t nsurf, const int64_t* ns, const int64_t* nc,
-                                         const int64_t* nwv, REAL* const* gamma_wake, REAL* const* colloc,
-                                         REAL* const* normals, REAL* const* verts_wake, const int32_t* lifting,
-                                         int32_t iteration, REAL sigma) {
+static void FN(wake_influence_all)(REAL* rhs, int32_t nsurf, const int64_t* ns, const int64_t* nc,
+                                   const int64_t* nwv, REAL* const* gamma_wake, REAL* const* colloc,
+                                   REAL* const* normals, REAL* const* verts_wake, const int32_t* lifting,
+                                   int32_t iteration, REAL sigma, const int32_t* far) {
     int64_t offset = 0;
     for (int32_t mi = 0; mi < nsurf; mi++) {
         const int64_t np = ns[mi] * nc[mi];
@@ -193,12 +266,32 @@ void FN(orc_rhs_assemble_wake_influence)(REAL* rhs, int32_t nsurf, const int64_t
             const REAL n[3] = {ni[idx], ni[np + idx], ni[2 * np + idx]};
             for (int32_t mj = 0; mj < nsurf; mj++) {
                 if (!lifting[mj]) continue;
-                FN(wake_influence_one)(c, n, verts_wake[mj], (ns[mj] + 1) * nwv[mj], nwv[mj], ns[mj] + 1, gamma_wake[mj],
-                                       rhs + offset + idx, sigma, iteration);
+                if (far && far[0] > 0)
+                    FN(wake_influence_far_one)(c, n, verts_wake[mj], (ns[mj] + 1) * nwv[mj], nwv[mj], ns[mj] + 1,
+                                               gamma_wake[mj], rhs + offset + idx, sigma, iteration, far);
+                else
+                    FN(wake_influence_one)(c, n, verts_wake[mj], (ns[mj] + 1) * nwv[mj], nwv[mj], ns[mj] + 1,
+                                           gamma_wake[mj], rhs + offset + idx, sigma, iteration);
             }
         }
         offset += np;
     }
+}
+void FN(orc_rhs_assemble_wake_influence)(REAL* rhs, int32_t nsurf, const int64_t* ns, const int64_t* nc,
+                                         const int64_t* nwv, REAL* const* gamma_wake, REAL* const* colloc,
+                                         REAL* const* normals, REAL* const* verts_wake, const int32_t* lifting,
+                                         int32_t iteration, REAL sigma) {
+    FN(wake_influence_all)(rhs, nsurf, ns, nc, nwv, gamma_wake, colloc, normals, verts_wake, lifting, iteration, sigma,
+                           NULL);
+}
+
+/* The same with far-wake aggregation (far as in orc_uvlm_run_far; NULL or L = 0: exact). */
+void FN(orc_rhs_assemble_wake_influence_far)(REAL* rhs, int32_t nsurf, const int64_t* ns, const int64_t* nc,
+                                             const int64_t* nwv, REAL* const* gamma_wake, REAL* const* colloc,
+                                             REAL* const* normals, REAL* const* verts_wake, const int32_t* lifting,
+                                             int32_t iteration, REAL sigma, const int32_t* far) {
+    FN(wake_influence_all)(rhs, nsurf, ns, nc, nwv, gamma_wake, colloc, normals, verts_wake, lifting, iteration, sigma,
+                           far);
 }
 
 /* Backend::mesh_metrics (vlm_backend_cpu.cpp:391-433, vlm_backend_cpud.cpp:258-300). */
@@ -664,9 +757,9 @@ int FN(orc_vlm_run)(int32_t nsurf, const int64_t* ns, const int64_t* nc, REAL* c
  * t_steps = (int64)(t_final/dt) unless t_steps_override > 0; the loop runs i = 0 .. t_steps-2.
  * Outputs per step i (valid for i >= 1; step 0 writes 0): cl_out[i], cm_out[i] (y component,
  * about T(t)*(0.5,0,0)).  gamma_out receives the last step's wing circulation (N values). */
-int FN(orc_uvlm_run)(int32_t nsurf, const int64_t* ns, const int64_t* nc, REAL* const* verts_init, const REAL* motion,
-                     int64_t t_steps_override, REAL t_final, REAL* cl_out, REAL* cm_out, REAL* gamma_out,
-                     int64_t* t_steps_out) {
+int FN(orc_uvlm_run_far)(int32_t nsurf, const int64_t* ns, const int64_t* nc, REAL* const* verts_init,
+                         const REAL* motion, int64_t t_steps_override, REAL t_final, REAL* cl_out, REAL* cm_out,
+                         REAL* gamma_out, int64_t* t_steps_out, const int32_t* far) {
     if (nsurf < 1 || nsurf > 64) return -1;
     double mot[7];
     for (int k = 0; k < 7; k++) mot[k] = (double)motion[k];
@@ -736,7 +829,7 @@ int FN(orc_uvlm_run)(int32_t nsurf, const int64_t* ns, const int64_t* nc, REAL* 
         }
         memset(rhs, 0, sizeof(REAL) * (size_t)N);
         FN(orc_rhs_assemble_velocities)(rhs, nsurf, ns, nc, no, ve);
-        FN(orc_rhs_assemble_wake_influence)(rhs, nsurf, ns, nc, nwv, gk, co, no, vk, lifting, (int32_t)i, (REAL)0);
+        FN(wake_influence_all)(rhs, nsurf, ns, nc, nwv, gk, co, no, vk, lifting, (int32_t)i, (REAL)0, far);
         FN(orc_getrs)(N, 1, lhs, N, ipiv, rhs, N);
         int64_t off = 0;
         const REAL inv_dt = (REAL)1 / dt;
@@ -779,4 +872,13 @@ int FN(orc_uvlm_run)(int32_t nsurf, const int64_t* ns, const int64_t* nc, REAL* 
     free(rhs);
     free(ipiv);
     return info;
+}
+
+/* UVLM::run with the exact wake (the reference), or with far-wake aggregation: far = {L, A_1, G_1, S_1,
+ * ...} (far-wake aggregation above), NULL or L = 0 for the exact wake. */
+int FN(orc_uvlm_run)(int32_t nsurf, const int64_t* ns, const int64_t* nc, REAL* const* verts_init, const REAL* motion,
+                     int64_t t_steps_override, REAL t_final, REAL* cl_out, REAL* cm_out, REAL* gamma_out,
+                     int64_t* t_steps_out) {
+    return FN(orc_uvlm_run_far)(nsurf, ns, nc, verts_init, motion, t_steps_override, t_final, cl_out, cm_out,
+                                gamma_out, t_steps_out, NULL);
 }

@@ -129,6 +129,11 @@ struct WakeSurface {
     int32_t row_begin, row_end;  // ring rows [row_begin, row_end)
     int32_t sw, rh;              // tile extent: spans, rows
     int32_t tiles_span, tile_begin;
+    // k_wake_lattice only: a coarse lattice over a finer wake of nsf spans (0: this is the wake
+    // itself).  Coarse vertex (column c, row r) is the fine vertex (min(c vcs, nsf), vrow0 + r vrs);
+    // coarse ring (r, c) carries the mean circulation of the fine rings of rows
+    // [vrow0 + r vrs, vrow0 + (r+1) vrs) and spans [c vcs, min((c+1) vcs, nsf)).
+    int32_t vrs = 1, vcs = 1, vrow0 = 0, nsf = 0;
 };
 
 template <typename T>
@@ -384,16 +389,30 @@ __global__ __launch_bounds__(kThreads) void k_wake_lattice(TS* __restrict__ part
     const int swe = min(ws.sw, ws.ns - s0);
     const int rhe = min(ws.rh, ws.row_end - r0);
     const int vw = swe + 1;
+    const int nsf = ws.nsf ? ws.nsf : ws.ns;
     for (int e = threadIdx.x; e < (rhe + 1) * vw; e += kThreads) {
         const int q = e / vw, sp = e % vw;
-        const int64_t g = (int64_t)(s0 + sp) + (int64_t)(r0 + q) * (ws.ns + 1);
+        const int col = min((s0 + sp) * ws.vcs, nsf);
+        const int64_t g = (int64_t)col + (int64_t)(ws.vrow0 + (r0 + q) * ws.vrs) * (nsf + 1);
         sverts[3 * e + 0] = (double)ws.verts[g];
         sverts[3 * e + 1] = (double)ws.verts[g + ws.ws2];
         sverts[3 * e + 2] = (double)ws.verts[g + 2 * ws.ws2];
     }
-    for (int e = threadIdx.x; e < rhe * swe; e += kThreads) {
-        const int q = e / swe, sp = e % swe;
-        sgam[e] = (double)ws.gamma[(int64_t)(r0 + q) * ws.ns + s0 + sp];
+    if (ws.vrs == 1 && ws.vcs == 1) {
+        for (int e = threadIdx.x; e < rhe * swe; e += kThreads) {
+            const int q = e / swe, sp = e % swe;
+            sgam[e] = (double)ws.gamma[(int64_t)(ws.vrow0 + r0 + q) * nsf + s0 + sp];
+        }
+    } else {  // coarse ring: mean of the fine rings it replaces, rows then spans in order
+        for (int e = threadIdx.x; e < rhe * swe; e += kThreads) {
+            const int q = e / swe, sp = e % swe;
+            const int fr0 = ws.vrow0 + (r0 + q) * ws.vrs, fc0 = (s0 + sp) * ws.vcs;
+            const int fc1 = min(fc0 + ws.vcs, nsf);
+            double sum = 0.0;
+            for (int fr = fr0; fr < fr0 + ws.vrs; fr++)
+                for (int fc = fc0; fc < fc1; fc++) sum += (double)ws.gamma[(int64_t)fr * nsf + fc];
+            sgam[e] = sum / (double)(ws.vrs * (fc1 - fc0));
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & (kRecv - 1);

@@ -459,6 +459,9 @@ struct tvlm_uvlm_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> wake_events;  // start/stop around the replayed wake launches
+    // far-wake aggregation (fast mode): levels of age threshold far_A, row group far_G, span group far_S
+    int32_t far_n = 0;
+    int32_t far_A[8] = {}, far_G[8] = {}, far_S[8] = {};
 
     int alloc(void** p, size_t bytes) {
         TVLM_HIP(hipMalloc(p, bytes));
@@ -482,7 +485,42 @@ int plan_views(tvlm_uvlm_plan* P, int m, tvlm_view& verts, tvlm_view& colloc, tv
     return TVLM_OK;
 }
 
-// The wake tile table for iteration `it` (rows [t_steps-1-it, t_steps-1) of every surface).
+// Far-wake partition of iteration `it` (ring rows [end - it, end) exist, end = t_steps - 1; the
+// newest row is end - it): from the oldest row on, level l (coarsest first) takes nb[l] blocks of
+// far_G[l] rows, rows [lo[l], lo[l] + nb[l] far_G[l]); a level's blocks are counted from the oldest
+// row, the C = floor((it - far_A[l]) / far_G[l]) oldest of them are older than far_A[l] steps, and
+// the coarser levels' blocks are taken out first.  Rows [end - it, exact_end) stay exact.
+// Returns exact_end.
+int far_partition(const tvlm_uvlm_plan* P, int it, int* lo, int* nb) {
+    const int end = (int)P->t_steps - 1;
+    int hi = end;
+    for (int l = P->far_n - 1; l >= 0; l--) {
+        const int A = P->far_A[l], G = P->far_G[l];
+        const int C = it >= A ? (it - A) / G : 0;
+        const int cov = (end - hi) / G;
+        nb[l] = C > cov ? C - cov : 0;
+        lo[l] = hi - nb[l] * G;
+        hi = lo[l];
+    }
+    return hi;
+}
+
+// (receiver, wake ring) pairs of iteration `it`: exact rings plus the far levels' coarse rings.
+double wake_pairs(const tvlm_uvlm_plan* P, int it) {
+    int lo[8], nb[8];
+    const int exact_end = far_partition(P, it, lo, nb);
+    const int end = (int)P->t_steps - 1;
+    double rings = 0.0;
+    for (int m = 0; m < P->n_surf; m++) {
+        const int ns = P->tab.s[m].ns;
+        rings += (double)ns * (double)(exact_end - (end - it));
+        for (int l = 0; l < P->far_n; l++) rings += (double)((ns + P->far_S[l] - 1) / P->far_S[l]) * nb[l];
+    }
+    return rings * (double)P->N;
+}
+
+// The wake tile table for iteration `it` (rows [t_steps-1-it, t_steps-1) of every surface; in fast
+// mode with far-wake levels, the exact rows plus one coarse lattice per level that holds blocks).
 void wake_tables(const tvlm_uvlm_plan* P, int it, RecvTable<double>& rt, WakeTable<double>& wt, int& tiles) {
     rt = {};
     wt = {};
@@ -499,24 +537,47 @@ void wake_tables(const tvlm_uvlm_plan* P, int it, RecvTable<double>& rt, WakeTab
     rt.offset[P->n_surf] = P->N;
     rt.total = P->N;
     tiles = 0;
+    int lo[8], nb[8];
+    const bool far = P->mode == 1 && P->far_n > 0;
+    const int exact_end = far ? far_partition(P, it, lo, nb) : (int)P->t_steps - 1;
+    const int recv_blocks = (P->N + kRecv - 1) / kRecv;
     for (int m = 0; m < P->n_surf; m++) {
         const USurf& S = P->tab.s[m];
-        WakeSurface<double>& w = wt.s[wt.n];
-        w.verts = S.wake;
-        w.ws2 = (int64_t)(S.ns + 1) * P->t_steps;
-        w.gamma = S.gamma_wake;
-        w.ns = S.ns;
-        w.nwv = (int32_t)P->t_steps;
+        WakeSurface<double> base{};
+        base.verts = S.wake;
+        base.ws2 = (int64_t)(S.ns + 1) * P->t_steps;
+        base.gamma = S.gamma_wake;
+        base.ns = S.ns;
+        base.nwv = (int32_t)P->t_steps;
+        auto add = [&](const WakeSurface<double>& w0) {
+            WakeSurface<double>& w = wt.s[wt.n];
+            w = w0;
+            if (P->mode == 1)
+                lattice_tile_shape(w.ns, w.row_end - w.row_begin, recv_blocks, w.sw, w.rh);
+            else
+                wake_tile_shape(w.ns, w.row_end - w.row_begin, recv_blocks, w.sw, w.rh);
+            w.tiles_span = (w.ns + w.sw - 1) / w.sw;
+            w.tile_begin = tiles;
+            tiles += w.tiles_span * ((w.row_end - w.row_begin + w.rh - 1) / w.rh);
+            wt.n++;
+        };
+        WakeSurface<double> w = base;
         w.row_begin = (int32_t)P->t_steps - 1 - it;
-        w.row_end = (int32_t)P->t_steps - 1;
-        if (P->mode == 1)
-            lattice_tile_shape(S.ns, w.row_end - w.row_begin, (P->N + kRecv - 1) / kRecv, w.sw, w.rh);
-        else
-            wake_tile_shape(S.ns, w.row_end - w.row_begin, (P->N + kRecv - 1) / kRecv, w.sw, w.rh);
-        w.tiles_span = (S.ns + w.sw - 1) / w.sw;
-        w.tile_begin = tiles;
-        tiles += w.tiles_span * ((w.row_end - w.row_begin + w.rh - 1) / w.rh);
-        wt.n++;
+        w.row_end = exact_end;
+        if (w.row_end > w.row_begin || !far) add(w);
+        if (!far) continue;
+        for (int l = 0; l < P->far_n; l++) {
+            if (nb[l] == 0) continue;
+            WakeSurface<double> c = base;
+            c.ns = (S.ns + P->far_S[l] - 1) / P->far_S[l];
+            c.nsf = S.ns;
+            c.vrs = P->far_G[l];
+            c.vcs = P->far_S[l];
+            c.vrow0 = lo[l];
+            c.row_begin = 0;
+            c.row_end = nb[l];
+            add(c);
+        }
     }
     wt.total_tiles = tiles;
 }
@@ -841,15 +902,49 @@ int tvlm_uvlm_time_wake(tvlm_uvlm_plan* P, double* total_ms, int32_t* launches, 
     }
     TVLM_HIP(hipEventRecord(P->wake_events[1], P->stream));
     TVLM_HIP(hipStreamSynchronize(P->stream));
-    int64_t rings_per_row = 0;
-    for (int m = 0; m < P->n_surf; m++) rings_per_row += P->tab.s[m].ns;
     double pr = 0.0;
-    for (int i = 1; i < last; i++) pr += (double)P->N * (double)rings_per_row * (double)i;
+    for (int i = 1; i < last; i++) pr += wake_pairs(P, i);
     float ms = 0.f;
     TVLM_HIP(hipEventElapsedTime(&ms, P->wake_events[0], P->wake_events[1]));
     *total_ms = ms;
     *launches = last - 1;
     *pairs = pr;
+    return TVLM_OK;
+}
+
+// Far-wake aggregation of the fast mode's wake induction (see far_partition): n_levels (0: exact
+// wake) triples {age threshold, row group, span group}, ages increasing, each row group a multiple
+// of the one before.  Drops a captured graph (its launches change).
+int tvlm_uvlm_set_far_wake(tvlm_uvlm_plan* P, int32_t n_levels, const int32_t* levels) {
+    TVLM_REQUIRE(P, "null plan");
+    TVLM_REQUIRE(P->mode == 1, "far-wake aggregation needs mode 1 (the plain mode is the exact reference)");
+    TVLM_REQUIRE(n_levels >= 0 && n_levels <= 8, "n_levels %d out of range [0, 8]", n_levels);
+    TVLM_REQUIRE(n_levels == 0 || levels, "null levels");
+    TVLM_REQUIRE(P->n_surf * (1 + n_levels) <= tvlm::kMaxWakes, "%d surfaces x (1 + %d levels) exceed %d wake tables",
+                 P->n_surf, n_levels, tvlm::kMaxWakes);
+    for (int l = 0; l < n_levels; l++) {
+        const int32_t A = levels[3 * l], G = levels[3 * l + 1], S = levels[3 * l + 2];
+        TVLM_REQUIRE(A >= 0 && G >= 1 && G <= (1 << 20) && S >= 1, "level %d: bad {age %d, rows %d, spans %d}", l, A,
+                     G, S);
+        if (l > 0)
+            TVLM_REQUIRE(A > levels[3 * (l - 1)] && G % levels[3 * (l - 1) + 1] == 0,
+                         "level %d: ages must increase and row groups nest", l);
+    }
+    if (P->exec) {
+        TVLM_HIP(hipStreamSynchronize(P->stream));
+        TVLM_HIP(hipGraphExecDestroy(P->exec));
+        P->exec = nullptr;
+    }
+    if (P->graph) {
+        TVLM_HIP(hipGraphDestroy(P->graph));
+        P->graph = nullptr;
+    }
+    P->far_n = n_levels;
+    for (int l = 0; l < n_levels; l++) {
+        P->far_A[l] = levels[3 * l];
+        P->far_G[l] = levels[3 * l + 1];
+        P->far_S[l] = levels[3 * l + 2];
+    }
     return TVLM_OK;
 }
 

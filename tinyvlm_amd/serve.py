@@ -204,7 +204,9 @@ class Server:
                  budget_bytes: int | None = None, plans_per_mesh: int | None = None, u_inf: float = 1.0,
                  rho: float = 1.0, keep_gamma: bool = False, unsteady: dict | None = None):
         """meshes: name -> list of verts arrays (one per surface, as read_plot3d returns them).
-        unsteady: name -> (t_steps, dt) for the meshes that take unsteady requests."""
+        unsteady: name -> (t_steps, dt) or (t_steps, dt, far_wake) for the meshes that take unsteady
+        requests; far_wake = [(age, row_group, span_group), ...] aggregates the wake beyond `age` steps
+        (UVLMPlan.set_far_wake; accept it only where it keeps |dCL| within the bar)."""
         if lanes < 1 or max_batch < 1:
             raise ValueError("lanes and max_batch must be positive")
         self.meshes = {k: [np.asarray(v, dtype=np.float64) for v in vl] for k, vl in meshes.items()}
@@ -233,7 +235,7 @@ class Server:
         return 12 * n * n * b + 400 * n * b
 
     def _uestimate(self, key: str) -> int:
-        t_steps, _ = self.unsteady[key]
+        t_steps = self.unsteady[key][0]
         n = self.N[key]
         return 16 * n * n + 8 * n * t_steps + 256 * n + (1 << 20)  # LHS, inverse, rhs of all steps, wake
 
@@ -282,13 +284,16 @@ class Server:
         s = self.uslots.get((key, lane))
         if s is not None:
             return None if s.busy else s
-        t_steps, dt = self.unsteady[key]
+        t_steps, dt = self.unsteady[key][:2]
+        far = self.unsteady[key][2] if len(self.unsteady[key]) > 2 else None
         need = self._uestimate(key)
         if need > self.budget:
             raise RuntimeError(f"mesh {key}: one unsteady plan ({need} bytes) exceeds the memory budget {self.budget}")
         if self.used_bytes + need > self.budget and not self._evict_for(need, now):
             return None
         plan = UVLMPlan(self.meshes[key], t_steps, dt, mode=MODE_FAST, stream=self.lanes[lane])
+        if far:
+            plan.set_far_wake(far)
         s = _Slot(plan, need, unsteady=True)
         self.uslots[(key, lane)] = s
         self.used_bytes += need

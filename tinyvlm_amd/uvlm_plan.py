@@ -62,6 +62,14 @@ class UVLMPlan:
                                          ctypes.c_int64(self.t_steps), ctypes.c_double(self.dt),
                                          ctypes.c_int32(mode), H.stream_ptr(self.stream)), "tvlm_uvlm_create")
 
+    def set_far_wake(self, levels):
+        """Far-wake aggregation of the fast mode: levels = [(age, row_group, span_group), ...] (None or
+        [] for the exact wake); see tvlm_uvlm_set_far_wake."""
+        lv = [] if not levels else [tuple(int(x) for x in t) for t in levels]
+        arr = (ctypes.c_int32 * max(1, 3 * len(lv)))(*[x for t in lv for x in t])
+        H.check(H.lib().tvlm_uvlm_set_far_wake(self._plan, ctypes.c_int32(len(lv)), arr), "tvlm_uvlm_set_far_wake")
+        self.far_wake = lv
+
     def set_pitch_heave(self, u, h_amp=0.0, h_phase=0.0, a_amp=0.0, a_phase=0.0, omega=0.0, x_pivot=0.25):
         d = ctypes.c_double
         H.check(H.lib().tvlm_uvlm_set_pitch_heave(self._plan, d(u), d(h_amp), d(h_phase), d(a_amp), d(a_phase),
