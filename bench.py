@@ -677,7 +677,8 @@ def main():
     ap.add_argument("--mix-saturation", type=int, default=384, help="requests of the saturation run")
     ap.add_argument("--mix-lanes", type=int, default=3)
     ap.add_argument("--uvlm-lanes", type=int, default=4, help="lanes of the served_uvlm run")
-    ap.add_argument("--no-mix", action="store_true")
+    ap.add_argument("--no-mix", action="store_true", help="skip the served mix and the getrf probe")
+    ap.add_argument("--no-served-uvlm", action="store_true")
     ap.add_argument("--uvlm-requests", type=int, default=32, help="concurrent UVLM requests of the served_uvlm run")
     ap.add_argument("--lu-probe", type=int, default=0, help="only time getrf of one N x N matrix (FP32, FP64)")
     ap.add_argument("--mix-only", action="store_true", help="only the served_mix section")
@@ -848,7 +849,7 @@ def main():
         out["cpu_baseline"] = None
     fast.close()
     plain.close()
-    if not args.no_mix:
+    if not args.no_served_uvlm:
         out["served_uvlm"] = serve_uvlm_bench(verts, dt, res_fast["cl"], plain_value, args.uvlm_requests,
                                               args.uvlm_lanes, None if args.exact_wake else FAR_WAKE)
     if not args.no_steady:

@@ -257,6 +257,11 @@ int tvlm_uvlm_time_wake(tvlm_uvlm_plan* plan, double* total_ms, int32_t* launche
  * the mean circulation of the rings it replaces.  An approximation: accept it only where |dCL|
  * against the exact wake stays within the caller's bar.  Drops a captured graph. */
 int tvlm_uvlm_set_far_wake(tvlm_uvlm_plan* plan, int32_t n_levels, const int32_t* levels);
+/* Work-group target of the fast mode's wake-lattice launches (default 1024): a launch above it takes
+ * 2 or 4 wake rows per tile, a launch below it splits its spans (at least 2 per wave) to reach it.
+ * No counterpart in the reference (a tuning knob; tinyvlm_amd.autotune picks it per workload).
+ * Drops a captured graph. */
+int tvlm_uvlm_set_lattice_target(tvlm_uvlm_plan* plan, int32_t groups);
 
 /* ---- coupled pitch/heave flutter sweep ------------------------------------------------------
  * The coupled aeroelastic driver UVLM_2DOF (tests/uvlm_2dof.cpp:428-865, COUPLED path, with the

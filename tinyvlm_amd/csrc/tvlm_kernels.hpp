@@ -230,6 +230,25 @@ __global__ __launch_bounds__(kThreads) void k_wake_partials(TS* __restrict__ par
 // every 16th tile, then a fixed-order tree over the waves (bitwise reproducible).
 constexpr int kRedWaves = 16;
 
+// sum over t = first, first + step, ... < end of x[t * ld + p], in that order, with the loads of
+// eight terms issued before any is added (a plain loop waits for each load in turn).
+template <typename T>
+__device__ __forceinline__ T strided_sum(const T* __restrict__ x, int64_t ld, int p, int first, int end, int step) {
+    T s = T(0);
+    for (int t0 = first; t0 < end; t0 += 8 * step) {
+        T v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int t = t0 + k * step;
+            v[k] = t < end ? x[(int64_t)t * ld + p] : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (t0 + k * step < end) s += v[k];
+    }
+    return s;
+}
+
 template <typename T>
 __global__ __launch_bounds__(64 * kRedWaves) void k_wake_reduce(T* __restrict__ rhs, int64_t rhs_stride,
                                                                 const T* __restrict__ partial, int tiles, int total) {
@@ -237,8 +256,7 @@ __global__ __launch_bounds__(64 * kRedWaves) void k_wake_reduce(T* __restrict__ 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int p = blockIdx.x * 64 + lane;
     T s = T(0);
-    if (p < total)
-        for (int t = wave; t < tiles; t += kRedWaves) s += partial[(int64_t)t * total + p];
+    if (p < total) s = strided_sum(partial, (int64_t)total, p, wave, tiles, kRedWaves);
     red[wave][lane] = s;
     __syncthreads();
 #pragma unroll
@@ -447,6 +465,49 @@ inline void lattice_tile_shape(int ns, int rows, int recv_blocks, int& sw, int& 
     if (want_row_tiles < 1) want_row_tiles = 1;
     const int need = (rows + want_row_tiles - 1) / want_row_tiles;
     rh = need <= 1 ? 1 : need <= 2 ? 2 : 4;
+}
+
+// Span extent of a lattice tile when the launch's spans are split `split` ways: ceil(ns / split)
+// spans, at least 2 per wave (or all ns), at most kWaves * kLatChunkMax.
+inline int lattice_span_tile(int ns, int split) {
+    int sw = (ns + split - 1) / split;
+    const int lo = ns < 2 * kWaves ? ns : 2 * kWaves;
+    if (sw < lo) sw = lo;
+    if (sw > kWaves * kLatChunkMax) sw = kWaves * kLatChunkMax;
+    return sw;
+}
+
+// One shape for every surface of a k_wake_lattice launch (rows per tile rh, the kernel's template
+// value, and a span split): a launch of more than `target` work-groups with one row per tile takes
+// 2 or 4 rows per tile (the smallest that gets within the target, else 4); a smaller launch splits
+// its spans (2, 3, ... ways, at least 2 spans per wave) until it reaches the target or cannot split
+// further, so that a short wake still spreads over the CUs.  A split launch holds fewer than
+// 2 target work-groups.
+template <typename T>
+inline void lattice_launch_shape(const WakeTable<T>& wt, int recv_blocks, int target, int& rh, int& split) {
+    auto groups = [&](int rh_, int k) {
+        int64_t g = 0;
+        for (int m = 0; m < wt.n; m++) {
+            const WakeSurface<T>& w = wt.s[m];
+            const int sw = lattice_span_tile(w.ns, k);
+            g += (int64_t)recv_blocks * ((w.ns + sw - 1) / sw) * ((w.row_end - w.row_begin + rh_ - 1) / rh_);
+        }
+        return g;
+    };
+    rh = 1;
+    split = 1;
+    if (groups(1, 1) > target) {
+        rh = groups(2, 1) <= target ? 2 : 4;
+        return;
+    }
+    for (int k = 2; k <= 64 && groups(1, split) < target; k++) {
+        bool finer = false;  // does k split any surface finer than the current split?
+        for (int m = 0; m < wt.n; m++)
+            if (lattice_span_tile(wt.s[m].ns, k) < lattice_span_tile(wt.s[m].ns, split)) finer = true;
+        if (!finer) continue;
+        if (groups(1, k) > 2 * (int64_t)target) break;
+        split = k;
+    }
 }
 
 }  // namespace

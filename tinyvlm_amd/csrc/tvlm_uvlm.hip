@@ -99,7 +99,18 @@ __global__ __launch_bounds__(256) void k_gemv_part(const double* __restrict__ A,
     const int c0 = blockIdx.y * cbs, c1 = min(n, c0 + cbs);
     double s = 0.0;
     if (row < n)
-        for (int c = c0 + wave; c < c1; c += 4) s += A[row + (int64_t)c * lda] * x[c];
+        for (int cb0 = c0 + wave; cb0 < c1; cb0 += 4 * 8) {  // eight loads in flight, summed in order
+            double a[8], xv[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int c = cb0 + 4 * k;
+                a[k] = c < c1 ? A[row + (int64_t)c * lda] : 0.0;
+                xv[k] = c < c1 ? x[c] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (cb0 + 4 * k < c1) s += a[k] * xv[k];
+        }
     red[wave][lane] = s;
     __syncthreads();
     if (wave == 0 && row < n) part[(int64_t)blockIdx.y * n + row] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
@@ -146,11 +157,7 @@ __device__ __forceinline__ void post_step(const UTab& tab, const double* __restr
                                           double (*lds)[6]) {
     const int t = threadIdx.x;
     const int N = tab.N;
-    auto gsum = [&](int P) {
-        double g = 0.0;
-        for (int c = 0; c < cb; c++) g += gsrc[(int64_t)c * N + P];
-        return g;
-    };
+    auto gsum = [&](int P) { return strided_sum(gsrc, (int64_t)N, P, 0, cb, 1); };
     if constexpr (!FROM_PARTIALS) {
         for (int P = t; P < N; P += NT) gamma[P] = gsum(P);
         __syncthreads();
@@ -410,8 +417,7 @@ __global__ __launch_bounds__(64 * kRedWaves) void k_wake_reduce_base(double* __r
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int p = blockIdx.x * 64 + lane;
     double s = 0.0;
-    if (p < total)
-        for (int t = wave; t < tiles; t += kRedWaves) s += partial[(int64_t)t * total + p];
+    if (p < total) s = strided_sum(partial, (int64_t)total, p, wave, tiles, kRedWaves);
     red[wave][lane] = s;
     __syncthreads();
 #pragma unroll
@@ -433,6 +439,12 @@ __global__ void k_eye(double* a, int n) {
 }
 
 inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+inline int64_t off_total(const int64_t* ns, const int64_t* nc, int n_surf) {
+    int64_t n = 0;
+    for (int m = 0; m < n_surf; m++) n += ns[m] * nc[m];
+    return n;
+}
 
 }  // namespace
 }  // namespace tvlm
@@ -462,6 +474,7 @@ struct tvlm_uvlm_plan {
     // far-wake aggregation (fast mode): levels of age threshold far_A, row group far_G, span group far_S
     int32_t far_n = 0;
     int32_t far_A[8] = {}, far_G[8] = {}, far_S[8] = {};
+    int32_t lat_target = kLatTargetGroups;  // work-group target of the fast mode's wake launches
 
     int alloc(void** p, size_t bytes) {
         TVLM_HIP(hipMalloc(p, bytes));
@@ -636,14 +649,16 @@ int launch_wake(tvlm_uvlm_plan* P, int i, int& tiles) {
     TVLM_REQUIRE((size_t)tiles * P->N <= P->wpart_elems, "wake partial buffer too small");
     dim3 grid((P->N + kRecv - 1) / kRecv, tiles);
     if (P->mode == 1) {
-        // one rows-per-tile value for every surface of the launch: the largest chosen
-        int rh = 1;
-        for (int m = 0; m < wt.n; m++) rh = wt.s[m].rh > rh ? wt.s[m].rh : rh;
-        for (int m = 0; m < wt.n; m++) wt.s[m].rh = rh;
+        int rh = 1, split = 1;
+        lattice_launch_shape(wt, (P->N + kRecv - 1) / kRecv, P->lat_target, rh, split);
         int t2 = 0;
         for (int m = 0; m < wt.n; m++) {
-            wt.s[m].tile_begin = t2;
-            t2 += wt.s[m].tiles_span * ((wt.s[m].row_end - wt.s[m].row_begin + rh - 1) / rh);
+            WakeSurface<double>& w = wt.s[m];
+            w.rh = rh;
+            w.sw = lattice_span_tile(w.ns, split);
+            w.tiles_span = (w.ns + w.sw - 1) / w.sw;
+            w.tile_begin = t2;
+            t2 += w.tiles_span * ((w.row_end - w.row_begin + rh - 1) / rh);
         }
         wt.total_tiles = t2;
         tiles = t2;
@@ -801,6 +816,12 @@ int tvlm_uvlm_create(tvlm_uvlm_plan** out, int32_t n_surf, const int64_t* ns, co
         const int sw = ns[m] < kMaxSpanTile ? (int)ns[m] : kMaxSpanTile;
         max_tiles += (size_t)((ns[m] + sw - 1) / sw) * (size_t)t_steps;  // worst case: one row per tile
     }
+    {
+        // fast mode: a launch whose spans are split holds fewer than 2 kLatTargetGroups work-groups
+        const size_t recv_blocks = (size_t)((off_total(ns, nc, n_surf) + kRecv - 1) / kRecv);
+        const size_t split_tiles = 2 * (size_t)kLatTargetGroups / recv_blocks + (size_t)kMaxWakes;
+        if (mode == 1 && split_tiles > max_tiles) max_tiles = split_tiles;
+    }
     tab.N = off;
     tab.NV = voff;
     tab.NTE = teoff;
@@ -945,6 +966,34 @@ int tvlm_uvlm_set_far_wake(tvlm_uvlm_plan* P, int32_t n_levels, const int32_t* l
         P->far_G[l] = levels[3 * l + 1];
         P->far_S[l] = levels[3 * l + 2];
     }
+    return TVLM_OK;
+}
+
+// Work-group target of the fast mode's wake launches (lattice_launch_shape); grows the tile
+// partial buffer when a larger target needs it.  Drops a captured graph.
+int tvlm_uvlm_set_lattice_target(tvlm_uvlm_plan* P, int32_t groups) {
+    TVLM_REQUIRE(P, "null plan");
+    TVLM_REQUIRE(P->mode == 1, "the lattice target applies to mode 1");
+    TVLM_REQUIRE(groups >= 64 && groups <= 65536, "groups %d out of range [64, 65536]", groups);
+    const size_t recv_blocks = (size_t)((P->N + kRecv - 1) / kRecv);
+    const size_t need = (2 * (size_t)groups / recv_blocks + (size_t)kMaxWakes) * (size_t)P->N;
+    if (P->exec) {
+        TVLM_HIP(hipStreamSynchronize(P->stream));
+        TVLM_HIP(hipGraphExecDestroy(P->exec));
+        P->exec = nullptr;
+    }
+    if (P->graph) {
+        TVLM_HIP(hipGraphDestroy(P->graph));
+        P->graph = nullptr;
+    }
+    if (need > P->wpart_elems) {
+        TVLM_HIP(hipStreamSynchronize(P->stream));
+        double* w = nullptr;
+        TVLM_TRY(P->alloc((void**)&w, sizeof(double) * need));
+        P->wpart = w;  // the smaller buffer stays in the plan's allocations until destroy
+        P->wpart_elems = need;
+    }
+    P->lat_target = groups;
     return TVLM_OK;
 }
 

@@ -70,6 +70,12 @@ class UVLMPlan:
         H.check(H.lib().tvlm_uvlm_set_far_wake(self._plan, ctypes.c_int32(len(lv)), arr), "tvlm_uvlm_set_far_wake")
         self.far_wake = lv
 
+    def set_lattice_target(self, groups):
+        """Work-group target of the fast mode's wake launches (tvlm_uvlm_set_lattice_target)."""
+        H.check(H.lib().tvlm_uvlm_set_lattice_target(self._plan, ctypes.c_int32(int(groups))),
+                "tvlm_uvlm_set_lattice_target")
+        self.lattice_target = int(groups)
+
     def set_pitch_heave(self, u, h_amp=0.0, h_phase=0.0, a_amp=0.0, a_phase=0.0, omega=0.0, x_pivot=0.25):
         d = ctypes.c_double
         H.check(H.lib().tvlm_uvlm_set_pitch_heave(self._plan, d(u), d(h_amp), d(h_phase), d(a_amp), d(a_phase),
