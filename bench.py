@@ -392,29 +392,50 @@ def assembly_bench(reps=5):
     ns, nc = v.shape[0] - 1, v.shape[1] - 1
     N = ns * nc
     out = {"workload": "lhs_assemble_elliptic_64x64_N4096", "pairs": N * N, "flops_per_pair": FLOPS_PER_PAIR}
-    for name, dt, peak in (("f32", torch.float32, PEAK_FP64_VALU_TFLOPS), ("f64", torch.float64, PEAK_FP64_VALU_TFLOPS)):
+    exact64 = None
+    for name, dt, peak in (("f32", torch.float32, PEAK_FP64_VALU_TFLOPS), ("f64", torch.float64, PEAK_FP64_VALU_TFLOPS),
+                           ("f64_seeded", torch.float64, PEAK_FP64_VALU_TFLOPS)):
         verts = [H.to_device(v, dt)]
         colloc, normals, areas = [H.colmajor((ns, nc, 3), dt)], [H.colmajor((ns, nc, 3), dt)], [H.colmajor((ns, nc), dt)]
         H.mesh_metrics(0.0, verts, colloc, normals, areas)
         wake = [H.colmajor((ns + 1, 2, 4), dt)]
         lhs = H.colmajor((N, N), dt)
-        H.lhs_assemble(lhs, colloc, normals, verts, wake, 0)
+
+        def assemble():
+            lhs.zero_()
+            if name == "f64_seeded":
+                H.lhs_assemble_seeded(lhs, colloc, normals, verts, wake, 0)
+            else:
+                H.lhs_assemble(lhs, colloc, normals, verts, wake, 0)
+
+        assemble()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        z0, z1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        z0.record()
+        for _ in range(reps):
+            lhs.zero_()
+        z1.record()
         e0.record()
         for _ in range(reps):
-            H.lhs_assemble(lhs, colloc, normals, verts, wake, 0)
+            assemble()
         e1.record()
         torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
+        ms = (e0.elapsed_time(e1) - z0.elapsed_time(z1)) / reps  # the zeroing fills timed alone, subtracted
         tf = N * N * FLOPS_PER_PAIR / (ms * 1e-3) / 1e12
         out[name] = {"ms": round(ms, 4), "achieved_tflops": round(tf, 2), "peak_tflops": peak,
                      "frac": round(tf / peak, 4)}
+        if name == "f64":
+            exact64 = lhs.clone()
+        if name == "f64_seeded":
+            out[name]["max_rel_diff_vs_exact_f64"] = float((lhs - exact64).abs().max() / exact64.abs().max())
+            out[name]["note"] = "the kernel the served steady path assembles with (tvlm_lhs_assemble_seeded_f64)"
         if name == "f32":
             out[name]["frac_of_packed_fp32_peak"] = round(tf / PEAK_FP32_VALU_PACKED_TFLOPS, 4)
         del lhs
     out["note"] = ("FP32 rsqrt form against the non-packed FP32 VALU rate (78.6 TFLOP/s: packed FP32 "
-                   "instructions are switched off in every kernel); FP64 exact form against the FP64 VALU peak")
+                   "instructions are switched off in every kernel); FP64 exact form and the FP64 seeded form (the "
+                   "served steady path's) against the FP64 VALU peak")
     return out
 
 
@@ -734,6 +755,12 @@ def main():
                           "copy_bytes_each_way": COPY_PROBE_BYTES, "cl_last": float(fast.results()["cl"][-1])}))
         fast.close()
         return
+    # tile-size autotuner: the wake launches' work-group target for this workload (cached in memory;
+    # the choice also becomes the library-wide default that the served_uvlm section's plans take)
+    from tinyvlm_amd import autotune as AT
+
+    fast.set_pitch_heave(**MOTION)
+    tuned = AT.tune_uvlm(fast, cache=AT.Cache(None), apply_globally=True)
     _, _, _ = run_plan(fast, max(args.warmup, 1), use_graph=True)
     wall, gpu_s, lat = run_plan(fast, args.steps, use_graph=True)
     res_fast = fast.results()
@@ -792,6 +819,8 @@ def main():
     exact_sec = None
     if not args.exact_wake:
         ex = UVLMPlan([verts], T_STEPS, dt, mode=MODE_FAST)
+        ex.set_pitch_heave(**MOTION)
+        tuned_x = AT.tune_uvlm(ex, cache=AT.Cache(None))
         run_plan(ex, 1, use_graph=True)
         wall_x, _, lat_x = run_plan(ex, args.steps, use_graph=True)
         res_x = ex.results()
@@ -800,7 +829,8 @@ def main():
                      "speedup_vs_plain_fp64": round(value_x / plain_value, 3),
                      "max_abs_dCL_vs_plain_fp64": float(np.max(np.abs(res_x["cl"] - res_plain["cl"]))),
                      "latency_ms_per_sim_p50": round(float(np.percentile(lat_x, 50)), 4),
-                     "roofline": counters(wake_roofline(ex, wall_x / args.steps), False)}
+                     "roofline": counters(wake_roofline(ex, wall_x / args.steps), False),
+                     "autotune": {"lattice_target": tuned_x["best"], "sim_ms": tuned_x["sim_ms"]}}
         ex.close()
 
     out = {
@@ -834,6 +864,8 @@ def main():
         "gpu_event_s": round(gpu_s, 6),
         "roofline": roofline,
     }
+    out["autotune"] = {"lattice_target": tuned["best"], "sim_ms_per_candidate": tuned["sim_ms"],
+                       "note": "tinyvlm_amd.autotune: whole simulations per work-group target; the fastest is used"}
     if exact_sec is not None:
         out["served_exact_wake"] = exact_sec
     if not args.no_cpu:

@@ -189,6 +189,21 @@ int tvlm_lu_update(int32_t elem_bytes, void* c, int64_t ldc, const void* a, int6
  * wait that gives up never hangs or faults; it leaves that matrix's factors invalid and info -999. */
 int tvlm_lu_set_coop_spin_limit(uint32_t limit);
 
+/* Library-wide work-group targets of the wake-induction launches (no counterpart in the reference;
+ * tuning knobs that tinyvlm_amd.autotune sets from its cached measurements): wake_groups for the
+ * ring-by-ring form (Backend::rhs_assemble_wake_influence and the plain UVLM loop, default 2048),
+ * lattice_groups for UVLM plans created afterwards (fast mode, default 1024; per plan:
+ * tvlm_uvlm_set_lattice_target).  0 restores a default; range [64, 65536]. */
+int tvlm_set_tile_targets(int32_t wake_groups, int32_t lattice_groups);
+int tvlm_get_tile_targets(int32_t* wake_groups, int32_t* lattice_groups);
+
+/* Backend::lhs_assemble (vlm_backend.hpp:44) in the FP64 seeded form the served steady path uses
+ * (FP64 arithmetic; 1/sqrt and 1/x from the FP64 hardware estimates plus one Newton step each; no
+ * core model, sigma = 0): same arguments and result as tvlm_lhs_assemble_f64 to about 1e-15
+ * relative.  No counterpart in the reference, which assembles in FP32 or exact FP64. */
+int tvlm_lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc,
+                                 const tvlm_view* normals, const tvlm_view* verts_wing, const tvlm_view* verts_wake,
+                                 int32_t iteration, void* stream);
 /* Allocation-free form of tvlm_rhs_assemble_wake_influence_f64 (same arguments, same kernels, same
  * result bit for bit).  No counterpart in the reference: it is for callers that cannot allocate
  * device memory -- code under HIP graph capture, tight serving loops, harnesses that forbid

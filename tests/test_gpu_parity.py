@@ -93,6 +93,13 @@ def test_metrics_lhs_solve_vs_golden(H, torch, golden, oracle, prec, name, nsurf
     oracle.mesh_metrics(alpha, osurf)
     Lo = oracle.lhs_assemble(osurf, 1)
     assert rel(L, Lo) < (1e-13 if prec == "f64" else 2e-5)
+    if prec == "f64":
+        # the served steady path's seeded form (hardware 1/sqrt and 1/x estimates + one Newton step)
+        # assembles the same matrix to FP64 accuracy
+        lhs_s = H.colmajor((N, N), dt)
+        H.lhs_assemble_seeded(lhs_s, [s["colloc"] for s in S], [s["normals"] for s in S], [s["verts"] for s in S],
+                              [s["wake"] for s in S], 1)
+        assert rel(H.to_host(lhs_s), gl) < 1e-13
     # solve with the freestream rhs
     fs = [math.cos(alpha), 0.0, math.sin(alpha)]
     vel = []

@@ -179,6 +179,17 @@ def lhs_assemble(lhs, colloc, normals, verts_wing, verts_wake, iteration, sigma=
                                   _real(lhs)(sigma), stream_ptr(stream)), "lhs_assemble")
 
 
+def lhs_assemble_seeded(lhs, colloc, normals, verts_wing, verts_wake, iteration, stream=None):
+    """FP64 seeded-form assembly (tvlm_lhs_assemble_seeded_f64), the served steady path's kernel."""
+    import torch
+
+    if lhs.dtype != torch.float64:
+        raise TypeError("the seeded form is FP64")
+    check(lib().tvlm_lhs_assemble_seeded_f64(ctypes.byref(view(lhs)), ctypes.c_int32(len(colloc)), views(colloc),
+                                             views(normals), views(verts_wing), views(verts_wake),
+                                             ctypes.c_int32(iteration), stream_ptr(stream)), "lhs_assemble_seeded")
+
+
 def rhs_assemble_velocities(rhs, normals, velocities, stream=None):
     check(_fn("rhs_assemble_velocities", rhs)(ctypes.byref(view(rhs)), ctypes.c_int32(len(normals)), views(normals),
                                              views(velocities), stream_ptr(stream)), "rhs_assemble_velocities")

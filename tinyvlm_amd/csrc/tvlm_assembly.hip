@@ -231,6 +231,12 @@ int lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_vie
 
 extern "C" {
 
+int tvlm_lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc,
+                                 const tvlm_view* normals, const tvlm_view* verts_wing, const tvlm_view* verts_wake,
+                                 int32_t iteration, void* stream) {
+    return tvlm::lhs_assemble_seeded_f64(lhs, n_surf, colloc, normals, verts_wing, verts_wake, iteration, stream);
+}
+
 int tvlm_lhs_assemble_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, const tvlm_view* normals,
                           const tvlm_view* verts_wing, const tvlm_view* verts_wake, int32_t iteration, double sigma,
                           void* stream) {

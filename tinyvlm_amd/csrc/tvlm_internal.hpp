@@ -75,7 +75,15 @@ int getrf_single_group_f64(const tvlm_view* a, int32_t* ipiv, int32_t* info, voi
 int getrf_single_group_f32(const tvlm_view* a, int32_t* ipiv, int32_t* info, void* stream);
 constexpr int32_t kGetrfStalled = -999;  // info of a matrix whose cooperative panel gave up waiting
 
-// lhs_assemble in the FP64 seeded form (not part of the C ABI; the served steady solve uses it)
+// Work-group targets of the wake-induction launches, library-wide (tvlm_set_tile_targets): the
+// ring-by-ring form's tile shape (wake_tile_shape) and new UVLM plans' lattice launches.
+constexpr int kTargetGroups = 2048;     // default of wake_target_groups()
+constexpr int kLatTargetGroups = 1024;  // default of lattice_target_groups()
+int wake_target_groups();
+int lattice_target_groups();
+
+// lhs_assemble in the FP64 seeded form (the served steady solve uses it; exported as
+// tvlm_lhs_assemble_seeded_f64)
 int lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, const tvlm_view* normals,
                             const tvlm_view* verts_wing, const tvlm_view* verts_wake, int32_t iteration, void* stream);
 

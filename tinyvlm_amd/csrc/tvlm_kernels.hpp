@@ -274,16 +274,15 @@ inline void launch_wake_reduce(T* rhs, int64_t rhs_stride, const T* partial, int
 }
 
 // Tile shape of one wake surface: spans per tile sw <= kMaxSpanTile, rows per tile rh with
-// rh * sw <= kTileRings, and small enough that the launch has about kTargetGroups work-groups
-// (receiver blocks x tiles) so every SIMD holds several waves.
-constexpr int kTargetGroups = 2048;
+// rh * sw <= kTileRings, and small enough that the launch has about wake_target_groups()
+// work-groups (receiver blocks x tiles; default kTargetGroups) so every SIMD holds several waves.
 
 inline void wake_tile_shape(int ns, int rows, int recv_blocks, int& sw, int& rh) {
     sw = ns < kMaxSpanTile ? ns : kMaxSpanTile;
     const int tiles_span = (ns + sw - 1) / sw;
     int rh_max = kTileRings / sw;
     if (rh_max < 1) rh_max = 1;
-    int want_row_tiles = kTargetGroups / (recv_blocks * tiles_span);
+    int want_row_tiles = wake_target_groups() / (recv_blocks * tiles_span);
     if (want_row_tiles < 1) want_row_tiles = 1;
     rh = (rows + want_row_tiles - 1) / want_row_tiles;
     if (rh < 1) rh = 1;
@@ -456,7 +455,6 @@ __global__ __launch_bounds__(kThreads) void k_wake_lattice(TS* __restrict__ part
 
 // Tile shape for k_wake_lattice: sw <= 4 * kLatChunkMax spans, rh in {1, 2, 4} rows, chosen so
 // that the launch has about kLatTargetGroups work-groups.
-constexpr int kLatTargetGroups = 1024;
 
 inline void lattice_tile_shape(int ns, int rows, int recv_blocks, int& sw, int& rh) {
     sw = ns < kWaves * kLatChunkMax ? ns : kWaves * kLatChunkMax;

@@ -4,6 +4,7 @@
 // column-major planes read with unit stride.  Reductions (CL, CM, MAC, sums, norms) run in one
 // work-group with a fixed summation tree so they are bitwise reproducible run to run; the CUDA
 // reference uses float atomics there (vlm_backend_cuda_kernels.cuh:89, :451, :501-503).
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <mutex>
@@ -443,9 +444,32 @@ int forces(const tvlm_view* verts, const tvlm_view* gd, const tvlm_view* gdt, co
 
 }  // namespace tvlm
 
+namespace tvlm {
+namespace {
+std::atomic<int> g_wake_target{kTargetGroups}, g_lattice_target{kLatTargetGroups};
+}  // namespace
+int wake_target_groups() { return g_wake_target.load(std::memory_order_relaxed); }
+int lattice_target_groups() { return g_lattice_target.load(std::memory_order_relaxed); }
+}  // namespace tvlm
+
 using namespace tvlm;
 
 extern "C" {
+
+int tvlm_set_tile_targets(int32_t wake_groups, int32_t lattice_groups) {
+    TVLM_REQUIRE((wake_groups == 0 || (wake_groups >= 64 && wake_groups <= 65536)) &&
+                     (lattice_groups == 0 || (lattice_groups >= 64 && lattice_groups <= 65536)),
+                 "targets %d / %d out of range [64, 65536] (0: default)", wake_groups, lattice_groups);
+    g_wake_target.store(wake_groups ? wake_groups : kTargetGroups);
+    g_lattice_target.store(lattice_groups ? lattice_groups : kLatTargetGroups);
+    return TVLM_OK;
+}
+
+int tvlm_get_tile_targets(int32_t* wake_groups, int32_t* lattice_groups) {
+    if (wake_groups) *wake_groups = wake_target_groups();
+    if (lattice_groups) *lattice_groups = lattice_target_groups();
+    return TVLM_OK;
+}
 
 const char* tvlm_last_error(void) { return g_last_error.c_str(); }
 const char* tvlm_backend_name(void) { return "hip"; }

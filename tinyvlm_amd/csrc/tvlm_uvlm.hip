@@ -474,7 +474,7 @@ struct tvlm_uvlm_plan {
     // far-wake aggregation (fast mode): levels of age threshold far_A, row group far_G, span group far_S
     int32_t far_n = 0;
     int32_t far_A[8] = {}, far_G[8] = {}, far_S[8] = {};
-    int32_t lat_target = kLatTargetGroups;  // work-group target of the fast mode's wake launches
+    int32_t lat_target = lattice_target_groups();  // work-group target of the fast mode's wake launches
 
     int alloc(void** p, size_t bytes) {
         TVLM_HIP(hipMalloc(p, bytes));
@@ -819,7 +819,7 @@ int tvlm_uvlm_create(tvlm_uvlm_plan** out, int32_t n_surf, const int64_t* ns, co
     {
         // fast mode: a launch whose spans are split holds fewer than 2 kLatTargetGroups work-groups
         const size_t recv_blocks = (size_t)((off_total(ns, nc, n_surf) + kRecv - 1) / kRecv);
-        const size_t split_tiles = 2 * (size_t)kLatTargetGroups / recv_blocks + (size_t)kMaxWakes;
+        const size_t split_tiles = 2 * (size_t)lattice_target_groups() / recv_blocks + (size_t)kMaxWakes;
         if (mode == 1 && split_tiles > max_tiles) max_tiles = split_tiles;
     }
     tab.N = off;
