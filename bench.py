@@ -64,9 +64,11 @@ COPY_PROBE_BYTES = 1 << 30
 # FETCH_SIZE / WRITE_SIZE counter passes over `bench.py --probe`, each scaled by the same counter's
 # reading for a 1 GiB device copy (FETCH x 2.000, WRITE x 1.000); and the executed FP64 flops per
 # pair from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 pass.  Counters cannot be read in-process.
+# "exact": the served loop without far-wake aggregation, measured when each launch still chose its
+# rows per tile alone (38.6 us per launch then, 38.0 us with the joint launch shape).
 WAKE_COUNTERS = {
     "exact": {"traffic_bytes": 3008144, "algorithmic_bytes": 349384, "executed_flops_per_pair": 252.7},
-    "far": None,
+    "far": {"traffic_bytes": 2465922, "algorithmic_bytes": 188327, "executed_flops_per_pair": 302.5},
 }
 NS, NC, HALF_SPAN, CHORD = 40, 10, 2.0, 1.0
 T_STEPS = 501  # UVLM::run loops t_steps-1 times -> 500 time steps
