@@ -179,24 +179,33 @@ __device__ __forceinline__ void post_step(const UTab& tab, const double* __restr
         double a_acc = 0.0, c_acc = 0.0, mx = 0.0, my = 0.0, mz = 0.0, mac_acc = 0.0;
         for (int p = t; p < np; p += NT) {
             const int i = p % ns, j = p / ns;
+            // every operand of the panel is loaded before the first store (the stores below go
+            // through pointers the compiler must assume may alias them), so the loads overlap
             const double g = FROM_PARTIALS ? gsum(S.off + p) : gamma[S.off + p];
-            const double gd = j > 0 ? g + (-1.0) * (FROM_PARTIALS ? gsum(S.off + p - ns) : gamma[S.off + p - ns]) : g;
+            const double gup = j > 0 ? (FROM_PARTIALS ? gsum(S.off + p - ns) : gamma[S.off + p - ns]) : 0.0;
+            const double gprev = S.gprev[p];
+            const int64_t a = i + (int64_t)j * vs1, b = a + 1;
+            v3<double> U{}, va{}, vb{}, nrm{};
+            double area = 0.0;
+            if (forces) {
+                U = {S.vel[p], S.vel[p + np], S.vel[p + 2 * np]};
+                va = {S.verts[a], S.verts[a + vs2], S.verts[a + 2 * vs2]};
+                vb = {S.verts[b], S.verts[b + vs2], S.verts[b + 2 * vs2]};
+                nrm = {S.normals[p], S.normals[p + np], S.normals[p + 2 * np]};
+                area = S.areas[p];
+            }
+            const double gd = j > 0 ? g + (-1.0) * gup : g;
             if constexpr (FROM_PARTIALS) gamma[S.off + p] = g;
-            const double gt = (g + (-1.0) * S.gprev[p]) * inv_dt;
+            const double gt = (g + (-1.0) * gprev) * inv_dt;
             S.gdelta[p] = gd;
             S.gdt[p] = gt;
             S.gprev[p] = g;
             if (j == nc - 1) S.gamma_wake[i + (int64_t)(tab.t_steps - 2 - step) * ns] = g;
             if (forces) {
-                const int64_t a = i + (int64_t)j * vs1, b = a + 1;
-                const v3<double> U = {S.vel[p], S.vel[p + np], S.vel[p + 2 * np]};
-                const v3<double> va = {S.verts[a], S.verts[a + vs2], S.verts[a + 2 * vs2]};
-                const v3<double> vb = {S.verts[b], S.verts[b + vs2], S.verts[b + 2 * vs2]};
                 const v3<double> cr = cross(U, vb - va);
                 const double s1 = rho * gd;
-                const double u = rho * gt * S.areas[p];
-                const v3<double> F = {s1 * cr.x + u * S.normals[p], s1 * cr.y + u * S.normals[p + np],
-                                      s1 * cr.z + u * S.normals[p + 2 * np]};
+                const double u = rho * gt * area;
+                const v3<double> F = {s1 * cr.x + u * nrm.x, s1 * cr.y + u * nrm.y, s1 * cr.z + u * nrm.z};
                 S.forces[p] = F.x;
                 S.forces[p + np] = F.y;
                 S.forces[p + 2 * np] = F.z;
@@ -207,7 +216,7 @@ __device__ __forceinline__ void post_step(const UTab& tab, const double* __restr
                 mx += mo.x;
                 my += mo.y;
                 mz += mo.z;
-                a_acc += S.areas[p];
+                a_acc += area;
             }
         }
         if (forces) {
