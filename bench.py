@@ -456,7 +456,51 @@ HB_OMEGA = 0.8
 HB_U_REF = 6.0
 
 
-def hb_bench(n_speeds, with_cpu):
+def hb_served(wing, flap, X, n_requests, workers):
+    """The harmonic-balance configuration served: n_requests airspeeds U = linspace(0.5, 1.5) U_ref as
+    concurrent cold-start requests through Server on `workers` HB worker threads (one HBVLM and stream
+    each), against the same cold solves one after another on one solver."""
+    from tinyvlm_amd import serve as S
+    from tinyvlm_amd.hb import HBVLM
+
+    K = 2 * HB_H + 1
+    speeds = np.linspace(0.5, 1.5, n_requests) * HB_U_REF
+    cfg = {"wf": {"meshes": [wing, flap], "harmonics": HB_H, "scaling": 1.0 / HB_B}}
+
+    def req(i, U):
+        return S.Request(i, "wf", kind="hb", hb={"omega": HB_OMEGA, "U": float(U), "dyn": X})
+
+    # one after another, cold (a fresh start for every airspeed): a sample of the same requests
+    seq = HBVLM("hip", [wing, flap])
+    seq.init(HB_H, 1.0 / HB_B)
+    F = np.zeros((3, K))
+    seq.run(HB_OMEGA, float(speeds[0]), X, F)  # warm-up
+    sample = speeds[:: max(1, n_requests // 8)]
+    seq_F = {}
+    t0 = time.perf_counter()
+    for U in sample:
+        seq.reset()
+        F = np.zeros((3, K))
+        seq.run(HB_OMEGA, float(U), X, F)
+        seq_F[float(U)] = F
+    seq_rate = len(sample) / (time.perf_counter() - t0)
+    reqs = [req(i, U) for i, U in enumerate(speeds)]
+    with S.Server({}, lanes=1, hb=cfg, hb_workers=workers) as srv:
+        srv.run([req(-1 - k, speeds[0]) for k in range(2 * workers)], realtime=False)  # solvers, scratch
+        t0 = time.perf_counter()
+        srv.run(reqs, realtime=False)
+        wall = time.perf_counter() - t0
+    lat = [r.latency * 1e3 for r in reqs]
+    same = all(np.array_equal(r.forces, seq_F[r.hb["U"]]) for r in reqs if r.hb["U"] in seq_F)
+    return {"requests": n_requests, "workers": workers, "value": round(n_requests / wall, 3), "unit": "HB solves/s",
+            "sequential_cold": {"value": round(seq_rate, 3), "unit": "HB solves/s", "solves": len(sample)},
+            "speedup_vs_sequential_cold": round(n_requests / wall / seq_rate, 3),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1), "p99": round(float(np.percentile(lat, 99)), 1)},
+            "anderson_iterations_mean": round(float(np.mean([r.iterations for r in reqs])), 1),
+            "served_equals_alone_bitwise": bool(same)}
+
+
+def hb_bench(n_speeds, with_cpu, n_served=64, workers=1):
     import tempfile
 
     import torch
@@ -497,6 +541,8 @@ def hb_bench(n_speeds, with_cpu):
         "anderson_iterations": iters, "dtype": "f64",
         "note": "continuation: each airspeed warm-started from the previous solution; the Anderson map (replayed from a hipGraph) is bound by its 11 exact-form FP64 wake inductions",
     }
+    if n_served:
+        out["served"] = hb_served(wing, flap, X, n_served, workers)
     if with_cpu:
         from oracle import hb as ohb
         from oracle import cpuref as C
@@ -707,6 +753,9 @@ def main():
     ap.add_argument("--mix-only", action="store_true", help="only the served_mix section")
     ap.add_argument("--hb-speeds", type=int, default=8, help="airspeeds of the timed HB continuation")
     ap.add_argument("--no-hb", action="store_true")
+    ap.add_argument("--hb-served", type=int, default=64, help="airspeeds served as concurrent HB requests (0: none)")
+    ap.add_argument("--hb-workers", type=int, default=1, help="HB worker threads of the served run")
+    ap.add_argument("--hb-only", action="store_true", help="only the hb section")
     ap.add_argument("--two-dof-steps", type=int, default=400, help="timed coupled pitch/heave steps")
     ap.add_argument("--two-dof-large-steps", type=int, default=100, help="timed steps of the 600-panel section")
     ap.add_argument("--no-two-dof", action="store_true")
@@ -734,6 +783,9 @@ def main():
         return
     if args.mix_only:
         print(json.dumps(serve_mix_bench(args.mix_span_hi, args.mix_span_lo, args.mix_saturation, args.mix_lanes)))
+        return
+    if args.hb_only:
+        print(json.dumps(hb_bench(args.hb_speeds, not args.no_cpu, args.hb_served, args.hb_workers)))
         return
     if args.flutter_only:
         print(json.dumps(flutter_bench(args.flutter_airspeeds, args.flutter_steps, 20, args.flutter_plain_steps,
@@ -901,7 +953,7 @@ def main():
         out["lu"] = lu
         out["served_mix"] = serve_mix_bench(args.mix_span_hi, args.mix_span_lo, args.mix_saturation, args.mix_lanes)
     if not args.no_hb:
-        out["hb"] = hb_bench(args.hb_speeds, not args.no_cpu)
+        out["hb"] = hb_bench(args.hb_speeds, not args.no_cpu, args.hb_served, args.hb_workers)
     if not args.no_two_dof:
         out["two_dof"] = two_dof_bench(args.two_dof_steps, not args.no_cpu)
         out["two_dof_600"] = two_dof_bench(args.two_dof_large_steps, not args.no_cpu, large=True)

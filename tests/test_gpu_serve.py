@@ -210,3 +210,47 @@ def test_large_mix_meshes_through_the_server(tmp_path):
         print(f"[case]   {r.mesh} rid {r.rid}: served vs plain |dCL| {abs(r.cl - plain['cl'][0]):.2e}", flush=True)
         assert np.max(np.abs(r.gamma - g)) <= TOL * np.max(np.abs(g)), r.rid
         assert abs(r.cl - plain["cl"][0]) <= TOL, r.rid
+
+
+def test_hb_requests_served_beside_steady(root, oracle):
+    """Harmonic-balance requests (the reference script's wing 9x5 + flap 3x5, H = 2, several
+    airspeeds) served on two worker threads beside steady requests: every request starts cold, so its
+    loads equal (bit for bit) the same solve on a fresh HBVLM alone; the steady results match their
+    alone solves as in the other mix tests."""
+    import os
+
+    from tinyvlm_amd import capi
+    from tinyvlm_amd import serve as S
+    from tinyvlm_amd.hb import HBVLM
+
+    capi.init(0)
+    paths = [os.path.join(root, "mesh", m) for m in ("3dof_wing_9x5.x", "3dof_flap_3x5.x")]
+    H, scale = 2, 1.0 / 0.127
+    K = 2 * H + 1
+    X = np.zeros((3, K))
+    X[0, 1] = 0.01 / 0.127
+    X[1, 2] = 0.02
+    X[2, 1] = 0.05
+    speeds = [5.0, 5.5, 6.0, 6.5, 7.0, 7.5]
+    print("[case] serve: hb requests beside steady", flush=True)
+    alone = {}
+    for U in speeds:
+        h = HBVLM("hip", paths)
+        h.init(H, scale)
+        h.use_graph = False  # as the serving loop's solvers (the replay gives the same bits anyway)
+        F = np.zeros((3, K))
+        h.run(0.8, U, X, F)
+        alone[U] = (F, h.iterations)
+    v = oracle.rectangular_wing(16, 8, chord=1.0, half_span=4.0)
+    reqs = [S.Request(i, "r16", float(np.deg2rad(a)), arrival=0.0) for i, a in enumerate((1.0, 2.0, 3.0, 4.0))]
+    reqs += [S.Request(100 + k, "wf", kind="hb", hb={"omega": 0.8, "U": U, "dyn": X}) for k, U in enumerate(speeds)]
+    with S.Server({"r16": [v]}, lanes=2, hb={"wf": {"meshes": paths, "harmonics": H, "scaling": scale}},
+                  hb_workers=4) as srv:
+        srv.run(reqs, realtime=False)
+    for r in reqs[4:]:
+        F, it = alone[r.hb["U"]]
+        assert np.array_equal(r.forces, F), (r.hb["U"], np.max(np.abs(r.forces - F)))
+        assert r.iterations == it
+    for r in reqs[:4]:
+        g, cl, cm, _ = oracle.vlm_run([v], r.alpha)
+        assert abs(r.cl - cl) <= 1e-10 * abs(cl)

@@ -236,3 +236,85 @@ def test_stalled_admission_raises_instead_of_spinning(S, monkeypatch):
         monkeypatch.setattr(srv, "_slot_for", lambda key, lane, now: None)
         with pytest.raises(RuntimeError, match="serving stalled"):
             srv.run(_requests(S, [("a", 0.0), ("a", 0.1)]), realtime=False)
+
+
+class FakeHB:
+    instances = []
+    lock = None
+
+    def __init__(self, backend, meshes):
+        import threading
+
+        assert backend == "hip" and len(meshes) == 2
+        self.thread = threading.get_ident()
+        self.stream = None  # torch.cuda.stream(None) is a no-op
+        self.use_graph = True
+        self.K = None
+        self.iterations = 0
+        self.warm = 0.0
+        FakeHB.instances.append(self)
+
+    def init(self, harmonics, scaling):
+        self.K = 2 * harmonics + 1
+
+    def reset(self):
+        self.warm = 0.0
+
+    def run(self, omega, U, dyn, force_t):
+        import time as _t
+
+        assert threading_ident() == self.thread  # a worker's solver stays in its thread
+        assert not self.use_graph  # served solvers never capture
+        _t.sleep(0.002)
+        force_t[0, :] = U + self.warm  # a result that would show a missing reset
+        force_t[1, :] = omega
+        force_t[2, :] = dyn[0, 1]
+        self.warm += 1.0  # the next run would be warm-started
+        self.iterations = 7
+        if U < 0:
+            raise RuntimeError("HB solve diverged")
+        return force_t
+
+
+def threading_ident():
+    import threading
+
+    return threading.get_ident()
+
+
+def test_hb_requests_run_on_workers_cold_and_beside_steady(S, monkeypatch):
+    import tinyvlm_amd.hb as hbmod
+
+    monkeypatch.setattr(hbmod, "HBVLM", FakeHB)
+    FakeHB.instances = []
+    meshes = {"a": _mesh(16)}
+    hbcfg = {"wf": {"meshes": ["w.x", "f.x"], "harmonics": 2, "scaling": 1.0}}
+    dyn = np.zeros((3, 5))
+    dyn[0, 1] = 0.1
+    reqs = [S.Request(i, "a", 0.01 * i, arrival=0.0) for i in range(10)]
+    reqs += [S.Request(100 + k, "wf", kind="hb", hb={"omega": 0.8, "U": 1.0 + k, "dyn": dyn}, arrival=0.0)
+             for k in range(9)]
+    with S.Server(meshes, lanes=2, max_batch=4, budget_bytes=1 << 40, hb=hbcfg, hb_workers=3) as srv:
+        srv.run(reqs, realtime=False)
+    for r in reqs[10:]:
+        # each request started cold (a stale warm start would add to the first row)
+        assert np.all(r.forces[0] == r.hb["U"]) and np.all(r.forces[1] == 0.8) and np.all(r.forces[2] == 0.1)
+        assert r.iterations == 7 and r.finished >= r.admitted >= 0
+    for r in reqs[:10]:
+        assert r.cl == pytest.approx(2 * np.pi * r.alpha)
+    assert 1 <= len(FakeHB.instances) <= 3  # one solver per worker thread and configuration
+
+
+def test_hb_request_errors(S, monkeypatch):
+    import tinyvlm_amd.hb as hbmod
+
+    monkeypatch.setattr(hbmod, "HBVLM", FakeHB)
+    hbcfg = {"wf": {"meshes": ["w.x", "f.x"], "harmonics": 1, "scaling": 1.0}}
+    with S.Server({"a": _mesh(8)}, lanes=1, budget_bytes=1 << 40, hb=hbcfg) as srv:
+        with pytest.raises(KeyError):
+            srv.run([S.Request(0, "nope", kind="hb", hb={"omega": 1, "U": 1, "dyn": np.zeros((3, 3))})])
+        with pytest.raises(KeyError):
+            srv.run([S.Request(0, "wf", kind="hb")])
+        with pytest.raises(RuntimeError, match="diverged"):
+            srv.run([S.Request(0, "wf", kind="hb", hb={"omega": 1, "U": -1, "dyn": np.zeros((3, 3))})],
+                    realtime=False)

@@ -203,6 +203,14 @@ class HBVLM:
         } for _ in range(Kc)]
         self.rhs_vel = z(N, Kc)
 
+    def reset(self):
+        """Forget the warm start: the next run starts from zero Fourier coefficients, as the first
+        run after init does (a served request is then independent of the ones before it)."""
+        if self.harmonics is None:
+            raise RuntimeError("HBVLM.init must be called before reset")
+        self.gamma_coeffs.zero_()
+        self.iterations = 0
+
     # ---- the binding's run (lib/libhbvlm3.cpp:552-641) -----------------------------------------
     def run(self, omega: float, U: float, dyn_f, force_t):
         """force_t (numpy, 3 x K) receives CL, CM_y about the elastic axis and the flap's CM_y
@@ -326,7 +334,10 @@ class HBVLM:
                 return
             if state["graph"] is None:
                 g = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(g, stream=self.stream):
+                # thread-local capture: other threads keep calling the runtime while this one
+                # captures (the capture itself first synchronizes the device, so concurrent solvers in
+                # other threads run with use_graph = False, as the serving loop's do)
+                with torch.cuda.graph(g, stream=self.stream, capture_error_mode="thread_local"):
                     hb_vlm_iter(x_in, x_out)
                 state["graph"], state["ptrs"] = g, ptrs
             state["graph"].replay()
