@@ -277,6 +277,27 @@ int tvlm_uvlm_set_far_wake(tvlm_uvlm_plan* plan, int32_t n_levels, const int32_t
  * No counterpart in the reference (a tuning knob; tinyvlm_amd.autotune picks it per workload).
  * Drops a captured graph. */
 int tvlm_uvlm_set_lattice_target(tvlm_uvlm_plan* plan, int32_t groups);
+/* Several simulations of one mesh in one plan (mode 1 only; sims in [1, 65535]), each with its own
+ * prescribed motion: every launch of the time loop covers all of them, so a batch of short
+ * simulations fills the device that one alone leaves idle.  Each simulation's results equal those
+ * of a one-simulation plan with its motion.  No counterpart in the reference (one UVLM per
+ * Backend, vlm/src/vlm.cpp:358); for callers that run many motions of one mesh (a sweep, a
+ * served queue).  tvlm_uvlm_create is this with sims = 1; tvlm_uvlm_set_motion and
+ * tvlm_uvlm_set_pitch_heave set every simulation's motion, tvlm_uvlm_results reads simulation 0.
+ * tvlm_uvlm_time_wake counts the pairs of all simulations. */
+int tvlm_uvlm_create_batched(tvlm_uvlm_plan** plan, int32_t n_surf, const int64_t* ns, const int64_t* nc,
+                             const double* const* verts_init, int64_t t_steps, double dt, int32_t mode,
+                             int32_t sims, void* stream);
+/* motion of simulation sim, laid out as tvlm_uvlm_set_motion takes it */
+int tvlm_uvlm_set_motion_sim(tvlm_uvlm_plan* plan, int32_t sim, const double* T, const double* dT,
+                             const double* freestream, const double* ref_pt);
+/* tvlm_uvlm_set_pitch_heave for simulation sim (-1: every simulation) */
+int tvlm_uvlm_set_pitch_heave_sim(tvlm_uvlm_plan* plan, int32_t sim, double u, double h_amp,
+                                  double h_phase, double a_amp, double a_phase, double omega,
+                                  double x_pivot);
+/* tvlm_uvlm_results for simulation sim */
+int tvlm_uvlm_results_sim(tvlm_uvlm_plan* plan, int32_t sim, double* cl, double* cm, double* gamma,
+                          int32_t* info);
 
 /* ---- coupled pitch/heave flutter sweep ------------------------------------------------------
  * The coupled aeroelastic driver UVLM_2DOF (tests/uvlm_2dof.cpp:428-865, COUPLED path, with the

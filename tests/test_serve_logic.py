@@ -62,26 +62,30 @@ class FakePlan:
 class FakeUVLM:
     live = []
 
-    def __init__(self, verts_list, t_steps, dt, mode=1, stream=None):
-        self.stream, self.t_steps = stream, t_steps
-        self.motion = None
+    def __init__(self, verts_list, t_steps, dt, mode=1, stream=None, sims=1):
+        self.stream, self.t_steps, self.sims = stream, t_steps, sims
+        self.motion = [None] * sims
+        self.runs = 0
         self.polls = 0
         self.closed = False
         FakeUVLM.live.append(self)
 
-    def set_pitch_heave(self, **m):
-        self.motion = m
+    def set_pitch_heave(self, sim=None, **m):
+        for z in range(self.sims) if sim is None else [sim]:
+            self.motion[z] = m
 
     def run(self, use_graph=True):
+        assert all(m is not None for m in self.motion)
+        self.runs += 1
         self.polls = 2
 
     def done(self):
         self.polls -= 1
         return self.polls <= 0
 
-    def results(self):
+    def results(self, sim=0):
         n = self.t_steps - 1
-        cl = np.full(n, self.motion["h_amp"])
+        cl = np.full(n, self.motion[sim]["h_amp"])
         return {"cl": cl, "cm": np.zeros((n, 3)), "gamma": np.zeros(4)}
 
     def close(self):
@@ -173,13 +177,48 @@ def test_unsteady_requests_stay_on_their_lane(S):
         reqs.append(S.Request(100 + i, "s", 0.05 * i))
     with S.Server(meshes, lanes=3, budget_bytes=1 << 40, unsteady={"w": (11, 0.1)}) as srv:
         srv.run(reqs, realtime=False)
-        for (key, lane), slot in srv.uslots.items():
-            assert slot.plan.stream == srv.lanes[lane]
+        for (key, lane, sims), slot in srv.uslots.items():
+            assert slot.plan.stream == srv.lanes[lane] and sims == 1
     for r in reqs:
         if r.kind == "unsteady":
             assert len(r.cl_t) == 10 and np.all(r.cl_t == r.motion["h_amp"])
         else:
             assert r.cl == pytest.approx(2 * np.pi * r.alpha)
+
+
+def test_unsteady_batches_take_waiting_simulations_of_one_mesh(S):
+    """unsteady_batch = 4: 11 waiting simulations run as 4 + 4 + 3 on one lane's plan of 4
+    simulations (3 waiting take the power of two above), each result from its own simulation."""
+    meshes = {"w": _mesh(8)}
+    reqs = [S.Request(i, "w", kind="unsteady", motion=dict(u=1.0, h_amp=0.01 * (i + 1))) for i in range(11)]
+    with S.Server(meshes, lanes=1, budget_bytes=1 << 40, unsteady={"w": (11, 0.1)}, unsteady_batch=4) as srv:
+        st = srv.run(reqs, realtime=False)
+        assert sorted(k[2] for k in srv.uslots) == [4] and FakeUVLM.live[0].runs == 3
+    assert st.batch_sizes == [4, 4, 3]
+    for r in reqs:
+        assert len(r.cl_t) == 10 and np.all(r.cl_t == r.motion["h_amp"])
+    assert [r.batch for r in reqs] == [4] * 8 + [3] * 3
+
+
+def test_unsteady_batch_sizes_are_powers_of_two(S):
+    meshes = {"w": _mesh(8)}
+    with S.Server(meshes, lanes=1, budget_bytes=1 << 40, unsteady={"w": (11, 0.1)}, unsteady_batch=6) as srv:
+        assert [srv.usims(n) for n in (1, 2, 3, 4, 5, 6, 7, 100)] == [1, 2, 4, 4, 6, 6, 6, 6]
+        e1, e2, e4 = (srv._uestimate("w", k) for k in (1, 2, 4))
+        assert e4 - e1 == 3 * (e2 - e1) > 0  # linear in the simulations held
+    with pytest.raises(ValueError):
+        S.Server(meshes, lanes=1, budget_bytes=1 << 40, unsteady_batch=0)
+
+
+def test_partial_unsteady_batch_repeats_the_first_motion(S):
+    meshes = {"w": _mesh(8)}
+    reqs = [S.Request(i, "w", kind="unsteady", motion=dict(u=1.0, h_amp=0.1 * (i + 1))) for i in range(3)]
+    with S.Server(meshes, lanes=1, budget_bytes=1 << 40, unsteady={"w": (11, 0.1)}, unsteady_batch=8) as srv:
+        srv.run(reqs, realtime=False)
+        (key, slot), = srv.uslots.items()
+        assert key[2] == 4 and slot.plan.runs == 1
+        assert [m["h_amp"] for m in slot.plan.motion] == [0.1, 0.2, 0.30000000000000004, 0.1]
+    assert [float(r.cl_t[0]) for r in reqs] == [0.1, 0.2, 0.30000000000000004]
 
 
 def test_request_errors(S):

@@ -389,9 +389,13 @@ __device__ __forceinline__ void lattice_chunk(const double* sverts, const double
     }
 }
 
+// zs: simulations (blockIdx.z) of one launch find their receivers, wake and circulations z * zs
+// elements after simulation 0's (the tables' pointers) and write their partials z * wzs after.
 template <typename TS, int RH>
 __global__ __launch_bounds__(kThreads) void k_wake_lattice(TS* __restrict__ partial, RecvTable<TS> rt,
-                                                           WakeTable<TS> wt) {
+                                                           WakeTable<TS> wt, int64_t zs = 0, int64_t wzs = 0) {
+    const int64_t zo = (int64_t)blockIdx.z * zs;
+    partial += (int64_t)blockIdx.z * wzs;
     constexpr int SWG = kWaves * kLatChunkMax;
     __shared__ double sverts[(RH + 1) * (SWG + 1) * 3];
     __shared__ double sgam[RH * SWG];
@@ -410,7 +414,7 @@ __global__ __launch_bounds__(kThreads) void k_wake_lattice(TS* __restrict__ part
     for (int e = threadIdx.x; e < (rhe + 1) * vw; e += kThreads) {
         const int q = e / vw, sp = e % vw;
         const int col = min((s0 + sp) * ws.vcs, nsf);
-        const int64_t g = (int64_t)col + (int64_t)(ws.vrow0 + (r0 + q) * ws.vrs) * (nsf + 1);
+        const int64_t g = zo + (int64_t)col + (int64_t)(ws.vrow0 + (r0 + q) * ws.vrs) * (nsf + 1);
         sverts[3 * e + 0] = (double)ws.verts[g];
         sverts[3 * e + 1] = (double)ws.verts[g + ws.ws2];
         sverts[3 * e + 2] = (double)ws.verts[g + 2 * ws.ws2];
@@ -418,7 +422,7 @@ __global__ __launch_bounds__(kThreads) void k_wake_lattice(TS* __restrict__ part
     if (ws.vrs == 1 && ws.vcs == 1) {
         for (int e = threadIdx.x; e < rhe * swe; e += kThreads) {
             const int q = e / swe, sp = e % swe;
-            sgam[e] = (double)ws.gamma[(int64_t)(ws.vrow0 + r0 + q) * nsf + s0 + sp];
+            sgam[e] = (double)ws.gamma[zo + (int64_t)(ws.vrow0 + r0 + q) * nsf + s0 + sp];
         }
     } else {  // coarse ring: mean of the fine rings it replaces, rows then spans in order
         for (int e = threadIdx.x; e < rhe * swe; e += kThreads) {
@@ -427,7 +431,7 @@ __global__ __launch_bounds__(kThreads) void k_wake_lattice(TS* __restrict__ part
             const int fc1 = min(fc0 + ws.vcs, nsf);
             double sum = 0.0;
             for (int fr = fr0; fr < fr0 + ws.vrs; fr++)
-                for (int fc = fc0; fc < fc1; fc++) sum += (double)ws.gamma[(int64_t)fr * nsf + fc];
+                for (int fc = fc0; fc < fc1; fc++) sum += (double)ws.gamma[zo + (int64_t)fr * nsf + fc];
             sgam[e] = sum / (double)(ws.vrs * (fc1 - fc0));
         }
     }
@@ -442,10 +446,12 @@ __global__ __launch_bounds__(kThreads) void k_wake_lattice(TS* __restrict__ part
         const int si = find_surface(rt, p);
         const Receivers<TS>& rc = rt.r[si];
         const int li = p - rt.offset[si];
-        const double cx = rc.colloc[li], cy = rc.colloc[li + rc.cs2], cz = rc.colloc[li + 2 * rc.cs2];
+        const TS* co = rc.colloc + zo;
+        const TS* no = rc.normals + zo;
+        const double cx = co[li], cy = co[li + rc.cs2], cz = co[li + 2 * rc.cs2];
         double ax = 0.0, ay = 0.0, az = 0.0;
         lattice_chunk<RH>(sverts, sgam, swe, rhe, vw, sa, sb, cx, cy, cz, ax, ay, az);
-        part = ax * (double)rc.normals[li] + ay * (double)rc.normals[li + rc.ns2] + az * (double)rc.normals[li + 2 * rc.ns2];
+        part = ax * (double)no[li] + ay * (double)no[li + rc.ns2] + az * (double)no[li + 2 * rc.ns2];
     }
     sred[wave][lane] = part;
     __syncthreads();

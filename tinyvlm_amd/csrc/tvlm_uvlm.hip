@@ -22,6 +22,7 @@
 // computes every step's pose, panel metrics, kinematic velocities, velocity rhs and wake vertex row
 // once at setup; a step is then the wake lattice, its reduction into rhs = rhs_vel - wake, the GEMV
 // partials of A^{-1} rhs and k_post (gamma post-processing, wake Gamma, forces, CL/CM).
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -47,6 +48,31 @@ struct UTab {
     int32_t t_steps;
     USurf s[kMaxS];
 };
+
+// Simulation z of a plan holding several simulations (fast mode): every per-simulation buffer of
+// simulation z lies z * zs doubles after simulation 0's (one contiguous block per simulation).
+__device__ __forceinline__ USurf sim_surf(const USurf& a, int64_t o) {
+    USurf b = a;
+    b.verts_pos += o;
+    b.verts += o;
+    b.colloc += o;
+    b.colloc_h += o;
+    b.normals += o;
+    b.areas += o;
+    b.vel += o;
+    b.forces += o;
+    b.gprev += o;
+    b.gdelta += o;
+    b.gdt += o;
+    b.wake += o;
+    b.gamma_wake += o;
+    b.g_verts += o;
+    b.g_colloc += o;
+    b.g_normals += o;
+    b.g_areas += o;
+    b.g_vel += o;
+    return b;
+}
 
 __device__ __forceinline__ int surf_of(const UTab& t, int idx, int which) {
     // which: 0 panel offsets, 1 vertex offsets, 2 trailing-edge copy offsets
@@ -92,8 +118,12 @@ __global__ void k_pre(UTab tab, const double* __restrict__ dT_all, int step, dou
 
 // y(partial cb) = A(:, cols of block cb) x(cols); 64 rows per work-group, 4 waves split the columns
 __global__ __launch_bounds__(256) void k_gemv_part(const double* __restrict__ A, int64_t lda, int n, int cbs,
-                                                   const double* __restrict__ x, double* __restrict__ part) {
+                                                   const double* __restrict__ x, double* __restrict__ part,
+                                                   int64_t zs = 0) {
     __shared__ double red[4][64];
+    A += (int64_t)blockIdx.z * zs;  // simulation blockIdx.z
+    x += (int64_t)blockIdx.z * zs;
+    part += (int64_t)blockIdx.z * zs;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row = blockIdx.x * 64 + lane;
     const int c0 = blockIdx.y * cbs, c1 = min(n, c0 + cbs);
@@ -154,7 +184,7 @@ __device__ __forceinline__ void post_step(const UTab& tab, const double* __restr
                                           double inv_dt, const double* __restrict__ fs_all,
                                           const double* __restrict__ ref_all, double rho, double* __restrict__ gamma,
                                           double* __restrict__ cl_out, double* __restrict__ cm_out,
-                                          double (*lds)[6]) {
+                                          double (*lds)[6], int64_t soff = 0) {
     const int t = threadIdx.x;
     const int N = tab.N;
     auto gsum = [&](int P) { return strided_sum(gsrc, (int64_t)N, P, 0, cb, 1); };
@@ -173,7 +203,7 @@ __device__ __forceinline__ void post_step(const UTab& tab, const double* __restr
     double cmn[3] = {0.0, 0.0, 0.0};
     const bool forces = step > 0;  // the reference skips the coefficients of step 0 (vlm.cpp:461)
     for (int m = 0; m < tab.n; m++) {
-        const USurf& S = tab.s[m];
+        const USurf S = sim_surf(tab.s[m], soff);
         const int ns = S.ns, nc = S.nc, np = ns * nc;
         const int64_t vs1 = ns + 1, vs2 = vs1 * (nc + 1);
         double a_acc = 0.0, c_acc = 0.0, mx = 0.0, my = 0.0, mz = 0.0, mac_acc = 0.0;
@@ -344,14 +374,18 @@ __global__ __launch_bounds__(kPostThreads) void k_post_advance(UTab tab, const d
 // trailing-edge vertex row into wake row t_steps-1-s (vlm.cpp:412-418).  Same arithmetic as
 // k_pre / advance_item.
 __global__ void k_geom_all(UTab tab, const double* __restrict__ T_all, const double* __restrict__ dT_all,
-                           int nsteps, double* __restrict__ g_rhs) {
+                           int nsteps, double* __restrict__ g_rhs, int64_t zs = 0) {
+    const int64_t so = (int64_t)blockIdx.y * zs;  // simulation blockIdx.y
+    T_all += so;
+    dT_all += so;
+    g_rhs += so;
     const int per = tab.NV + tab.N;
     const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (e >= (int64_t)per * nsteps) return;
     const int s = (int)(e / per), r = (int)(e % per);
     if (r < tab.NV) {
         const int m = surf_of(tab, r, 1);
-        const USurf& S = tab.s[m];
+        const USurf S = sim_surf(tab.s[m], so);
         const int p = r - S.v_off;
         const int64_t nv = (int64_t)(S.ns + 1) * (S.nc + 1);
         const double* M = T_all + ((int64_t)s * tab.n + m) * 16;
@@ -374,7 +408,7 @@ __global__ void k_geom_all(UTab tab, const double* __restrict__ T_all, const dou
     }
     const int P = r - tab.NV;
     const int m = surf_of(tab, P, 0);
-    const USurf& S = tab.s[m];
+    const USurf S = sim_surf(tab.s[m], so);
     const int p = P - S.off;
     const int ns = S.ns, nc = S.nc, np = ns * nc;
     const int i = p % ns, j = p / ns;
@@ -412,17 +446,22 @@ __global__ __launch_bounds__(kPostThreads) void k_post(UTab tab, const double* _
                                                        double inv_dt, const double* __restrict__ fs_all,
                                                        const double* __restrict__ ref_all, double rho,
                                                        double* __restrict__ gamma, double* __restrict__ cl_out,
-                                                       double* __restrict__ cm_out) {
+                                                       double* __restrict__ cm_out, int64_t zs = 0) {
     __shared__ double lds[kPostThreads / 64][6];
-    post_step<kPostThreads, true>(tab, gsrc, cb, step, inv_dt, fs_all, ref_all, rho, gamma, cl_out, cm_out, lds);
+    const int64_t o = (int64_t)blockIdx.x * zs;  // simulation blockIdx.x
+    post_step<kPostThreads, true>(tab, gsrc + o, cb, step, inv_dt, fs_all + o, ref_all + o, rho, gamma + o, cl_out + o,
+                                  cm_out + o, lds, o);
 }
 
 // rhs[p] = base[p] - sum over tiles of partial[tile][p] (fixed order, as k_wake_reduce)
 __global__ __launch_bounds__(64 * kRedWaves) void k_wake_reduce_base(double* __restrict__ rhs,
                                                                      const double* __restrict__ base,
                                                                      const double* __restrict__ partial, int tiles,
-                                                                     int total) {
+                                                                     int total, int64_t zs = 0, int64_t wzs = 0) {
     __shared__ double red[kRedWaves][64];
+    rhs += (int64_t)blockIdx.z * zs;  // simulation blockIdx.z
+    base += (int64_t)blockIdx.z * zs;
+    partial += (int64_t)blockIdx.z * wzs;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int p = blockIdx.x * 64 + lane;
     double s = 0.0;
@@ -477,6 +516,12 @@ struct tvlm_uvlm_plan {
     int cb = 1, cbs = 0;
     size_t wpart_elems = 0;
     bool motion_set = false;
+    // simulations held by the plan (fast mode may hold several of one mesh, each with its own
+    // motion): simulation z's per-simulation buffers lie z * zs doubles after simulation 0's, its
+    // wake-induction partials z * wzs after
+    int32_t sims = 1;
+    int64_t zs = 0, wzs = 0;
+    std::vector<uint8_t> sim_motion;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> wake_events;  // start/stop around the replayed wake launches
@@ -496,14 +541,14 @@ namespace tvlm {
 namespace {
 
 int plan_views(tvlm_uvlm_plan* P, int m, tvlm_view& verts, tvlm_view& colloc, tvlm_view& normals, tvlm_view& areas,
-               tvlm_view& wake) {
+               tvlm_view& wake, int sim = 0) {
     const USurf& S = P->tab.s[m];
-    const int64_t ns = S.ns, nc = S.nc;
-    verts = {S.verts, {ns + 1, nc + 1, 4}, {1, ns + 1, (ns + 1) * (nc + 1)}};
-    colloc = {S.colloc, {ns, nc, 3}, {1, ns, ns * nc}};
-    normals = {S.normals, {ns, nc, 3}, {1, ns, ns * nc}};
-    areas = {S.areas, {ns, nc, 1}, {1, ns, ns * nc}};
-    wake = {S.wake, {ns + 1, P->t_steps, 4}, {1, ns + 1, (ns + 1) * P->t_steps}};
+    const int64_t ns = S.ns, nc = S.nc, o = (int64_t)sim * P->zs;
+    verts = {S.verts + o, {ns + 1, nc + 1, 4}, {1, ns + 1, (ns + 1) * (nc + 1)}};
+    colloc = {S.colloc + o, {ns, nc, 3}, {1, ns, ns * nc}};
+    normals = {S.normals + o, {ns, nc, 3}, {1, ns, ns * nc}};
+    areas = {S.areas + o, {ns, nc, 1}, {1, ns, ns * nc}};
+    wake = {S.wake + o, {ns + 1, P->t_steps, 4}, {1, ns + 1, (ns + 1) * P->t_steps}};
     return TVLM_OK;
 }
 
@@ -607,42 +652,52 @@ void wake_tables(const tvlm_uvlm_plan* P, int it, RecvTable<double>& rt, WakeTab
 int setup(tvlm_uvlm_plan* P) {
     hipStream_t st = P->stream;
     const int n = P->n_surf;
-    // pose at t = 0 (vlm.cpp:366-373): verts_pos = T(0) verts_init; verts = verts_pos; metrics; colloc_h
-    for (int m = 0; m < n; m++) {
-        USurf& S = P->tab.s[m];
-        const int64_t nv = (int64_t)(S.ns + 1) * (S.nc + 1);
-        tvlm_view tr = {P->T_all + (int64_t)m * 16, {4, 4, 1}, {1, 4, 16}};
-        tvlm_view vi = {P->init[m], {S.ns + 1, S.nc + 1, 4}, {1, S.ns + 1, nv}};
-        tvlm_view vp = {S.verts_pos, {S.ns + 1, S.nc + 1, 4}, {1, S.ns + 1, nv}};
-        TVLM_TRY(tvlm_displace_wing_f64(1, &tr, &vp, &vi, st));
-        TVLM_HIP(hipMemcpyAsync(S.verts, S.verts_pos, sizeof(double) * nv * 4, hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_fill64, dim3(nblk(S.ns * S.nc)), dim3(256), 0, st, S.gprev, (int64_t)S.ns * S.nc, 0.0);
-        TVLM_LAUNCHED();
-    }
-    std::vector<tvlm_view> V(n), C(n), Nn(n), A(n), W(n);
-    for (int m = 0; m < n; m++) plan_views(P, m, V[m], C[m], Nn[m], A[m], W[m]);
-    TVLM_TRY(tvlm_mesh_metrics_f64(0.0, n, V.data(), C.data(), Nn.data(), A.data(), st));
-    for (int m = 0; m < n; m++) {
-        USurf& S = P->tab.s[m];
-        TVLM_HIP(hipMemcpyAsync(S.colloc_h, S.colloc, sizeof(double) * S.ns * S.nc * 3, hipMemcpyDeviceToDevice, st));
-    }
     const int64_t NN = (int64_t)P->N * P->N;
-    hipLaunchKernelGGL(k_fill64, dim3(nblk(NN)), dim3(256), 0, st, P->lhs, NN, 0.0);
-    TVLM_LAUNCHED();
-    hipLaunchKernelGGL(k_fill64, dim3(nblk(P->t_steps)), dim3(256), 0, st, P->cl, (int64_t)P->t_steps, 0.0);
-    TVLM_LAUNCHED();
-    // LHS once, iteration 0, and its LU (vlm.cpp:402-403)
-    tvlm_view lhs = {P->lhs, {P->N, P->N, 1}, {1, P->N, NN}};
-    TVLM_TRY(tvlm_lhs_assemble_f64(&lhs, n, C.data(), Nn.data(), V.data(), W.data(), 0, 0.0, st));
+    for (int z = 0; z < P->sims; z++) {
+        const int64_t o = (int64_t)z * P->zs;
+        // pose at t = 0 (vlm.cpp:366-373): verts_pos = T(0) verts_init; verts = verts_pos; metrics; colloc_h
+        for (int m = 0; m < n; m++) {
+            USurf& S = P->tab.s[m];
+            const int64_t nv = (int64_t)(S.ns + 1) * (S.nc + 1);
+            tvlm_view tr = {P->T_all + o + (int64_t)m * 16, {4, 4, 1}, {1, 4, 16}};
+            tvlm_view vi = {P->init[m], {S.ns + 1, S.nc + 1, 4}, {1, S.ns + 1, nv}};
+            tvlm_view vp = {S.verts_pos + o, {S.ns + 1, S.nc + 1, 4}, {1, S.ns + 1, nv}};
+            TVLM_TRY(tvlm_displace_wing_f64(1, &tr, &vp, &vi, st));
+            TVLM_HIP(hipMemcpyAsync(S.verts + o, S.verts_pos + o, sizeof(double) * nv * 4, hipMemcpyDeviceToDevice,
+                                    st));
+            hipLaunchKernelGGL(k_fill64, dim3(nblk(S.ns * S.nc)), dim3(256), 0, st, S.gprev + o,
+                               (int64_t)S.ns * S.nc, 0.0);
+            TVLM_LAUNCHED();
+        }
+        std::vector<tvlm_view> V(n), C(n), Nn(n), A(n), W(n);
+        for (int m = 0; m < n; m++) plan_views(P, m, V[m], C[m], Nn[m], A[m], W[m], z);
+        TVLM_TRY(tvlm_mesh_metrics_f64(0.0, n, V.data(), C.data(), Nn.data(), A.data(), st));
+        for (int m = 0; m < n; m++) {
+            USurf& S = P->tab.s[m];
+            TVLM_HIP(hipMemcpyAsync(S.colloc_h + o, S.colloc + o, sizeof(double) * S.ns * S.nc * 3,
+                                    hipMemcpyDeviceToDevice, st));
+        }
+        hipLaunchKernelGGL(k_fill64, dim3(nblk(NN)), dim3(256), 0, st, P->lhs + o, NN, 0.0);
+        TVLM_LAUNCHED();
+        hipLaunchKernelGGL(k_fill64, dim3(nblk(P->t_steps)), dim3(256), 0, st, P->cl + o, (int64_t)P->t_steps, 0.0);
+        TVLM_LAUNCHED();
+        // LHS once, iteration 0 (vlm.cpp:402)
+        tvlm_view lhs = {P->lhs + o, {P->N, P->N, 1}, {1, P->N, NN}};
+        TVLM_TRY(tvlm_lhs_assemble_f64(&lhs, n, C.data(), Nn.data(), V.data(), W.data(), 0, 0.0, st));
+        if (P->mode == 1) {
+            hipLaunchKernelGGL(k_eye, dim3(nblk(NN)), dim3(256), 0, st, P->ainv + o, P->N);
+            TVLM_LAUNCHED();
+        }
+    }
+    // the LU of every simulation's LHS at once (vlm.cpp:403), in fast mode its inverse
+    tvlm_view lhs = {P->lhs, {P->N, P->N, P->sims}, {1, P->N, P->sims > 1 ? P->zs : NN}};
     TVLM_TRY(tvlm_getrf_f64(&lhs, P->ipiv, P->info, st));
     if (P->mode == 1) {
-        hipLaunchKernelGGL(k_eye, dim3(nblk(NN)), dim3(256), 0, st, P->ainv, P->N);
-        TVLM_LAUNCHED();
-        tvlm_view x = {P->ainv, {P->N, P->N, 1}, {1, P->N, NN}};
+        tvlm_view x = {P->ainv, {P->N, P->N, P->sims}, {1, P->N, P->sims > 1 ? P->zs : NN}};
         TVLM_TRY(tvlm_getrs_f64(&lhs, P->ipiv, &x, st));
         const int nsteps = (int)P->t_steps - 1;
-        hipLaunchKernelGGL(k_geom_all, dim3(nblk((int64_t)(P->tab.NV + P->N) * nsteps)), dim3(256), 0, st, P->tab,
-                           P->T_all, P->dT_all, nsteps, P->g_rhs);
+        hipLaunchKernelGGL(k_geom_all, dim3(nblk((int64_t)(P->tab.NV + P->N) * nsteps), P->sims), dim3(256), 0, st,
+                           P->tab, P->T_all, P->dT_all, nsteps, P->g_rhs, P->zs);
         TVLM_LAUNCHED();
     }
     return TVLM_OK;
@@ -656,10 +711,11 @@ int launch_wake(tvlm_uvlm_plan* P, int i, int& tiles) {
     tiles = 0;
     wake_tables(P, i, rt, wt, tiles);
     TVLM_REQUIRE((size_t)tiles * P->N <= P->wpart_elems, "wake partial buffer too small");
-    dim3 grid((P->N + kRecv - 1) / kRecv, tiles);
+    dim3 grid((P->N + kRecv - 1) / kRecv, tiles, P->sims);
     if (P->mode == 1) {
         int rh = 1, split = 1;
-        lattice_launch_shape(wt, (P->N + kRecv - 1) / kRecv, P->lat_target, rh, split);
+        // the launch's work-groups count every simulation's receiver blocks
+        lattice_launch_shape(wt, (P->N + kRecv - 1) / kRecv * P->sims, P->lat_target, rh, split);
         int t2 = 0;
         for (int m = 0; m < wt.n; m++) {
             WakeSurface<double>& w = wt.s[m];
@@ -674,11 +730,14 @@ int launch_wake(tvlm_uvlm_plan* P, int i, int& tiles) {
         grid.y = tiles;
         TVLM_REQUIRE((size_t)tiles * P->N <= P->wpart_elems, "wake partial buffer too small");
         if (rh == 1)
-            hipLaunchKernelGGL((k_wake_lattice<double, 1>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt);
+            hipLaunchKernelGGL((k_wake_lattice<double, 1>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt, P->zs,
+                               P->wzs);
         else if (rh == 2)
-            hipLaunchKernelGGL((k_wake_lattice<double, 2>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt);
+            hipLaunchKernelGGL((k_wake_lattice<double, 2>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt, P->zs,
+                               P->wzs);
         else
-            hipLaunchKernelGGL((k_wake_lattice<double, 4>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt);
+            hipLaunchKernelGGL((k_wake_lattice<double, 4>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt, P->zs,
+                               P->wzs);
     } else
         hipLaunchKernelGGL((k_wake_partials<double, double, 0>), grid, dim3(kThreads), 0, st, P->wpart, rt, wt, 0.0);
     TVLM_LAUNCHED();
@@ -693,8 +752,8 @@ int enqueue_step_fast(tvlm_uvlm_plan* P, int i) {
     if (i > 0) {
         int tiles = 0;
         TVLM_TRY(launch_wake(P, i, tiles));
-        hipLaunchKernelGGL(k_wake_reduce_base, dim3((P->N + 63) / 64), dim3(64 * kRedWaves), 0, st, P->rhs,
-                           P->g_rhs + (int64_t)i * P->N, P->wpart, tiles, P->N);
+        hipLaunchKernelGGL(k_wake_reduce_base, dim3((P->N + 63) / 64, 1, P->sims), dim3(64 * kRedWaves), 0, st,
+                           P->rhs, P->g_rhs + (int64_t)i * P->N, P->wpart, tiles, P->N, P->zs, P->wzs);
         TVLM_LAUNCHED();
         x = P->rhs;
     }
@@ -707,11 +766,11 @@ int enqueue_step_fast(tvlm_uvlm_plan* P, int i) {
         S.normals = S.g_normals + i * np * 3;
         S.areas = S.g_areas + i * np;
     }
-    hipLaunchKernelGGL(k_gemv_part, dim3((P->N + 63) / 64, P->cb), dim3(256), 0, st, P->ainv, (int64_t)P->N, P->N,
-                       P->cbs, x, P->gpart);
+    hipLaunchKernelGGL(k_gemv_part, dim3((P->N + 63) / 64, P->cb, P->sims), dim3(256), 0, st, P->ainv,
+                       (int64_t)P->N, P->N, P->cbs, x, P->gpart, P->zs);
     TVLM_LAUNCHED();
-    hipLaunchKernelGGL(k_post, dim3(1), dim3(kPostThreads), 0, st, t, P->gpart, P->cb, i, 1.0 / P->dt, P->fs_all,
-                       P->ref_all, 1.0, P->gamma, P->cl, P->cm);
+    hipLaunchKernelGGL(k_post, dim3(P->sims), dim3(kPostThreads), 0, st, t, P->gpart, P->cb, i, 1.0 / P->dt,
+                       P->fs_all, P->ref_all, 1.0, P->gamma, P->cl, P->cm, P->zs);
     TVLM_LAUNCHED();
     return TVLM_OK;
 }
@@ -757,18 +816,23 @@ int enqueue_step(tvlm_uvlm_plan* P, int i) {
 
 extern "C" {
 
-int tvlm_uvlm_create(tvlm_uvlm_plan** out, int32_t n_surf, const int64_t* ns, const int64_t* nc,
-                     const double* const* verts_init, int64_t t_steps, double dt, int32_t mode, void* stream) {
+int tvlm_uvlm_create_batched(tvlm_uvlm_plan** out, int32_t n_surf, const int64_t* ns, const int64_t* nc,
+                             const double* const* verts_init, int64_t t_steps, double dt, int32_t mode,
+                             int32_t sims, void* stream) {
     TVLM_REQUIRE(out && ns && nc && verts_init, "null argument");
     TVLM_REQUIRE(n_surf >= 1 && n_surf <= TVLM_MAX_SURFACES, "n_surf %d out of range", n_surf);
     TVLM_REQUIRE(t_steps >= 2 && t_steps < (1 << 24), "t_steps %lld out of range", (long long)t_steps);
     TVLM_REQUIRE(dt > 0.0, "dt must be positive");
     TVLM_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (plain FP64) or 1 (FP32 wake, inverse solve, graph)");
+    TVLM_REQUIRE(sims >= 1 && sims <= 65535, "sims %d out of range", sims);
+    TVLM_REQUIRE(sims == 1 || mode == 1, "several simulations per plan need mode 1");
     auto* P = new tvlm_uvlm_plan();
     P->mode = mode;
     P->n_surf = n_surf;
     P->t_steps = t_steps;
     P->dt = dt;
+    P->sims = sims;
+    P->sim_motion.assign((size_t)sims, 0);
     P->stream = as_stream(stream);
     UTab& tab = P->tab;
     tab.n = n_surf;
@@ -792,36 +856,9 @@ int tvlm_uvlm_create(tvlm_uvlm_plan** out, int32_t n_surf, const int64_t* ns, co
         S.off = off;
         S.v_off = voff;
         S.te_off = teoff;
-        const size_t np = (size_t)ns[m] * nc[m], nv = (size_t)(ns[m] + 1) * (nc[m] + 1);
-        off += (int)np;
-        voff += (int)nv;
+        off += (int)(ns[m] * nc[m]);
+        voff += (int)((ns[m] + 1) * (nc[m] + 1));
         teoff += 4 * (int)(ns[m] + 1);
-        double** bufs3[] = {&S.colloc, &S.colloc_h, &S.normals, &S.vel, &S.forces};
-        for (double** b : bufs3)
-            if ((rc = P->alloc((void**)b, sizeof(double) * np * 3)) != TVLM_OK) return fail(rc);
-        double** bufs1[] = {&S.areas, &S.gprev, &S.gdelta, &S.gdt};
-        for (double** b : bufs1)
-            if ((rc = P->alloc((void**)b, sizeof(double) * np)) != TVLM_OK) return fail(rc);
-        if ((rc = P->alloc((void**)&S.verts_pos, sizeof(double) * nv * 4)) != TVLM_OK) return fail(rc);
-        if ((rc = P->alloc((void**)&S.verts, sizeof(double) * nv * 4)) != TVLM_OK) return fail(rc);
-        if ((rc = P->alloc((void**)&P->init[m], sizeof(double) * nv * 4)) != TVLM_OK) return fail(rc);
-        if ((rc = P->alloc((void**)&S.wake, sizeof(double) * (size_t)(ns[m] + 1) * t_steps * 4)) != TVLM_OK)
-            return fail(rc);
-        if ((rc = P->alloc((void**)&S.gamma_wake, sizeof(double) * (size_t)ns[m] * (t_steps - 1))) != TVLM_OK)
-            return fail(rc);
-        if (mode == 1) {
-            const size_t st1 = (size_t)(t_steps - 1);
-            if ((rc = P->alloc((void**)&S.g_verts, sizeof(double) * nv * 3 * st1)) != TVLM_OK) return fail(rc);
-            double** gb3[] = {&S.g_colloc, &S.g_normals, &S.g_vel};
-            for (double** b : gb3)
-                if ((rc = P->alloc((void**)b, sizeof(double) * np * 3 * st1)) != TVLM_OK) return fail(rc);
-            if ((rc = P->alloc((void**)&S.g_areas, sizeof(double) * np * st1)) != TVLM_OK) return fail(rc);
-        }
-        if (hipMemcpyAsync(P->init[m], verts_init[m], sizeof(double) * nv * 4, hipMemcpyHostToDevice, P->stream) !=
-            hipSuccess) {
-            set_error("upload of surface %d failed", m);
-            return fail(TVLM_ERR_HIP);
-        }
         const int sw = ns[m] < kMaxSpanTile ? (int)ns[m] : kMaxSpanTile;
         max_tiles += (size_t)((ns[m] + sw - 1) / sw) * (size_t)t_steps;  // worst case: one row per tile
     }
@@ -839,23 +876,60 @@ int tvlm_uvlm_create(tvlm_uvlm_plan** out, int32_t n_surf, const int64_t* ns, co
     P->cbs = 64;
     P->cb = (int)((N + P->cbs - 1) / P->cbs);
     P->wpart_elems = max_tiles * N;
-    if ((rc = P->alloc((void**)&P->lhs, sizeof(double) * N * N)) != TVLM_OK) return fail(rc);
-    if (mode == 1 && (rc = P->alloc((void**)&P->ainv, sizeof(double) * N * N)) != TVLM_OK) return fail(rc);
-    if ((rc = P->alloc((void**)&P->rhs, sizeof(double) * N)) != TVLM_OK) return fail(rc);
-    if ((rc = P->alloc((void**)&P->gamma, sizeof(double) * N)) != TVLM_OK) return fail(rc);
-    if ((rc = P->alloc((void**)&P->gpart, sizeof(double) * N * P->cb)) != TVLM_OK) return fail(rc);
-    if ((rc = P->alloc((void**)&P->wpart, sizeof(double) * (P->wpart_elems ? P->wpart_elems : 1))) != TVLM_OK)
+    // every per-simulation buffer: one block per simulation (256-byte aligned parts)
+    const size_t st1 = (size_t)(t_steps - 1), ts = (size_t)t_steps;
+    std::vector<std::pair<double**, size_t>> lay;
+    for (int m = 0; m < n_surf; m++) {
+        USurf& S = tab.s[m];
+        const size_t np = (size_t)ns[m] * nc[m], nv = (size_t)(ns[m] + 1) * (nc[m] + 1);
+        for (double** b3 : {&S.colloc, &S.colloc_h, &S.normals, &S.vel, &S.forces}) lay.push_back({b3, np * 3});
+        for (double** b1 : {&S.areas, &S.gprev, &S.gdelta, &S.gdt}) lay.push_back({b1, np});
+        lay.push_back({&S.verts_pos, nv * 4});
+        lay.push_back({&S.verts, nv * 4});
+        lay.push_back({&S.wake, (size_t)(ns[m] + 1) * ts * 4});
+        lay.push_back({&S.gamma_wake, (size_t)ns[m] * st1});
+        if (mode == 1) {
+            lay.push_back({&S.g_verts, nv * 3 * st1});
+            for (double** g3 : {&S.g_colloc, &S.g_normals, &S.g_vel}) lay.push_back({g3, np * 3 * st1});
+            lay.push_back({&S.g_areas, np * st1});
+        }
+    }
+    lay.push_back({&P->lhs, N * N});
+    if (mode == 1) lay.push_back({&P->ainv, N * N});
+    lay.push_back({&P->rhs, N});
+    lay.push_back({&P->gamma, N});
+    lay.push_back({&P->gpart, N * (size_t)P->cb});
+    lay.push_back({&P->cl, ts});
+    lay.push_back({&P->cm, 3 * ts});
+    lay.push_back({&P->T_all, 16 * (size_t)n_surf * ts});
+    lay.push_back({&P->dT_all, 16 * (size_t)n_surf * ts});
+    lay.push_back({&P->fs_all, 3 * ts});
+    lay.push_back({&P->ref_all, 3 * ts});
+    if (mode == 1) lay.push_back({&P->g_rhs, N * st1});
+    std::vector<size_t> where(lay.size());
+    size_t blk = 0;
+    for (size_t k = 0; k < lay.size(); k++) {
+        where[k] = blk;
+        blk += (lay[k].second + 31) / 32 * 32;
+    }
+    P->zs = (int64_t)blk;
+    double* base = nullptr;
+    if ((rc = P->alloc((void**)&base, sizeof(double) * blk * (size_t)sims)) != TVLM_OK) return fail(rc);
+    for (size_t k = 0; k < lay.size(); k++) *lay[k].first = base + where[k];
+    P->wzs = (int64_t)((P->wpart_elems + 31) / 32 * 32);  // every simulation's partials 256-byte aligned
+    if ((rc = P->alloc((void**)&P->wpart, sizeof(double) * (size_t)P->wzs * (size_t)sims)) != TVLM_OK)
         return fail(rc);
-    if ((rc = P->alloc((void**)&P->cl, sizeof(double) * t_steps)) != TVLM_OK) return fail(rc);
-    if ((rc = P->alloc((void**)&P->cm, sizeof(double) * 3 * t_steps)) != TVLM_OK) return fail(rc);
-    if ((rc = P->alloc((void**)&P->T_all, sizeof(double) * 16 * n_surf * t_steps)) != TVLM_OK) return fail(rc);
-    if ((rc = P->alloc((void**)&P->dT_all, sizeof(double) * 16 * n_surf * t_steps)) != TVLM_OK) return fail(rc);
-    if ((rc = P->alloc((void**)&P->fs_all, sizeof(double) * 3 * t_steps)) != TVLM_OK) return fail(rc);
-    if ((rc = P->alloc((void**)&P->ref_all, sizeof(double) * 3 * t_steps)) != TVLM_OK) return fail(rc);
-    if ((rc = P->alloc((void**)&P->ipiv, sizeof(int32_t) * N)) != TVLM_OK) return fail(rc);
-    if ((rc = P->alloc((void**)&P->info, sizeof(int32_t))) != TVLM_OK) return fail(rc);
-    if (mode == 1 && (rc = P->alloc((void**)&P->g_rhs, sizeof(double) * N * (size_t)(t_steps - 1))) != TVLM_OK)
-        return fail(rc);
+    if ((rc = P->alloc((void**)&P->ipiv, sizeof(int32_t) * N * (size_t)sims)) != TVLM_OK) return fail(rc);
+    if ((rc = P->alloc((void**)&P->info, sizeof(int32_t) * (size_t)sims)) != TVLM_OK) return fail(rc);
+    for (int m = 0; m < n_surf; m++) {
+        const size_t nv = (size_t)(ns[m] + 1) * (nc[m] + 1);
+        if ((rc = P->alloc((void**)&P->init[m], sizeof(double) * nv * 4)) != TVLM_OK) return fail(rc);
+        if (hipMemcpyAsync(P->init[m], verts_init[m], sizeof(double) * nv * 4, hipMemcpyHostToDevice, P->stream) !=
+            hipSuccess) {
+            set_error("upload of surface %d failed", m);
+            return fail(TVLM_ERR_HIP);
+        }
+    }
     if (hipStreamSynchronize(P->stream) != hipSuccess) {
         set_error("stream synchronize failed");
         return fail(TVLM_ERR_HIP);
@@ -864,18 +938,33 @@ int tvlm_uvlm_create(tvlm_uvlm_plan** out, int32_t n_surf, const int64_t* ns, co
     return TVLM_OK;
 }
 
+int tvlm_uvlm_create(tvlm_uvlm_plan** out, int32_t n_surf, const int64_t* ns, const int64_t* nc,
+                     const double* const* verts_init, int64_t t_steps, double dt, int32_t mode, void* stream) {
+    return tvlm_uvlm_create_batched(out, n_surf, ns, nc, verts_init, t_steps, dt, mode, 1, stream);
+}
+
 // T, dT: [t_steps][n_surf][16] column-major 4x4 value and time derivative of each surface transform
 // at t_i = i dt; freestream, ref_pt: [t_steps][3] (the CM reference point moves with surface 0).
+int tvlm_uvlm_set_motion_sim(tvlm_uvlm_plan* P, int32_t sim, const double* T, const double* dT,
+                             const double* freestream, const double* ref_pt) {
+    TVLM_REQUIRE(P && T && dT && freestream && ref_pt, "null argument");
+    TVLM_REQUIRE(sim >= 0 && sim < P->sims, "simulation %d of %d", sim, P->sims);
+    const size_t n16 = sizeof(double) * 16 * P->n_surf * P->t_steps, n3 = sizeof(double) * 3 * P->t_steps;
+    const int64_t o = (int64_t)sim * P->zs;
+    TVLM_HIP(hipMemcpyAsync(P->T_all + o, T, n16, hipMemcpyHostToDevice, P->stream));
+    TVLM_HIP(hipMemcpyAsync(P->dT_all + o, dT, n16, hipMemcpyHostToDevice, P->stream));
+    TVLM_HIP(hipMemcpyAsync(P->fs_all + o, freestream, n3, hipMemcpyHostToDevice, P->stream));
+    TVLM_HIP(hipMemcpyAsync(P->ref_all + o, ref_pt, n3, hipMemcpyHostToDevice, P->stream));
+    TVLM_HIP(hipStreamSynchronize(P->stream));
+    P->sim_motion[(size_t)sim] = 1;
+    P->motion_set = std::all_of(P->sim_motion.begin(), P->sim_motion.end(), [](uint8_t v) { return v != 0; });
+    return TVLM_OK;
+}
+
 int tvlm_uvlm_set_motion(tvlm_uvlm_plan* P, const double* T, const double* dT, const double* freestream,
                          const double* ref_pt) {
-    TVLM_REQUIRE(P && T && dT && freestream && ref_pt, "null argument");
-    const size_t n16 = sizeof(double) * 16 * P->n_surf * P->t_steps, n3 = sizeof(double) * 3 * P->t_steps;
-    TVLM_HIP(hipMemcpyAsync(P->T_all, T, n16, hipMemcpyHostToDevice, P->stream));
-    TVLM_HIP(hipMemcpyAsync(P->dT_all, dT, n16, hipMemcpyHostToDevice, P->stream));
-    TVLM_HIP(hipMemcpyAsync(P->fs_all, freestream, n3, hipMemcpyHostToDevice, P->stream));
-    TVLM_HIP(hipMemcpyAsync(P->ref_all, ref_pt, n3, hipMemcpyHostToDevice, P->stream));
-    TVLM_HIP(hipStreamSynchronize(P->stream));
-    P->motion_set = true;
+    TVLM_REQUIRE(P, "null plan");
+    for (int32_t z = 0; z < P->sims; z++) TVLM_TRY(tvlm_uvlm_set_motion_sim(P, z, T, dT, freestream, ref_pt));
     return TVLM_OK;
 }
 
@@ -933,7 +1022,7 @@ int tvlm_uvlm_time_wake(tvlm_uvlm_plan* P, double* total_ms, int32_t* launches, 
     TVLM_HIP(hipEventRecord(P->wake_events[1], P->stream));
     TVLM_HIP(hipStreamSynchronize(P->stream));
     double pr = 0.0;
-    for (int i = 1; i < last; i++) pr += wake_pairs(P, i);
+    for (int i = 1; i < last; i++) pr += wake_pairs(P, i) * (double)P->sims;
     float ms = 0.f;
     TVLM_HIP(hipEventElapsedTime(&ms, P->wake_events[0], P->wake_events[1]));
     *total_ms = ms;
@@ -998,9 +1087,11 @@ int tvlm_uvlm_set_lattice_target(tvlm_uvlm_plan* P, int32_t groups) {
     if (need > P->wpart_elems) {
         TVLM_HIP(hipStreamSynchronize(P->stream));
         double* w = nullptr;
-        TVLM_TRY(P->alloc((void**)&w, sizeof(double) * need));
+        const size_t wzs = (need + 31) / 32 * 32;
+        TVLM_TRY(P->alloc((void**)&w, sizeof(double) * wzs * (size_t)P->sims));
         P->wpart = w;  // the smaller buffer stays in the plan's allocations until destroy
         P->wpart_elems = need;
+        P->wzs = (int64_t)wzs;
     }
     P->lat_target = groups;
     return TVLM_OK;
@@ -1009,16 +1100,22 @@ int tvlm_uvlm_set_lattice_target(tvlm_uvlm_plan* P, int32_t groups) {
 // Copies (synchronously) CL[t_steps-1], CM[t_steps-1][3] (step 0 entries are 0), the last
 // circulation gamma[N] and the LU info; any pointer may be null.  The copies are queued on the
 // plan's stream (not the null stream), so they do not wait for work on other streams.
-int tvlm_uvlm_results(tvlm_uvlm_plan* P, double* cl, double* cm, double* gamma, int32_t* info) {
+int tvlm_uvlm_results_sim(tvlm_uvlm_plan* P, int32_t sim, double* cl, double* cm, double* gamma, int32_t* info) {
     TVLM_REQUIRE(P, "null plan");
+    TVLM_REQUIRE(sim >= 0 && sim < P->sims, "simulation %d of %d", sim, P->sims);
     hipStream_t st = P->stream;
     const size_t n = (size_t)(P->t_steps - 1);
-    if (cl) TVLM_HIP(hipMemcpyAsync(cl, P->cl, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    if (cm) TVLM_HIP(hipMemcpyAsync(cm, P->cm, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
-    if (gamma) TVLM_HIP(hipMemcpyAsync(gamma, P->gamma, sizeof(double) * P->N, hipMemcpyDeviceToHost, st));
-    if (info) TVLM_HIP(hipMemcpyAsync(info, P->info, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    const int64_t o = (int64_t)sim * P->zs;
+    if (cl) TVLM_HIP(hipMemcpyAsync(cl, P->cl + o, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (cm) TVLM_HIP(hipMemcpyAsync(cm, P->cm + o, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+    if (gamma) TVLM_HIP(hipMemcpyAsync(gamma, P->gamma + o, sizeof(double) * P->N, hipMemcpyDeviceToHost, st));
+    if (info) TVLM_HIP(hipMemcpyAsync(info, P->info + sim, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     TVLM_HIP(hipStreamSynchronize(st));
     return TVLM_OK;
+}
+
+int tvlm_uvlm_results(tvlm_uvlm_plan* P, double* cl, double* cm, double* gamma, int32_t* info) {
+    return tvlm_uvlm_results_sim(P, 0, cl, cm, gamma, info);
 }
 
 // Prescribed freestream + heave + pitch chain shared by all surfaces, evaluated with dual numbers
@@ -1092,14 +1189,21 @@ int tvlm_motion_pitch_heave(double u, double h_amp, double h_phase, double a_amp
     return TVLM_OK;
 }
 
-int tvlm_uvlm_set_pitch_heave(tvlm_uvlm_plan* P, double u, double h_amp, double h_phase, double a_amp,
-                              double a_phase, double omega, double x_pivot) {
+int tvlm_uvlm_set_pitch_heave_sim(tvlm_uvlm_plan* P, int32_t sim, double u, double h_amp, double h_phase,
+                                  double a_amp, double a_phase, double omega, double x_pivot) {
     TVLM_REQUIRE(P, "null plan");
+    TVLM_REQUIRE(sim >= -1 && sim < P->sims, "simulation %d of %d", sim, P->sims);
     const int64_t ts = P->t_steps;
     std::vector<double> T(16 * P->n_surf * ts), dT(16 * P->n_surf * ts), fs(3 * ts), ref(3 * ts);
     TVLM_TRY(tvlm_motion_pitch_heave(u, h_amp, h_phase, a_amp, a_phase, omega, x_pivot, P->n_surf, ts, P->dt, T.data(),
                                      dT.data(), fs.data(), ref.data()));
-    return tvlm_uvlm_set_motion(P, T.data(), dT.data(), fs.data(), ref.data());
+    if (sim < 0) return tvlm_uvlm_set_motion(P, T.data(), dT.data(), fs.data(), ref.data());
+    return tvlm_uvlm_set_motion_sim(P, sim, T.data(), dT.data(), fs.data(), ref.data());
+}
+
+int tvlm_uvlm_set_pitch_heave(tvlm_uvlm_plan* P, double u, double h_amp, double h_phase, double a_amp,
+                              double a_phase, double omega, double x_pivot) {
+    return tvlm_uvlm_set_pitch_heave_sim(P, -1, u, h_amp, h_phase, a_amp, a_phase, omega, x_pivot);
 }
 
 int tvlm_uvlm_destroy(tvlm_uvlm_plan* P) {

@@ -14,9 +14,13 @@ Structure
 - memory: the device bytes of all plans (tvlm_vlm_footprint) stay under `budget_bytes`.  A new plan
   that would exceed it first evicts the least recently used idle plans; if that is not enough, its
   mesh waits for memory while other meshes are served.
-- unsteady requests: one device-resident UVLM plan (tvlm_uvlm_*, served mode, hipGraph of the time
+- unsteady requests: device-resident UVLM plans (tvlm_uvlm_*, served mode, hipGraph of the time
   loop) per (mesh, lane), created on that lane when first needed and kept there (its graph and the
-  scratch buffers it captured belong to the lane's stream); one simulation per launch.
+  scratch buffers it captured belong to the lane's stream).  With unsteady_batch = B > 1 a lane
+  takes up to B waiting simulations of one mesh into one plan that holds several simulations
+  (tvlm_uvlm_create_batched): every launch of the time loop advances all of them.  Plans hold a
+  power of two of simulations (at most B); a batch of fewer requests fills the rest of its plan
+  with copies of its first motion, whose results are dropped.
 - admission: whenever a lane is free, the queue (steady: per mesh; unsteady: per mesh) whose oldest
   waiting request arrived first is served: for steady requests an idle plan of that mesh (or a new
   one) takes up to its cap of that mesh's waiting requests, in arrival order.  Requests that arrive
@@ -27,7 +31,9 @@ A request's result does not depend on the batch it shared: each matrix of a batc
 independently, and the FP64 refinement drives every solution to the FP64 solution of its own FP64
 system; only FP32 rounding inside the LU (panel width, GEMM kernel choice, which follow the batch
 size) differs, and the refinement removes it to ~1e-14 relative.  An unsteady simulation runs the
-same kernels on the same operands whatever runs beside it: its CL(t) is bitwise the alone result.
+same kernels on the same operands whatever runs beside it: its CL(t) is bitwise the alone result
+when it runs in a one-simulation plan; in a plan of several simulations the setup's batched LU may
+block its updates differently, and CL(t) agrees with the alone result to ~1e-13 relative.
 
 The request mix used to measure the server is `steady_mix_trace`: 4096 requests, PCG64 seed 0,
 exponential inter-arrival gaps (unit rate, scaled by 1/lambda when run), mesh family in
@@ -211,18 +217,20 @@ class Server:
     def __init__(self, meshes: dict, lanes: int = 3, max_batch: int = 32, plan_bytes: int = 16 << 30,
                  budget_bytes: int | None = None, plans_per_mesh: int | None = None, u_inf: float = 1.0,
                  rho: float = 1.0, keep_gamma: bool = False, unsteady: dict | None = None,
-                 hb: dict | None = None, hb_workers: int = 1):
+                 hb: dict | None = None, hb_workers: int = 1, unsteady_batch: int = 1):
         """meshes: name -> list of verts arrays (one per surface, as read_plot3d returns them).
         unsteady: name -> (t_steps, dt) or (t_steps, dt, far_wake) for the meshes that take unsteady
         requests; far_wake = [(age, row_group, span_group), ...] aggregates the wake beyond `age` steps
         (UVLMPlan.set_far_wake; accept it only where it keeps |dCL| within the bar).
+        unsteady_batch: most simulations of one mesh a lane runs together in one plan.
         hb: name -> {"meshes": [wing_path, flap_path], "harmonics": H, "scaling": s} for the HB
         configurations that take kind = "hb" requests; they run on hb_workers worker threads, each
         with its own HBVLM (and stream) per configuration, beside the lanes.  An HB solve's Anderson
         loop is driven from Python launch by launch, so more workers in one process contend for the
         interpreter (measured: 1 worker 8.8, 2 workers 7.6, 4 workers 6.9 solves/s)."""
-        if lanes < 1 or max_batch < 1:
-            raise ValueError("lanes and max_batch must be positive")
+        if lanes < 1 or max_batch < 1 or unsteady_batch < 1:
+            raise ValueError("lanes, max_batch and unsteady_batch must be positive")
+        self.unsteady_batch = int(unsteady_batch)
         self.meshes = {k: [np.asarray(v, dtype=np.float64) for v in vl] for k, vl in meshes.items()}
         self.N = {k: sum((v.shape[0] - 1) * (v.shape[1] - 1) for v in vl) for k, vl in self.meshes.items()}
         self.cap = {k: batch_cap(n, max_batch, plan_bytes) for k, n in self.N.items()}
@@ -239,7 +247,7 @@ class Server:
         for k in self.unsteady:
             if k not in self.meshes:
                 raise KeyError(f"unsteady mesh {k!r} is not among the meshes")
-        self.uslots: dict[tuple[str, int], _Slot] = {}
+        self.uslots: dict[tuple[str, int, int], _Slot] = {}  # (mesh, lane, simulations) -> plan
         self.used_bytes = 0
         self.stats = ServeStats()
         self.hb = dict(hb or {})
@@ -256,10 +264,19 @@ class Server:
         n, b = self.N[key], self.cap[key]
         return 12 * n * n * b + 400 * n * b
 
-    def _uestimate(self, key: str) -> int:
+    def _uestimate(self, key: str, sims: int = 1) -> int:
+        """Device bytes of an unsteady plan of `sims` simulations: per simulation the LHS and its
+        inverse, every step's geometry (vertices, collocation points, normals, velocities, areas,
+        rhs), the wake and its circulations, and the wake-induction partials (one row of N per
+        64-span tile and step, at most)."""
         t_steps = self.unsteady[key][0]
         n = self.N[key]
-        return 16 * n * n + 8 * n * t_steps + 256 * n + (1 << 20)  # LHS, inverse, rhs of all steps, wake
+        vl = self.meshes[key]
+        nv = sum(v.shape[0] * v.shape[1] for v in vl)
+        ns1 = sum(v.shape[0] for v in vl)
+        tiles = sum(-(-(v.shape[0] - 1) // 64) for v in vl) * t_steps + 64
+        per = 16 * n * n + 8 * t_steps * (3 * nv + 12 * n + 5 * ns1) + 8 * tiles * n + 512 * n
+        return sims * per + (1 << 20)
 
     def _evict_for(self, need: int, now: float) -> bool:
         """Frees the least recently used idle plans (steady and unsteady) until `need` bytes fit the
@@ -302,22 +319,30 @@ class Server:
         self.stats.peak_bytes = max(self.stats.peak_bytes, self.used_bytes)
         return s
 
-    def _uslot_for(self, key: str, lane: int, now: float) -> _Slot | None:
-        s = self.uslots.get((key, lane))
+    def usims(self, waiting: int) -> int:
+        """Simulations of the plan that takes `waiting` requests of one mesh: the smallest power of two
+        that holds min(waiting, unsteady_batch) of them, at most unsteady_batch."""
+        b = 1
+        while b < min(waiting, self.unsteady_batch):
+            b *= 2
+        return min(b, self.unsteady_batch)
+
+    def _uslot_for(self, key: str, lane: int, now: float, sims: int = 1) -> _Slot | None:
+        s = self.uslots.get((key, lane, sims))
         if s is not None:
             return None if s.busy else s
         t_steps, dt = self.unsteady[key][:2]
         far = self.unsteady[key][2] if len(self.unsteady[key]) > 2 else None
-        need = self._uestimate(key)
+        need = self._uestimate(key, sims)
         if need > self.budget:
             raise RuntimeError(f"mesh {key}: one unsteady plan ({need} bytes) exceeds the memory budget {self.budget}")
         if self.used_bytes + need > self.budget and not self._evict_for(need, now):
             return None
-        plan = UVLMPlan(self.meshes[key], t_steps, dt, mode=MODE_FAST, stream=self.lanes[lane])
+        plan = UVLMPlan(self.meshes[key], t_steps, dt, mode=MODE_FAST, stream=self.lanes[lane], sims=sims)
         if far:
             plan.set_far_wake(far)
         s = _Slot(plan, need, unsteady=True)
-        self.uslots[(key, lane)] = s
+        self.uslots[(key, lane, sims)] = s
         self.used_bytes += need
         self.stats.plans_created += 1
         self.stats.peak_bytes = max(self.stats.peak_bytes, self.used_bytes)
@@ -399,19 +424,19 @@ class Server:
                 if not slot.plan.done():
                     continue
                 if slot.unsteady:
-                    res = slot.plan.results()
-                    r = batch[0]
-                    r.finished = time.perf_counter() - t0
-                    r.cl_t, r.cm_t = res["cl"], res["cm"]
-                    r.cl = float(res["cl"][-1])
-                    r.cm = res["cm"][-1].copy()
-                    if self.keep_gamma:
-                        r.gamma = res["gamma"]
+                    for j, r in enumerate(batch):
+                        res = slot.plan.results(sim=j)
+                        r.finished = time.perf_counter() - t0
+                        r.cl_t, r.cm_t = res["cl"], res["cm"]
+                        r.cl = float(res["cl"][-1])
+                        r.cm = res["cm"][-1].copy()
+                        if self.keep_gamma:
+                            r.gamma = res["gamma"]
                     slot.busy = False
-                    slot.last_used = r.finished
+                    slot.last_used = batch[-1].finished
                     del running[lane]
                     free_lanes.append(lane)
-                    finished += 1
+                    finished += len(batch)
                     progressed = True
                     continue
                 res = slot.plan.results(gamma=self.keep_gamma)
@@ -441,7 +466,7 @@ class Server:
                 for _, _, k in heads:
                     kind, mesh = k
                     if kind == "unsteady":
-                        slot = self._uslot_for(mesh, free_lanes[0], now)
+                        slot = self._uslot_for(mesh, free_lanes[0], now, self.usims(len(waiting[k])))
                     else:
                         slot = self._slot_for(mesh, free_lanes[0], now)
                     if slot is not None:
@@ -453,15 +478,19 @@ class Server:
                 lane = free_lanes.popleft()
                 q = waiting[(kind, mesh)]
                 if kind == "unsteady":
-                    r = q.popleft()
+                    sims = slot.plan.sims
+                    batch = [q.popleft() for _ in range(min(sims, len(q)))]
                     slot.busy = True
-                    slot.plan.set_pitch_heave(**r.motion)
+                    for j in range(sims):  # unused simulations repeat the first motion
+                        slot.plan.set_pitch_heave(**batch[j if j < len(batch) else 0].motion, sim=j)
                     slot.plan.run(use_graph=True)
-                    r.admitted = time.perf_counter() - t0
-                    r.batch, r.lane = 1, lane
-                    running[lane] = (slot, [r])
+                    t = time.perf_counter() - t0
+                    for r in batch:
+                        r.admitted = t
+                        r.batch, r.lane = len(batch), lane
+                    running[lane] = (slot, batch)
                     st.batches += 1
-                    st.batch_sizes.append(1)
+                    st.batch_sizes.append(len(batch))
                     st.max_running = max(st.max_running, len(running))
                     progressed = True
                     continue

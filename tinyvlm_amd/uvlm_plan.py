@@ -40,11 +40,15 @@ def motion_pitch_heave(n_surf, t_steps, dt, u, h_amp=0.0, h_phase=0.0, a_amp=0.0
 
 
 class UVLMPlan:
-    """One reusable device-resident simulation (buffers, LU, wake, optional captured hipGraph)."""
+    """One reusable device-resident simulation (buffers, LU, wake, optional captured hipGraph).
 
-    def __init__(self, verts_init_list, t_steps, dt, mode=MODE_FAST, stream=None):
+    sims > 1 (fast mode): that many simulations of the mesh, each with its own motion, advanced by
+    the same launches (tvlm_uvlm_create_batched); ``sim`` arguments pick one, None means all."""
+
+    def __init__(self, verts_init_list, t_steps, dt, mode=MODE_FAST, stream=None, sims=1):
         import torch
 
+        self.sims = int(sims)
         self.n_surf = len(verts_init_list)
         self.t_steps = int(t_steps)
         self.dt = float(dt)
@@ -58,9 +62,10 @@ class UVLMPlan:
         nc = (ctypes.c_int64 * self.n_surf)(*[v.shape[1] - 1 for v in self._verts])
         ptrs = (ctypes.c_void_p * self.n_surf)(*[v.ctypes.data for v in self._verts])
         self._plan = ctypes.c_void_p()
-        H.check(H.lib().tvlm_uvlm_create(ctypes.byref(self._plan), ctypes.c_int32(self.n_surf), ns, nc, ptrs,
-                                         ctypes.c_int64(self.t_steps), ctypes.c_double(self.dt),
-                                         ctypes.c_int32(mode), H.stream_ptr(self.stream)), "tvlm_uvlm_create")
+        H.check(H.lib().tvlm_uvlm_create_batched(ctypes.byref(self._plan), ctypes.c_int32(self.n_surf), ns, nc, ptrs,
+                                                 ctypes.c_int64(self.t_steps), ctypes.c_double(self.dt),
+                                                 ctypes.c_int32(mode), ctypes.c_int32(self.sims),
+                                                 H.stream_ptr(self.stream)), "tvlm_uvlm_create_batched")
 
     def set_far_wake(self, levels):
         """Far-wake aggregation of the fast mode: levels = [(age, row_group, span_group), ...] (None or
@@ -76,15 +81,22 @@ class UVLMPlan:
                 "tvlm_uvlm_set_lattice_target")
         self.lattice_target = int(groups)
 
-    def set_pitch_heave(self, u, h_amp=0.0, h_phase=0.0, a_amp=0.0, a_phase=0.0, omega=0.0, x_pivot=0.25):
+    def set_pitch_heave(self, u, h_amp=0.0, h_phase=0.0, a_amp=0.0, a_phase=0.0, omega=0.0, x_pivot=0.25,
+                        sim=None):
         d = ctypes.c_double
-        H.check(H.lib().tvlm_uvlm_set_pitch_heave(self._plan, d(u), d(h_amp), d(h_phase), d(a_amp), d(a_phase),
-                                                  d(omega), d(x_pivot)), "tvlm_uvlm_set_pitch_heave")
+        z = -1 if sim is None else int(sim)
+        H.check(H.lib().tvlm_uvlm_set_pitch_heave_sim(self._plan, ctypes.c_int32(z), d(u), d(h_amp), d(h_phase),
+                                                      d(a_amp), d(a_phase), d(omega), d(x_pivot)),
+                "tvlm_uvlm_set_pitch_heave_sim")
 
-    def set_motion(self, T, dT, fs, ref):
+    def set_motion(self, T, dT, fs, ref, sim=None):
         arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (T, dT, fs, ref)]
-        H.check(H.lib().tvlm_uvlm_set_motion(self._plan, *[ctypes.c_void_p(a.ctypes.data) for a in arrs]),
-                "tvlm_uvlm_set_motion")
+        ptrs = [ctypes.c_void_p(a.ctypes.data) for a in arrs]
+        if sim is None:
+            H.check(H.lib().tvlm_uvlm_set_motion(self._plan, *ptrs), "tvlm_uvlm_set_motion")
+        else:
+            H.check(H.lib().tvlm_uvlm_set_motion_sim(self._plan, ctypes.c_int32(int(sim)), *ptrs),
+                    "tvlm_uvlm_set_motion_sim")
 
     def run(self, use_graph=None):
         """One whole simulation, asynchronous.  use_graph (default: the served mode's hipGraph replay;
@@ -93,14 +105,15 @@ class UVLMPlan:
             use_graph = self.mode == MODE_FAST
         H.check(H.lib().tvlm_uvlm_run(self._plan, ctypes.c_int32(int(use_graph))), "tvlm_uvlm_run")
 
-    def results(self):
+    def results(self, sim=0):
         n = self.t_steps - 1
         cl = np.zeros(n)
         cm = np.zeros((n, 3))
         g = np.zeros(self.N)
         info = ctypes.c_int32(0)
-        H.check(H.lib().tvlm_uvlm_results(self._plan, ctypes.c_void_p(cl.ctypes.data), ctypes.c_void_p(cm.ctypes.data),
-                                          ctypes.c_void_p(g.ctypes.data), ctypes.byref(info)), "tvlm_uvlm_results")
+        H.check(H.lib().tvlm_uvlm_results_sim(self._plan, ctypes.c_int32(int(sim)), ctypes.c_void_p(cl.ctypes.data),
+                                              ctypes.c_void_p(cm.ctypes.data), ctypes.c_void_p(g.ctypes.data),
+                                              ctypes.byref(info)), "tvlm_uvlm_results_sim")
         if info.value != 0:
             raise H.TvlmError(f"singular influence matrix (getrf info {info.value})")
         return {"cl": cl, "cm": cm, "gamma": g}
