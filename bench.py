@@ -17,6 +17,10 @@ loop as one hipGraph; it is accepted only if |dCL| <= 1e-6 at every step against
 (exact FP64 Biot-Savart ring by ring over the whole wake, LU substitution each step), which is timed
 too, and so is the served path with the exact wake ("served_exact_wake").
 
+Also reported under "served_uvlm": 64 such simulations requested at once through the serving loop
+(4 lanes, each running up to 8 simulations in one plan); under "batched_uvlm": one plan holding 64
+simulations replayed as one graph, with the roofline of its wake launches (each covers all 64).
+
 Also reported: the roofline of the dominant kernel (wake induction, 398 algorithmic flops per
 receiver x wake-ring pair -- the ring-by-ring count of the reference algorithm -- timed with HIP
 events on the plan's stream, against the FP64 VALU peak since it computes in FP64) and the CPU
@@ -352,30 +356,62 @@ def lu_probe(n, reps=2):
     return out
 
 
-def serve_uvlm_bench(verts, dt, cl_alone, plain_value, n_requests, lanes, far=None):
+def serve_uvlm_bench(verts, dt, cl_alone, plain_value, n_requests, lanes, far=None, batch=1):
     """Unsteady requests through the serving loop: n_requests simulations of the headline workload
     (same wing, motion and 500 steps, the headline's wake: far-wake levels `far` or exact) all waiting
-    at the start, `lanes` lanes, one served UVLM plan per lane (hipGraph replay).  value = n_requests x
-    500 / (last completion - start); each CL(t) history is compared with the single served simulation
-    of the headline measurement."""
+    at the start, `lanes` lanes, each lane running up to `batch` simulations in one plan (hipGraph
+    replay).  value = n_requests x 500 / (last completion - start); each CL(t) history is compared
+    with the single served simulation of the headline measurement."""
     from tinyvlm_amd import serve as S
 
     reqs = [S.Request(i, "ar4_10x40", kind="unsteady", motion=dict(MOTION)) for i in range(n_requests)]
     cfg = (T_STEPS, dt, far) if far else (T_STEPS, dt)
-    with S.Server({"ar4_10x40": [verts]}, lanes=lanes, unsteady={"ar4_10x40": cfg}) as srv:
-        srv.run([S.Request(-1 - k, "ar4_10x40", kind="unsteady", motion=dict(MOTION)) for k in range(lanes)],
-                realtime=False)  # warm-up: one plan per lane, graphs captured
+    with S.Server({"ar4_10x40": [verts]}, lanes=lanes, unsteady={"ar4_10x40": cfg}, unsteady_batch=batch) as srv:
+        # warm-up: every lane's plan of `batch` simulations created and its graph captured
+        srv.run([S.Request(-1 - k, "ar4_10x40", kind="unsteady", motion=dict(MOTION)) for k in range(lanes * batch)],
+                realtime=False)
         srv.stats = S.ServeStats()
         st = srv.run(reqs, realtime=False)
     span = max(r.finished for r in reqs)
     p50, p99 = S.latency_percentiles(reqs)
     value = n_requests * (T_STEPS - 1) / span
     return {"workload": "uvlm_pitch_heave_ar4_10c_x_40s_500steps, concurrent requests", "requests": n_requests,
-            "lanes": lanes, "value": round(value, 1), "unit": "UVLM steps/s (all requests)",
+            "lanes": lanes, "simulations_per_plan": batch, "batch_sizes": sorted(set(st.batch_sizes)),
+            "value": round(value, 1), "unit": "UVLM steps/s (all requests)",
             "latency_ms_per_sim": {"p50": round(p50 * 1e3, 2), "p99": round(p99 * 1e3, 2)},
             "max_lanes_busy": st.max_running, "speedup_vs_plain_fp64": round(value / plain_value, 3),
             "wake": far if far else "exact",
+            "max_abs_dCL_vs_alone": max(float(np.max(np.abs(r.cl_t - cl_alone))) for r in reqs),
             "cl_history_bitwise_equal_to_alone": bool(all(np.array_equal(r.cl_t, cl_alone) for r in reqs))}
+
+
+def batched_uvlm_bench(verts, dt, cl_alone, plain_value, sims, reps, far, wake_roofline):
+    """One plan holding `sims` simulations of the headline workload (tvlm_uvlm_create_batched): the
+    whole batch replayed `reps` times; value = sims x 500 x reps / wall.  The wake roofline is taken
+    over the batched launches (every launch covers all simulations)."""
+    import torch
+
+    from tinyvlm_amd.uvlm_plan import MODE_FAST, UVLMPlan
+
+    p = UVLMPlan([verts], T_STEPS, dt, mode=MODE_FAST, sims=sims)
+    if far:
+        p.set_far_wake(far)
+    p.set_pitch_heave(**MOTION)
+    p.run(use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        p.run(use_graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = sims * (T_STEPS - 1) * reps / wall
+    dcl = max(float(np.max(np.abs(p.results(sim=z)["cl"] - cl_alone))) for z in range(sims))
+    rl = wake_roofline(p, wall / reps)
+    p.close()
+    return {"workload": "uvlm_pitch_heave_ar4_10c_x_40s_500steps, one plan of several simulations",
+            "simulations": sims, "batches_timed": reps, "value": round(value, 1), "unit": "UVLM steps/s",
+            "ms_per_batch": round(wall / reps * 1e3, 3), "speedup_vs_plain_fp64": round(value / plain_value, 3),
+            "max_abs_dCL_vs_alone": dcl, "wake": far if far else "exact", "roofline": rl}
 
 
 PEAK_FP32_VALU_PACKED_TFLOPS = 157.3  # MI355X FP32 vector peak with packed FP32 (not used: see DESIGN)
@@ -746,9 +782,11 @@ def main():
     ap.add_argument("--mix-saturation", type=int, default=384, help="requests of the saturation run")
     ap.add_argument("--mix-lanes", type=int, default=3)
     ap.add_argument("--uvlm-lanes", type=int, default=4, help="lanes of the served_uvlm run")
+    ap.add_argument("--uvlm-batch", type=int, default=8, help="simulations per plan of the served_uvlm run")
+    ap.add_argument("--uvlm-sims", type=int, default=64, help="simulations of the batched_uvlm plan (0: skip)")
     ap.add_argument("--no-mix", action="store_true", help="skip the served mix and the getrf probe")
     ap.add_argument("--no-served-uvlm", action="store_true")
-    ap.add_argument("--uvlm-requests", type=int, default=32, help="concurrent UVLM requests of the served_uvlm run")
+    ap.add_argument("--uvlm-requests", type=int, default=64, help="concurrent UVLM requests of the served_uvlm run")
     ap.add_argument("--lu-probe", type=int, default=0, help="only time getrf of one N x N matrix (FP32, FP64)")
     ap.add_argument("--mix-only", action="store_true", help="only the served_mix section")
     ap.add_argument("--hb-speeds", type=int, default=8, help="airspeeds of the timed HB continuation")
@@ -937,7 +975,12 @@ def main():
     plain.close()
     if not args.no_served_uvlm:
         out["served_uvlm"] = serve_uvlm_bench(verts, dt, res_fast["cl"], plain_value, args.uvlm_requests,
-                                              args.uvlm_lanes, None if args.exact_wake else FAR_WAKE)
+                                              args.uvlm_lanes, None if args.exact_wake else FAR_WAKE,
+                                              args.uvlm_batch)
+    if args.uvlm_sims > 0:
+        out["batched_uvlm"] = batched_uvlm_bench(verts, dt, res_fast["cl"], plain_value, args.uvlm_sims,
+                                                 max(2, 128 // args.uvlm_sims),
+                                                 None if args.exact_wake else FAR_WAKE, wake_roofline)
     if not args.no_steady:
         out["steady"] = steady_bench(args.steady_batch, args.steady_reps, args.steady_plain, not args.no_cpu,
                                      args.steady_lanes)

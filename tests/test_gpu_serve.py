@@ -178,6 +178,38 @@ def test_unsteady_requests_in_the_mix(mix, oracle):
             _check_same(r, _alone(meshes, r))
 
 
+def test_unsteady_requests_batched_per_lane(oracle):
+    """unsteady_batch = 4, 2 lanes, 11 simulations of different motions waiting from the start: they
+    run as batches of several simulations per plan; each CL(t), CM(t) agrees with its alone
+    one-simulation plan to 1e-12 relative (the batched LU rounds differently) and with the oracle."""
+    from tinyvlm_amd.serve import Request, Server
+    from tinyvlm_amd.uvlm_plan import MODE_FAST, UVLMPlan, time_step
+
+    wing = oracle.rectangular_wing(8, 10, chord=1.0, half_span=2.0)
+    ts, dt = 61, time_step(wing, 1.0)
+    a_amp = 2.0 * math.pi / 180.0
+    motions = [dict(u=1.0, h_amp=0.05 * (1 + k % 4), h_phase=0.1 * k, a_amp=a_amp * (k % 3 + 1), a_phase=math.pi / 2,
+                    omega=1.0 + 0.125 * k, x_pivot=0.25) for k in range(11)]
+    requests = [Request(k, "w", kind="unsteady", motion=m) for k, m in enumerate(motions)]
+    print(f"[case] serve {len(requests)} unsteady requests, batches of up to 4 on 2 lanes", flush=True)
+    with Server({"w": [wing]}, lanes=2, unsteady={"w": (ts, dt)}, unsteady_batch=4) as srv:
+        st = srv.run(requests, realtime=False)
+        assert all(k[2] in (1, 2, 4) for k in srv.uslots)
+    assert max(st.batch_sizes) == 4 and sum(st.batch_sizes) == 11
+    for r in requests:
+        p = UVLMPlan([wing], ts, dt, mode=MODE_FAST)
+        p.set_pitch_heave(**r.motion)
+        p.run(use_graph=True)
+        alone = p.results()
+        p.close()
+        scale = np.max(np.abs(alone["cl"]))
+        assert np.max(np.abs(r.cl_t - alone["cl"])) <= 1e-12 * scale, r.rid
+        assert np.max(np.abs(r.cm_t - alone["cm"])) <= 1e-12 * max(np.max(np.abs(alone["cm"])), 1e-300), r.rid
+        mot = [r.motion[k] for k in ("u", "h_amp", "h_phase", "a_amp", "a_phase", "omega", "x_pivot")]
+        cl_o, _, _, _ = oracle.uvlm_run([wing], mot, t_steps=ts)
+        assert np.max(np.abs(r.cl_t[1:] - cl_o[1:])) <= 1e-9, r.rid
+
+
 def test_large_mix_meshes_through_the_server(tmp_path):
     """The mix's large meshes (N = 4096, 8100, 16384: rectangular 64 x 64, swept 90 x 90, the
     pivot-heavy elliptic 128 x 128) served beside small ones through Server: every request equals
