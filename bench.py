@@ -385,7 +385,27 @@ def serve_uvlm_bench(verts, dt, cl_alone, plain_value, n_requests, lanes, far=No
             "cl_history_bitwise_equal_to_alone": bool(all(np.array_equal(r.cl_t, cl_alone) for r in reqs))}
 
 
-def batched_uvlm_bench(verts, dt, cl_alone, plain_value, sims, reps, far, wake_roofline):
+def wake_roofline(plan, sim_s):
+    """Roofline of the dominant kernel: live HIP events around each wake launch on the plan's stream."""
+    wake_ms, launches, pairs = plan.time_wake()
+    avg_ms = wake_ms / launches
+    achieved = FLOPS_PER_PAIR * (pairs / launches) / (avg_ms * 1e-3) / 1e12
+    return {
+        "bound": "valu",
+        "kernel": "k_wake_lattice (FP64, seeded rsqrt/rcp, shared-segment lattice)",
+        "achieved": round(achieved, 3),
+        "peak": PEAK_FP64_VALU_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP64_VALU_TFLOPS, 4),
+        "avg_launch_us": round(avg_ms * 1e3, 3),
+        "launches": launches,
+        "pairs_per_launch": pairs / launches,
+        "flops_per_pair": FLOPS_PER_PAIR,
+        "wake_share_of_sim": round(wake_ms / 1e3 / sim_s, 4),
+    }
+
+
+def batched_uvlm_bench(verts, dt, cl_alone, plain_value, sims, reps, far):
     """One plan holding `sims` simulations of the headline workload (tvlm_uvlm_create_batched): the
     whole batch replayed `reps` times; value = sims x 500 x reps / wall.  The wake roofline is taken
     over the batched launches (every launch covers all simulations)."""
@@ -784,6 +804,7 @@ def main():
     ap.add_argument("--uvlm-lanes", type=int, default=4, help="lanes of the served_uvlm run")
     ap.add_argument("--uvlm-batch", type=int, default=8, help="simulations per plan of the served_uvlm run")
     ap.add_argument("--uvlm-sims", type=int, default=64, help="simulations of the batched_uvlm plan (0: skip)")
+    ap.add_argument("--batched-only", action="store_true", help="only the batched_uvlm section (2 batches timed)")
     ap.add_argument("--no-mix", action="store_true", help="skip the served mix and the getrf probe")
     ap.add_argument("--no-served-uvlm", action="store_true")
     ap.add_argument("--uvlm-requests", type=int, default=64, help="concurrent UVLM requests of the served_uvlm run")
@@ -832,6 +853,19 @@ def main():
     verts = rectangular_wing_f32(NS, NC, CHORD, HALF_SPAN)
     dt = time_step(verts, MOTION["u"])
     steps_per_sim = T_STEPS - 1
+    if args.batched_only:
+        one = UVLMPlan([verts], T_STEPS, dt, mode=MODE_FAST)
+        if not args.exact_wake:
+            one.set_far_wake(FAR_WAKE)
+        one.set_pitch_heave(**MOTION)
+        one.run(use_graph=True)
+        cl_one = one.results()["cl"]
+        one.close()
+        sec = batched_uvlm_bench(verts, dt, cl_one, float("nan"), args.uvlm_sims, 2,
+                                 None if args.exact_wake else FAR_WAKE)
+        sec.pop("speedup_vs_plain_fp64")
+        print(json.dumps(sec))
+        return
 
     # served path: seeded FP64 wake induction with far-wake aggregation, FP64 inverse solve, hipGraph replay
     fast = UVLMPlan([verts], T_STEPS, dt, mode=MODE_FAST)
@@ -866,25 +900,6 @@ def main():
     plain_value = steps_per_sim * args.plain_steps / wall_p
     dcl = float(np.max(np.abs(res_fast["cl"] - res_plain["cl"])))
     dcm = float(np.max(np.abs(res_fast["cm"] - res_plain["cm"])))
-
-    # roofline of the dominant kernel: live HIP events around each wake launch on the plan stream
-    def wake_roofline(plan, sim_s):
-        wake_ms, launches, pairs = plan.time_wake()
-        avg_ms = wake_ms / launches
-        achieved = FLOPS_PER_PAIR * (pairs / launches) / (avg_ms * 1e-3) / 1e12
-        return {
-            "bound": "valu",
-            "kernel": "k_wake_lattice (FP64, seeded rsqrt/rcp, shared-segment lattice)",
-            "achieved": round(achieved, 3),
-            "peak": PEAK_FP64_VALU_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_VALU_TFLOPS, 4),
-            "avg_launch_us": round(avg_ms * 1e3, 3),
-            "launches": launches,
-            "pairs_per_launch": pairs / launches,
-            "flops_per_pair": FLOPS_PER_PAIR,
-            "wake_share_of_sim": round(wake_ms / 1e3 / sim_s, 4),
-        }
 
     def counters(rl, far):
         c = WAKE_COUNTERS["far" if far else "exact"]
@@ -980,7 +995,7 @@ def main():
     if args.uvlm_sims > 0:
         out["batched_uvlm"] = batched_uvlm_bench(verts, dt, res_fast["cl"], plain_value, args.uvlm_sims,
                                                  max(2, 128 // args.uvlm_sims),
-                                                 None if args.exact_wake else FAR_WAKE, wake_roofline)
+                                                 None if args.exact_wake else FAR_WAKE)
     if not args.no_steady:
         out["steady"] = steady_bench(args.steady_batch, args.steady_reps, args.steady_plain, not args.no_cpu,
                                      args.steady_lanes)
