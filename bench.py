@@ -6,20 +6,24 @@ AR 4 (chord 1, half span 2), 10 chordwise x 40 spanwise vortex-ring panels (N = 
 0.1 c sin(t), pitch 2 deg sin(t + pi/2) about the quarter chord, reduced frequency k = 0.5,
 dt = c / (10 u_inf) = last panel chord / u_inf, 500 time steps; the wake grows to 500 x 40 rings.
 
-A bench "step" is one whole simulation (motion evaluation, pose, influence matrix, LU, and 500
-UVLM time steps with wake induction, solve, forces, CL and CM), all inside the HIP library.
-value = UVLM time steps per second = 500 * K / (time of K simulations), inputs resident on the
-device.  The served path evaluates the wake induction with FP64 arithmetic (1/sqrt and 1/x from
-the FP64 hardware estimates plus one Newton step) over the shared-segment lattice of the wake, with
-the far wake aggregated (ring rows older than 80 / 160 / 320 steps merged into 2x4 / 4x8 / 8x8
-rings of mean circulation; --exact-wake turns this off), solves with the FP64 inverse and replays the
-loop as one hipGraph; it is accepted only if |dCL| <= 1e-6 at every step against the plain FP64 path
-(exact FP64 Biot-Savart ring by ring over the whole wake, LU substitution each step), which is timed
-too, and so is the served path with the exact wake ("served_exact_wake").
+A bench "step" is one batch: 64 simulations of this workload (independent requests of the same
+configuration, each computed in full) held by one plan and advanced together by every launch (motion
+evaluation and upload before timing; then per batch: pose, influence matrix, LU, inverse, geometry of
+every step, and 500 UVLM time steps with wake induction, solve, forces, CL and CM for all 64), all
+inside the HIP library.  value = UVLM time steps per second = 64 * 500 * K / (time of K batches),
+inputs resident on the device.  The served path evaluates the wake induction with FP64 arithmetic (1/sqrt
+and 1/x from the FP64 hardware estimates plus one Newton step) over the shared-segment lattice of the
+wake, with the far wake aggregated (ring rows older than 80 / 160 / 320 steps merged into 2x4 / 4x8 /
+8x8 rings of mean circulation; --exact-wake turns this off), solves with the FP64 inverse and replays
+the loop as one hipGraph; it is accepted only if |dCL| <= 1e-6 at every step of every simulation
+against the plain FP64 path (exact FP64 Biot-Savart ring by ring over the whole wake, LU substitution
+each step, one simulation at a time), which is timed too, and so is the served batch with the exact
+wake ("served_exact_wake") and one simulation per plan ("single_simulation": the latency a lone
+request sees).
 
 Also reported under "served_uvlm": 64 such simulations requested at once through the serving loop
-(4 lanes, each running up to 8 simulations in one plan); under "batched_uvlm": one plan holding 64
-simulations replayed as one graph, with the roofline of its wake launches (each covers all 64).
+(4 lanes, each running up to 8 simulations in one plan).  --batched-only prints only a batch's rate
+and wake roofline (for profiler runs).
 
 Also reported: the roofline of the dominant kernel (wake induction, 398 algorithmic flops per
 receiver x wake-ring pair -- the ring-by-ring count of the reference algorithm -- timed with HIP
@@ -70,9 +74,13 @@ COPY_PROBE_BYTES = 1 << 30
 # pair from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 pass.  Counters cannot be read in-process.
 # "exact": the served loop without far-wake aggregation, measured when each launch still chose its
 # rows per tile alone (38.6 us per launch then, 38.0 us with the joint launch shape).
+# "*_sims64": one plan of 64 simulations (the bench step), each launch covering all 64, over
+# `bench.py --probe` and `bench.py --probe --exact-wake` (one batch of 64).
 WAKE_COUNTERS = {
     "exact": {"traffic_bytes": 3008144, "algorithmic_bytes": 349384, "executed_flops_per_pair": 252.7},
     "far": {"traffic_bytes": 2465922, "algorithmic_bytes": 188327, "executed_flops_per_pair": 302.5},
+    "far_sims64": {"traffic_bytes": 121996697, "algorithmic_bytes": 12052939, "executed_flops_per_pair": 241.2},
+    "exact_sims64": {"traffic_bytes": 174869909, "algorithmic_bytes": 22360576, "executed_flops_per_pair": 237.8},
 }
 NS, NC, HALF_SPAN, CHORD = 40, 10, 2.0, 1.0
 T_STEPS = 501  # UVLM::run loops t_steps-1 times -> 500 time steps
@@ -102,7 +110,9 @@ def rectangular_wing_f32(ns, nc, chord, half_span):
     return np.asfortranarray(v.astype(np.float64))
 
 
-def run_plan(plan, k, use_graph):
+def run_plan(plan, k, use_graph, motion=True):
+    """k whole simulations (or batches of them) one after another; motion=True evaluates and uploads the
+    motion before each (the single-simulation form), False keeps the motion already on the device."""
     import torch
 
     lat = []
@@ -114,7 +124,8 @@ def run_plan(plan, k, use_graph):
         start.record(plan.stream)
         for _ in range(k):
             t1 = time.perf_counter()
-            plan.set_pitch_heave(**MOTION)
+            if motion:
+                plan.set_pitch_heave(**MOTION)
             plan.run(use_graph=use_graph)
             plan.stream.synchronize()
             lat.append((time.perf_counter() - t1) * 1e3)
@@ -785,7 +796,7 @@ def flutter_cpu_sample(v, R, dt, iters_mean):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed simulations of 500 UVLM steps")
+    ap.add_argument("--steps", type=int, default=20, help="timed batches (of --uvlm-sims simulations of 500 steps)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--plain-steps", type=int, default=3, help="timed plain FP64 simulations")
     ap.add_argument("--cpu-steps", type=int, default=500, help="UVLM steps of the CPU baseline sample")
@@ -803,7 +814,7 @@ def main():
     ap.add_argument("--mix-lanes", type=int, default=3)
     ap.add_argument("--uvlm-lanes", type=int, default=4, help="lanes of the served_uvlm run")
     ap.add_argument("--uvlm-batch", type=int, default=8, help="simulations per plan of the served_uvlm run")
-    ap.add_argument("--uvlm-sims", type=int, default=64, help="simulations of the batched_uvlm plan (0: skip)")
+    ap.add_argument("--uvlm-sims", type=int, default=64, help="simulations per plan of the bench step (the batch)")
     ap.add_argument("--batched-only", action="store_true", help="only the batched_uvlm section (2 batches timed)")
     ap.add_argument("--no-mix", action="store_true", help="skip the served mix and the getrf probe")
     ap.add_argument("--no-served-uvlm", action="store_true")
@@ -867,42 +878,51 @@ def main():
         print(json.dumps(sec))
         return
 
-    # served path: seeded FP64 wake induction with far-wake aggregation, FP64 inverse solve, hipGraph replay
-    fast = UVLMPlan([verts], T_STEPS, dt, mode=MODE_FAST)
-    if not args.exact_wake:
-        fast.set_far_wake(FAR_WAKE)
+    B = args.uvlm_sims
+    far = None if args.exact_wake else FAR_WAKE
     if args.probe:
-        run_plan(fast, 1, use_graph=False)
+        # one served batch without graph replay + a device copy (the calibration kernel of the counter passes)
+        pr = UVLMPlan([verts], T_STEPS, dt, mode=MODE_FAST, sims=B)
+        if far:
+            pr.set_far_wake(far)
+        pr.set_pitch_heave(**MOTION)
+        run_plan(pr, 1, use_graph=False, motion=False)
         src = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"probe": "uvlm served simulation (no graph) + device copy",
-                          "copy_bytes_each_way": COPY_PROBE_BYTES, "cl_last": float(fast.results()["cl"][-1])}))
-        fast.close()
+        print(json.dumps({"probe": f"uvlm served batch of {B} simulations (no graph) + device copy",
+                          "copy_bytes_each_way": COPY_PROBE_BYTES, "cl_last": float(pr.results()["cl"][-1])}))
+        pr.close()
         return
-    # tile-size autotuner: the wake launches' work-group target for this workload (cached in memory;
-    # the choice also becomes the library-wide default that the served_uvlm section's plans take)
     from tinyvlm_amd import autotune as AT
 
+    # served path (the bench step): one plan holding B simulations of the workload -- B independent
+    # requests of the same configuration -- advanced together by every launch: seeded FP64 wake induction
+    # with far-wake aggregation, FP64 inverse solve, hipGraph replay.  The motion is uploaded once
+    # before timing (inputs resident); each timed batch redoes the setup (pose, LHS, LU, inverse,
+    # geometry of every step) and the 500 steps of all B simulations.
+    fast = UVLMPlan([verts], T_STEPS, dt, mode=MODE_FAST, sims=B)
+    if far:
+        fast.set_far_wake(far)
     fast.set_pitch_heave(**MOTION)
-    tuned = AT.tune_uvlm(fast, cache=AT.Cache(None), apply_globally=True)
-    _, _, _ = run_plan(fast, max(args.warmup, 1), use_graph=True)
-    wall, gpu_s, lat = run_plan(fast, args.steps, use_graph=True)
-    res_fast = fast.results()
-    value = steps_per_sim * args.steps / wall
+    tuned = AT.tune_uvlm(fast, cache=AT.Cache(None))
+    run_plan(fast, max(args.warmup, 1), use_graph=True, motion=False)
+    wall, gpu_s, lat = run_plan(fast, args.steps, use_graph=True, motion=False)
+    res_all = [fast.results(sim=z) for z in range(B)]
+    value = B * steps_per_sim * args.steps / wall
 
-    # plain FP64 path (exact FP64 kernels, LU substitution each step, no graph)
+    # plain FP64 path (exact FP64 kernels, LU substitution each step, no graph), one simulation at a time
     plain = UVLMPlan([verts], T_STEPS, dt, mode=MODE_PLAIN_FP64)
     run_plan(plain, 1, use_graph=False)
     wall_p, _, lat_p = run_plan(plain, args.plain_steps, use_graph=False)
     res_plain = plain.results()
     plain_value = steps_per_sim * args.plain_steps / wall_p
-    dcl = float(np.max(np.abs(res_fast["cl"] - res_plain["cl"])))
-    dcm = float(np.max(np.abs(res_fast["cm"] - res_plain["cm"])))
+    dcl = max(float(np.max(np.abs(r["cl"] - res_plain["cl"]))) for r in res_all)
+    dcm = max(float(np.max(np.abs(r["cm"] - res_plain["cm"]))) for r in res_all)
 
-    def counters(rl, far):
-        c = WAKE_COUNTERS["far" if far else "exact"]
+    def counters(rl, key):
+        c = WAKE_COUNTERS.get(key)
         if c is None:
             rl.update({"traffic": None, "traffic_note": "no counter pass recorded for this wake form"})
             return rl
@@ -916,28 +936,53 @@ def main():
         })
         return rl
 
-    roofline = counters(wake_roofline(fast, wall / args.steps), not args.exact_wake)
-    roofline["pairs_note"] = ("pairs = receivers x wake rings evaluated per launch (with far-wake aggregation: the "
-                              "exact rings plus the coarse rings); 398 flops per pair, the ring-by-ring count")
+    wkey = ("exact" if args.exact_wake else "far") + (f"_sims{B}" if B > 1 else "")
+    roofline = counters(wake_roofline(fast, wall / args.steps), wkey)
+    roofline["pairs_note"] = (f"pairs = receivers x wake rings evaluated per launch, over all {B} simulations "
+                              "(with far-wake aggregation: the exact rings plus the coarse rings); 398 flops per "
+                              "pair, the ring-by-ring count")
     wake_ms_p, launches_p, pairs_p = plain.time_wake()
     roofline["plain_fp64_wake_tflops"] = round(FLOPS_PER_PAIR * pairs_p / (wake_ms_p * 1e-3) / 1e12, 3)
+    fast.close()
 
-    # the same served loop with the exact wake (every ring evaluated), for comparison
+    # the latency path: one simulation per plan (what a lone request sees)
+    one = UVLMPlan([verts], T_STEPS, dt, mode=MODE_FAST)
+    if far:
+        one.set_far_wake(far)
+    one.set_pitch_heave(**MOTION)
+    tuned1 = AT.tune_uvlm(one, cache=AT.Cache(None), apply_globally=True)
+    run_plan(one, max(args.warmup, 1), use_graph=True)
+    wall1, _, lat1 = run_plan(one, args.steps, use_graph=True)
+    res_one = one.results()
+    value1 = steps_per_sim * args.steps / wall1
+    single = {"value": round(value1, 2), "unit": "UVLM steps/s", "sims_timed": args.steps,
+              "speedup_vs_plain_fp64": round(value1 / plain_value, 3),
+              "latency_ms_per_sim": {"p50": round(float(np.percentile(lat1, 50)), 4),
+                                     "p99": round(float(np.percentile(lat1, 99)), 4)},
+              "max_abs_dCL_vs_plain_fp64": float(np.max(np.abs(res_one["cl"] - res_plain["cl"]))),
+              "note": "motion evaluated and uploaded before each simulation, inside the timed loop",
+              "roofline": counters(wake_roofline(one, wall1 / args.steps), "exact" if args.exact_wake else "far"),
+              "autotune": {"lattice_target": tuned1["best"], "sim_ms": tuned1["sim_ms"]}}
+    one.close()
+
+    # the same served batch with the exact wake (every ring evaluated), for comparison
     exact_sec = None
     if not args.exact_wake:
-        ex = UVLMPlan([verts], T_STEPS, dt, mode=MODE_FAST)
+        ex = UVLMPlan([verts], T_STEPS, dt, mode=MODE_FAST, sims=B)
         ex.set_pitch_heave(**MOTION)
-        tuned_x = AT.tune_uvlm(ex, cache=AT.Cache(None))
-        run_plan(ex, 1, use_graph=True)
-        wall_x, _, lat_x = run_plan(ex, args.steps, use_graph=True)
-        res_x = ex.results()
-        value_x = steps_per_sim * args.steps / wall_x
-        exact_sec = {"value": round(value_x, 2), "unit": "UVLM steps/s",
-                     "speedup_vs_plain_fp64": round(value_x / plain_value, 3),
-                     "max_abs_dCL_vs_plain_fp64": float(np.max(np.abs(res_x["cl"] - res_plain["cl"]))),
-                     "latency_ms_per_sim_p50": round(float(np.percentile(lat_x, 50)), 4),
-                     "roofline": counters(wake_roofline(ex, wall_x / args.steps), False),
-                     "autotune": {"lattice_target": tuned_x["best"], "sim_ms": tuned_x["sim_ms"]}}
+        tuned_x = AT.tune_uvlm(ex, cache=AT.Cache(None), reps=1)
+        kx = max(2, args.steps // 4)
+        run_plan(ex, 1, use_graph=True, motion=False)
+        wall_x, _, lat_x = run_plan(ex, kx, use_graph=True, motion=False)
+        res_x = [ex.results(sim=z) for z in range(B)]
+        value_x = B * steps_per_sim * kx / wall_x
+        exact_sec = {"value": round(value_x, 2), "unit": "UVLM steps/s", "simulations_per_batch": B,
+                     "batches_timed": kx, "speedup_vs_plain_fp64": round(value_x / plain_value, 3),
+                     "max_abs_dCL_vs_plain_fp64": max(float(np.max(np.abs(r["cl"] - res_plain["cl"])))
+                                                      for r in res_x),
+                     "ms_per_batch": round(wall_x / kx * 1e3, 3),
+                     "roofline": counters(wake_roofline(ex, wall_x / kx), f"exact_sims{B}" if B > 1 else "exact"),
+                     "autotune": {"lattice_target": tuned_x["best"], "batch_ms": tuned_x["sim_ms"]}}
         ex.close()
 
     out = {
@@ -957,22 +1002,28 @@ def main():
             "workload": "uvlm_pitch_heave_ar4_10c_x_40s_500steps",
             "panels": NS * NC, "chordwise": NC, "spanwise": NS, "time_steps_per_sim": steps_per_sim,
             "wake_rings_final": NS * steps_per_sim, "heave": "0.1c sin(t)", "pitch": "2deg sin(t+pi/2)",
-            "reduced_frequency": 0.5, "dt": dt, "bench_step": "one whole 500-step simulation",
+            "reduced_frequency": 0.5, "dt": dt, "simulations_per_batch": B,
+            "bench_step": f"one batch: {B} simulations of this workload (independent requests of the same "
+                          "configuration, each computed in full) advanced together by one plan, 500 steps each",
             "wake": "exact" if args.exact_wake else {"far_wake_aggregation": FAR_WAKE,
                                                      "levels": "(age steps, row group, span group)"},
         },
         "plain_fp64": {"value": round(plain_value, 2), "unit": "UVLM steps/s", "sims": args.plain_steps,
+                       "note": "one simulation at a time (the drop-in path)",
                        "speedup_of_served_path": round(value / plain_value, 3)},
         "accuracy": {"max_abs_dCL_vs_plain_fp64": dcl, "max_abs_dCM_vs_plain_fp64": dcm, "bar": 1e-6,
-                     "accepted": bool(dcl <= 1e-6)},
+                     "accepted": bool(dcl <= 1e-6), "over": f"every one of the {B} simulations"},
         "latency_ms_per_sim": {"p50": round(float(np.percentile(lat, 50)), 4),
                                "p99": round(float(np.percentile(lat, 99)), 4),
+                               "note": f"a simulation of the batch completes with its batch of {B}",
+                               "single_simulation_p50": single["latency_ms_per_sim"]["p50"],
                                "plain_fp64_p50": round(float(np.percentile(lat_p, 50)), 4)},
         "gpu_event_s": round(gpu_s, 6),
         "roofline": roofline,
     }
-    out["autotune"] = {"lattice_target": tuned["best"], "sim_ms_per_candidate": tuned["sim_ms"],
-                       "note": "tinyvlm_amd.autotune: whole simulations per work-group target; the fastest is used"}
+    out["autotune"] = {"lattice_target": tuned["best"], "batch_ms_per_candidate": tuned["sim_ms"],
+                       "note": "tinyvlm_amd.autotune: whole batches per work-group target; the fastest is used"}
+    out["single_simulation"] = single
     if exact_sec is not None:
         out["served_exact_wake"] = exact_sec
     if not args.no_cpu:
@@ -986,16 +1037,11 @@ def main():
         }
     else:
         out["cpu_baseline"] = None
-    fast.close()
     plain.close()
     if not args.no_served_uvlm:
-        out["served_uvlm"] = serve_uvlm_bench(verts, dt, res_fast["cl"], plain_value, args.uvlm_requests,
+        out["served_uvlm"] = serve_uvlm_bench(verts, dt, res_one["cl"], plain_value, args.uvlm_requests,
                                               args.uvlm_lanes, None if args.exact_wake else FAR_WAKE,
                                               args.uvlm_batch)
-    if args.uvlm_sims > 0:
-        out["batched_uvlm"] = batched_uvlm_bench(verts, dt, res_fast["cl"], plain_value, args.uvlm_sims,
-                                                 max(2, 128 // args.uvlm_sims),
-                                                 None if args.exact_wake else FAR_WAKE)
     if not args.no_steady:
         out["steady"] = steady_bench(args.steady_batch, args.steady_reps, args.steady_plain, not args.no_cpu,
                                      args.steady_lanes)

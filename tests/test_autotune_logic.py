@@ -7,10 +7,11 @@ import numpy as np
 
 
 class _FakePlan:
-    def __init__(self, ns, nc, t_steps, far=None):
+    def __init__(self, ns, nc, t_steps, far=None, sims=1):
         self._verts = [np.zeros((ns + 1, nc + 1, 4))]
         self.t_steps = t_steps
         self.far_wake = far
+        self.sims = sims
 
 
 def test_cache_round_trip_and_corrupt_file(tmp_path):
@@ -40,6 +41,7 @@ def test_uvlm_keys_separate_workloads():
     assert k != A.uvlm_key(_FakePlan(40, 12, 501), dev)
     assert k != A.uvlm_key(_FakePlan(40, 10, 501, [(80, 2, 4)]), dev)
     assert k != A.uvlm_key(_FakePlan(40, 10, 501), "other")
+    assert k != A.uvlm_key(_FakePlan(40, 10, 501, sims=64), dev)
 
 
 def test_tile_target_setters_round_trip_and_range():

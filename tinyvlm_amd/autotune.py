@@ -71,7 +71,8 @@ def device_tag() -> str:
 def uvlm_key(plan, device: str) -> str:
     shapes = ",".join(f"{v.shape[0] - 1}x{v.shape[1] - 1}" for v in plan._verts)
     far = getattr(plan, "far_wake", None) or []
-    return f"uvlm-lattice|{shapes}|steps={plan.t_steps}|far={far}|{device}"
+    sims = getattr(plan, "sims", 1)
+    return f"uvlm-lattice|{shapes}|steps={plan.t_steps}|far={far}|sims={sims}|{device}"
 
 
 def tune_uvlm(plan, candidates=LATTICE_CANDIDATES, cache: Cache | None = None, reps: int = 2,
