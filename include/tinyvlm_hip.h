@@ -197,6 +197,29 @@ int tvlm_lu_set_coop_spin_limit(uint32_t limit);
 int tvlm_set_tile_targets(int32_t wake_groups, int32_t lattice_groups);
 int tvlm_get_tile_targets(int32_t* wake_groups, int32_t* lattice_groups);
 
+/* Grouped assembly: many FP64 influence matrices of any sizes -- different meshes, different
+ * requests -- in ONE persistent launch over a work list of (matrix, section, tile).  Each item is one
+ * Backend::lhs_assemble call (vlm_backend.hpp:44) with its own arguments; the result of every matrix
+ * is bitwise that of tvlm_lhs_assemble_f64 (seeded = 0) or tvlm_lhs_assemble_seeded_f64 (seeded = 1)
+ * on the same item.  Like those, it accumulates into lhs (zero it first).  No counterpart in the
+ * reference, which launches per surface pair (vlm_backend_cuda.cu:193-252).  The group keeps the
+ * items' device pointers (views and the surfaces' data must stay where they are); create uploads
+ * the work list once on `stream` and synchronizes it; assemble launches the first n_items items. */
+typedef struct tvlm_lhs_item {
+    tvlm_view lhs;               /* {N, N} column-major (stride[0] = 1), N = the item's panel count */
+    int32_t n_surf;              /* surfaces of the item; the views below have n_surf entries */
+    const tvlm_view* colloc;     /* host arrays of device views, as tvlm_lhs_assemble_f64 takes them */
+    const tvlm_view* normals;
+    const tvlm_view* verts_wing;
+    const tvlm_view* verts_wake;
+    int32_t iteration;
+} tvlm_lhs_item;
+typedef struct tvlm_lhs_group tvlm_lhs_group;
+int tvlm_lhs_group_create(tvlm_lhs_group** group, int32_t n_items, const tvlm_lhs_item* items, void* stream);
+int tvlm_lhs_group_assemble_f64(tvlm_lhs_group* group, int32_t n_items, int32_t seeded, double sigma,
+                                void* stream);
+int tvlm_lhs_group_destroy(tvlm_lhs_group* group);
+
 /* Backend::lhs_assemble (vlm_backend.hpp:44) in the FP64 seeded form the served steady path uses
  * (FP64 arithmetic; 1/sqrt and 1/x from the FP64 hardware estimates plus one Newton step each; no
  * core model, sigma = 0): same arguments and result as tvlm_lhs_assemble_f64 to about 1e-15

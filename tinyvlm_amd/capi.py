@@ -190,6 +190,49 @@ def lhs_assemble_seeded(lhs, colloc, normals, verts_wing, verts_wake, iteration,
                                              ctypes.c_int32(iteration), stream_ptr(stream)), "lhs_assemble_seeded")
 
 
+class LhsItem(ctypes.Structure):
+    """struct tvlm_lhs_item: one influence matrix of a grouped assembly."""
+
+    _fields_ = [("lhs", View), ("n_surf", ctypes.c_int32), ("colloc", ctypes.POINTER(View)),
+                ("normals", ctypes.POINTER(View)), ("verts_wing", ctypes.POINTER(View)),
+                ("verts_wake", ctypes.POINTER(View)), ("iteration", ctypes.c_int32)]
+
+
+class LhsGroup:
+    """Grouped FP64 assembly (tvlm_lhs_group_*): items = [(lhs, colloc, normals, verts_wing, verts_wake,
+    iteration), ...], each as lhs_assemble takes them, meshes of any sizes; assemble() fills every
+    matrix (the first n_items) in one launch, accumulating like lhs_assemble."""
+
+    def __init__(self, items, stream=None):
+        self._keep = []
+        arr = (LhsItem * len(items))()
+        for k, (lhs, colloc, normals, verts_wing, verts_wake, iteration) in enumerate(items):
+            vs = [views(colloc), views(normals), views(verts_wing), views(verts_wake)]
+            self._keep.extend(vs)
+            arr[k] = LhsItem(view(lhs), len(colloc), vs[0], vs[1], vs[2], vs[3], int(iteration))
+        self._g = ctypes.c_void_p()
+        check(lib().tvlm_lhs_group_create(ctypes.byref(self._g), ctypes.c_int32(len(items)), arr,
+                                          stream_ptr(stream)), "tvlm_lhs_group_create")
+        self.n_items = len(items)
+
+    def assemble(self, n_items=None, seeded=False, sigma=0.0, stream=None):
+        k = self.n_items if n_items is None else int(n_items)
+        check(lib().tvlm_lhs_group_assemble_f64(self._g, ctypes.c_int32(k), ctypes.c_int32(int(seeded)),
+                                                ctypes.c_double(sigma), stream_ptr(stream)),
+              "tvlm_lhs_group_assemble_f64")
+
+    def close(self):
+        if self._g:
+            lib().tvlm_lhs_group_destroy(self._g)
+            self._g = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def rhs_assemble_velocities(rhs, normals, velocities, stream=None):
     check(_fn("rhs_assemble_velocities", rhs)(ctypes.byref(view(rhs)), ctypes.c_int32(len(normals)), views(normals),
                                              views(velocities), stream_ptr(stream)), "rhs_assemble_velocities")

@@ -26,17 +26,16 @@ int validate_surface3(const tvlm_view& v, int64_t a, int64_t b, const char* what
     return TVLM_OK;
 }
 
-// seeded = true (FP64 only): the FP64 seeded form (hardware 1/sqrt and 1/x estimates + one Newton
-// step) instead of the exact form; used by the served steady solve, which refines against the exact
-// FP64 residual anyway.
+// The (receiver surface, ring surface) sections of one influence matrix, checked against the
+// reference's shapes, appended to `out` with their tiles numbered from `tiles` on (advanced).
 template <typename T>
-int lhs_assemble(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, const tvlm_view* normals,
-                 const tvlm_view* verts_wing, const tvlm_view* verts_wake, int32_t iteration, T sigma, void* stream,
-                 bool seeded = false) {
+int lhs_sections(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, const tvlm_view* normals,
+                 const tvlm_view* verts_wing, const tvlm_view* verts_wake, int32_t iteration,
+                 std::vector<LhsSection<T>>& out, int64_t& tiles) {
     TVLM_REQUIRE(lhs && colloc && normals && verts_wing && verts_wake, "null argument");
     TVLM_REQUIRE(n_surf >= 1 && n_surf <= TVLM_MAX_SURFACES, "n_surf %d out of range", n_surf);
     TVLM_REQUIRE(iteration >= 0, "negative iteration");
-    TVLM_REQUIRE(lhs->stride[0] == 1, "lhs must be column-major with unit stride[0]");
+    TVLM_REQUIRE(lhs->ptr != nullptr && lhs->stride[0] == 1, "lhs must be column-major with unit stride[0]");
     int64_t total = 0;
     for (int m = 0; m < n_surf; m++) {
         const int64_t ns = colloc[m].shape[0], nc = colloc[m].shape[1];
@@ -53,7 +52,7 @@ int lhs_assemble(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, 
     TVLM_REQUIRE(lhs->shape[0] >= total && lhs->shape[1] >= total, "lhs smaller than the total panel count %lld",
                  (long long)total);
     const int64_t ld = lhs->stride[1];
-    hipStream_t st = as_stream(stream);
+    constexpr int TC = sizeof(T) == 8 ? 32 : 64;
     int64_t offset_j = 0;
     for (int mj = 0; mj < n_surf; mj++) {
         RingSource<T> src;
@@ -68,35 +67,63 @@ int lhs_assemble(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, 
         const int n_rings = src.ns * src.nc;
         int64_t offset_i = 0;
         for (int mi = 0; mi < n_surf; mi++) {
-            Receivers<T> rc;
-            rc.colloc = static_cast<const T*>(colloc[mi].ptr);
-            rc.cs2 = colloc[mi].stride[2];
-            rc.normals = static_cast<const T*>(normals[mi].ptr);
-            rc.ns2 = normals[mi].stride[2];
-            rc.m = (int32_t)(colloc[mi].shape[0] * colloc[mi].shape[1]);
-            T* section = static_cast<T*>(lhs->ptr) + offset_i + offset_j * ld;
-            constexpr int TC = sizeof(T) == 8 ? 32 : 64;
-            dim3 grid((rc.m + kRecv - 1) / kRecv, (n_rings + TC - 1) / TC);
-            if constexpr (sizeof(T) == 4) {
-                if (sigma != T(0)) {
-                    // the fast form has no core model; use the exact float form for sigma != 0
-                    hipLaunchKernelGGL((k_lhs_block<T, TC, 1>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
-                } else {
-                    hipLaunchKernelGGL((k_lhs_block<T, TC, 2>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
-                }
-            } else {
-                if (sigma != T(0)) {
-                    hipLaunchKernelGGL((k_lhs_block<T, TC, 1>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
-                } else if (seeded) {
-                    hipLaunchKernelGGL((k_lhs_block<T, TC, 3>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
-                } else {
-                    hipLaunchKernelGGL((k_lhs_block<T, TC, 0>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
-                }
-            }
-            TVLM_LAUNCHED();
-            offset_i += rc.m;
+            LhsSection<T> sec;
+            sec.rc.colloc = static_cast<const T*>(colloc[mi].ptr);
+            sec.rc.cs2 = colloc[mi].stride[2];
+            sec.rc.normals = static_cast<const T*>(normals[mi].ptr);
+            sec.rc.ns2 = normals[mi].stride[2];
+            sec.rc.m = (int32_t)(colloc[mi].shape[0] * colloc[mi].shape[1]);
+            sec.src = src;
+            sec.lhs = static_cast<T*>(lhs->ptr) + offset_i + offset_j * ld;
+            sec.ld = ld;
+            sec.tiles_x = (sec.rc.m + kRecv - 1) / kRecv;
+            const int64_t tiles_y = (n_rings + TC - 1) / TC;
+            TVLM_REQUIRE(tiles + (int64_t)sec.tiles_x * tiles_y < (int64_t)1 << 31, "more than 2^31 tiles");
+            sec.tile0 = (int32_t)tiles;
+            tiles += (int64_t)sec.tiles_x * tiles_y;
+            out.push_back(sec);
+            offset_i += sec.rc.m;
         }
         offset_j += n_rings;
+    }
+    return TVLM_OK;
+}
+
+// seeded = true (FP64 only): the FP64 seeded form (hardware 1/sqrt and 1/x estimates + one Newton
+// step) instead of the exact form; used by the served steady solve, which refines against the exact
+// FP64 residual anyway.
+template <typename T>
+int lhs_assemble(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, const tvlm_view* normals,
+                 const tvlm_view* verts_wing, const tvlm_view* verts_wake, int32_t iteration, T sigma, void* stream,
+                 bool seeded = false) {
+    std::vector<LhsSection<T>> secs;
+    int64_t tiles = 0;
+    TVLM_TRY(lhs_sections<T>(lhs, n_surf, colloc, normals, verts_wing, verts_wake, iteration, secs, tiles));
+    hipStream_t st = as_stream(stream);
+    constexpr int TC = sizeof(T) == 8 ? 32 : 64;
+    for (const LhsSection<T>& sc : secs) {
+        T* section = sc.lhs;
+        const int64_t ld = sc.ld;
+        const Receivers<T>& rc = sc.rc;
+        const RingSource<T>& src = sc.src;
+        dim3 grid(sc.tiles_x, (src.ns * src.nc + TC - 1) / TC);
+        if constexpr (sizeof(T) == 4) {
+            if (sigma != T(0)) {
+                // the fast form has no core model; use the exact float form for sigma != 0
+                hipLaunchKernelGGL((k_lhs_block<T, TC, 1>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
+            } else {
+                hipLaunchKernelGGL((k_lhs_block<T, TC, 2>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
+            }
+        } else {
+            if (sigma != T(0)) {
+                hipLaunchKernelGGL((k_lhs_block<T, TC, 1>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
+            } else if (seeded) {
+                hipLaunchKernelGGL((k_lhs_block<T, TC, 3>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
+            } else {
+                hipLaunchKernelGGL((k_lhs_block<T, TC, 0>), grid, dim3(kThreads), 0, st, section, ld, rc, src, sigma);
+            }
+        }
+        TVLM_LAUNCHED();
     }
     return TVLM_OK;
 }
@@ -215,6 +242,8 @@ int scratch_address(void** out) {
     return TVLM_OK;
 }
 
+constexpr int kGroupedGroups = 2048;  // work-groups of a grouped launch: 8 per CU of 256
+
 }  // namespace
 
 int noalloc_scratch_setup() {
@@ -229,7 +258,78 @@ int lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_vie
 
 }  // namespace tvlm
 
+// Sections of several FP64 influence matrices, item by item, on the device for grouped launches.
+struct tvlm_lhs_group {
+    tvlm::LhsSection<double>* secs = nullptr;  // device
+    int32_t n_items = 0;
+    std::vector<int32_t> item_secs;   // sections of items [0, k): item_secs[k]
+    std::vector<int64_t> item_tiles;  // tiles of items [0, k): item_tiles[k]
+};
+
 extern "C" {
+
+int tvlm_lhs_group_create(tvlm_lhs_group** out, int32_t n_items, const tvlm_lhs_item* items, void* stream) {
+    TVLM_REQUIRE(out && items, "null argument");
+    TVLM_REQUIRE(n_items >= 1, "n_items %d < 1", n_items);
+    std::vector<tvlm::LhsSection<double>> secs;
+    std::vector<int32_t> isec(1, 0);
+    std::vector<int64_t> itile(1, 0);
+    int64_t tiles = 0;
+    for (int32_t k = 0; k < n_items; k++) {
+        const tvlm_lhs_item& it = items[k];
+        TVLM_TRY(tvlm::lhs_sections<double>(&it.lhs, it.n_surf, it.colloc, it.normals, it.verts_wing, it.verts_wake,
+                                            it.iteration, secs, tiles));
+        isec.push_back((int32_t)secs.size());
+        itile.push_back(tiles);
+    }
+    auto* G = new tvlm_lhs_group();
+    const size_t bytes = sizeof(tvlm::LhsSection<double>) * secs.size();
+    // uploaded on the caller's stream (not the null stream, which would wait for every other stream)
+    hipStream_t st = tvlm::as_stream(stream);
+    if (hipMalloc((void**)&G->secs, bytes) != hipSuccess ||
+        hipMemcpyAsync(G->secs, secs.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        if (G->secs) (void)hipFree(G->secs);
+        delete G;
+        tvlm::set_error("group table of %zu sections: device allocation or upload failed", secs.size());
+        return TVLM_ERR_HIP;
+    }
+    G->n_items = n_items;
+    G->item_secs = std::move(isec);
+    G->item_tiles = std::move(itile);
+    *out = G;
+    return TVLM_OK;
+}
+
+int tvlm_lhs_group_assemble_f64(tvlm_lhs_group* G, int32_t n_items, int32_t seeded, double sigma, void* stream) {
+    TVLM_REQUIRE(G, "null group");
+    TVLM_REQUIRE(n_items >= 0 && n_items <= G->n_items, "n_items %d of %d", n_items, G->n_items);
+    TVLM_REQUIRE(!(seeded && sigma != 0.0), "the seeded form has no core model (sigma must be 0)");
+    const int64_t tiles = G->item_tiles[n_items];
+    if (tiles == 0) return TVLM_OK;
+    const int nsec = G->item_secs[n_items];
+    const unsigned grid = (unsigned)(tiles < tvlm::kGroupedGroups ? tiles : tvlm::kGroupedGroups);
+    hipStream_t st = tvlm::as_stream(stream);
+    using tvlm::k_lhs_grouped;
+    if (sigma != 0.0)
+        hipLaunchKernelGGL((k_lhs_grouped<double, 32, 1>), dim3(grid), dim3(tvlm::kThreads), 0, st, G->secs, nsec,
+                           (int)tiles, sigma);
+    else if (seeded)
+        hipLaunchKernelGGL((k_lhs_grouped<double, 32, 3>), dim3(grid), dim3(tvlm::kThreads), 0, st, G->secs, nsec,
+                           (int)tiles, sigma);
+    else
+        hipLaunchKernelGGL((k_lhs_grouped<double, 32, 0>), dim3(grid), dim3(tvlm::kThreads), 0, st, G->secs, nsec,
+                           (int)tiles, sigma);
+    TVLM_LAUNCHED();
+    return TVLM_OK;
+}
+
+int tvlm_lhs_group_destroy(tvlm_lhs_group* G) {
+    if (!G) return TVLM_OK;
+    if (G->secs) (void)hipFree(G->secs);
+    delete G;
+    return TVLM_OK;
+}
 
 int tvlm_lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc,
                                  const tvlm_view* normals, const tvlm_view* verts_wing, const tvlm_view* verts_wake,

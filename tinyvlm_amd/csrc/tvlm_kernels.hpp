@@ -72,13 +72,14 @@ __device__ __forceinline__ void load_ring(T* dst, const T* grid, int64_t s2, int
     }
 }
 
-// lhs block of one (receiver surface, ring surface) pair.  lhs points at (offset_i, offset_j).
+// One tile of the lhs block of a (receiver surface, ring surface) pair: receivers [bx kRecv, +kRecv)
+// x rings [by TC, +TC).  lhs points at (offset_i, offset_j).  Every thread of the work-group calls it
+// (it holds a barrier); sv is the caller's LDS staging of TC rings.
 template <typename T, int TC, int MODE>  // MODE as ring_vel
-__global__ __launch_bounds__(kThreads) void k_lhs_block(T* __restrict__ lhs, int64_t ld, Receivers<T> rc,
-                                                        RingSource<T> src, T sigma) {
-    __shared__ T sv[TC][12];
+__device__ __forceinline__ void lhs_tile(T (*sv)[12], T* __restrict__ lhs, int64_t ld, const Receivers<T>& rc,
+                                         const RingSource<T>& src, T sigma, int bx, int by) {
     const int n_rings = src.ns * src.nc;
-    const int col0 = blockIdx.y * TC;
+    const int col0 = by * TC;
     for (int e = threadIdx.x; e < TC * 4; e += kThreads) {
         const int r = e >> 2, q = e & 3;
         const int lidx = col0 + r;
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(kThreads) void k_lhs_block(T* __restrict__ lhs, int
     __syncthreads();
     const int lane = threadIdx.x & (kRecv - 1);
     const int wave = threadIdx.x / kRecv;
-    const int i = blockIdx.x * kRecv + lane;
+    const int i = bx * kRecv + lane;
     if (i >= rc.m) return;
     const v3<T> c = {rc.colloc[i], rc.colloc[i + rc.cs2], rc.colloc[i + 2 * rc.cs2]};
     const v3<T> n = {rc.normals[i], rc.normals[i + rc.ns2], rc.normals[i + 2 * rc.ns2]};
@@ -114,6 +115,55 @@ __global__ __launch_bounds__(kThreads) void k_lhs_block(T* __restrict__ lhs, int
             }
         }
         *dst = acc;
+    }
+}
+
+// lhs block of one (receiver surface, ring surface) pair, one tile per work-group.
+template <typename T, int TC, int MODE>  // MODE as ring_vel
+__global__ __launch_bounds__(kThreads) void k_lhs_block(T* __restrict__ lhs, int64_t ld, Receivers<T> rc,
+                                                        RingSource<T> src, T sigma) {
+    __shared__ T sv[TC][12];
+    lhs_tile<T, TC, MODE>(sv, lhs, ld, rc, src, sigma, blockIdx.x, blockIdx.y);
+}
+
+// One (receiver surface, ring surface) block of one influence matrix in a grouped launch: the tiles
+// of all sections of all matrices are numbered in section order, section s owning tiles
+// [tile0, tile0 + tiles_x tiles_y); tile t of it is receiver block t % tiles_x, ring block t / tiles_x.
+template <typename T>
+struct LhsSection {
+    T* lhs;  // the section's origin in its matrix
+    int64_t ld;
+    Receivers<T> rc;
+    RingSource<T> src;
+    int32_t tiles_x, tile0;
+};
+
+// Grouped assembly: many influence matrices of any sizes (different meshes, requests) in one
+// launch.  Persistent: work-group g takes tiles g, g + gridDim.x, ... of the first `tiles` tiles
+// (a prefix of whole matrices), finding each tile's section by binary search over tile0.  Every
+// tile runs lhs_tile, so each matrix is bitwise the per-section launches' result.
+template <typename T, int TC, int MODE>
+__global__ __launch_bounds__(kThreads) void k_lhs_grouped(const LhsSection<T>* __restrict__ secs, int nsec, int tiles,
+                                                          T sigma) {
+    __shared__ T sv[TC][12];
+    __shared__ LhsSection<T> s_sec;
+    for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+        if (threadIdx.x == 0) {
+            int lo = 0, hi = nsec - 1;  // the last section with tile0 <= t
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (secs[mid].tile0 <= t)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            s_sec = secs[lo];
+        }
+        __syncthreads();
+        const LhsSection<T> S = s_sec;
+        const int local = t - S.tile0;
+        lhs_tile<T, TC, MODE>(sv, S.lhs, S.ld, S.rc, S.src, sigma, local % S.tiles_x, local / S.tiles_x);
+        __syncthreads();  // sv and s_sec are rewritten for the next tile
     }
 }
 

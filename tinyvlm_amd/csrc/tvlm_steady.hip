@@ -61,6 +61,7 @@ struct tvlm_vlm_plan {
     std::vector<double> alpha, beta, ref;  // per request; ref: CM reference point in the moved frame
     double u_inf = 1.0, rho = 1.0;
     bool solved = false;
+    tvlm_lhs_group* group = nullptr;  // served mode: every request's influence matrix, one grouped launch
 
     int alloc(void** p, size_t bytes) {
         TVLM_HIP(hipMalloc(p, bytes));
@@ -198,8 +199,9 @@ void views(tvlm_vlm_plan* P, int b, int m, tvlm_view& v, tvlm_view& w, tvlm_view
     gd = {P->gdelta[m] + b * P->psz[m], {ns, nc, 1}, {1, ns, ns * nc}};
 }
 
-// geometry, influence matrix and rhs of request b (vlm.cpp:97-124)
-int assemble(tvlm_vlm_plan* P, int b, bool seeded) {
+// geometry, influence matrix (matrix = false: zeroed only; the caller assembles it) and rhs of
+// request b (vlm.cpp:97-124)
+int assemble(tvlm_vlm_plan* P, int b, bool seeded, bool matrix = true) {
     hipStream_t st = P->stream;
     const int n = P->n_surf;
     std::vector<tvlm_view> V(n), W(n), C(n), Nn(n), A(n), VEL(n), F(n), GD(n), TR(n), INIT(n);
@@ -220,9 +222,9 @@ int assemble(tvlm_vlm_plan* P, int b, bool seeded) {
     TVLM_TRY(tvlm_fill_f64(0, lhs, 1, 0.0, NN, st));
     TVLM_TRY(tvlm_fill_f64(0, rhs, 1, 0.0, P->N, st));
     tvlm_view L = {lhs, {P->N, P->N, 1}, {1, P->N, NN}};
-    if (seeded)
+    if (matrix && seeded)
         TVLM_TRY(lhs_assemble_seeded_f64(&L, n, C.data(), Nn.data(), V.data(), W.data(), 1, st));
-    else
+    else if (matrix)
         TVLM_TRY(tvlm_lhs_assemble_f64(&L, n, C.data(), Nn.data(), V.data(), W.data(), 1, 0.0, st));
     const double a = P->alpha[b], be = P->beta[b], u = P->u_inf;
     const double fs[3] = {u * std::cos(a) * std::cos(be), -u * std::cos(a) * std::sin(be), u * std::sin(a)};
@@ -394,6 +396,7 @@ int tvlm_vlm_create(tvlm_vlm_plan** plan, int32_t n_surf, const int64_t* ns, con
     const int64_t NN = N * N;
     const int64_t nb = mode == 1 ? batch : 1;  // plain mode reuses one matrix for every request
     auto fail = [&](int rc) {
+        (void)tvlm_lhs_group_destroy(P->group);
         for (void* p : P->allocs) (void)hipFree(p);
         if (P->tr_host) (void)hipHostFree(P->tr_host);
         if (P->done) (void)hipEventDestroy(P->done);
@@ -449,6 +452,21 @@ int tvlm_vlm_create(tvlm_vlm_plan** plan, int32_t n_surf, const int64_t* ns, con
         set_error("hipEventRecord failed");
         rc = TVLM_ERR_HIP;
     }
+    if (rc == TVLM_OK && mode == 1) {
+        // the grouped assembly's work list: request b's matrix and surfaces (iteration 1: one wake row)
+        std::vector<tvlm_lhs_item> items(batch);
+        std::vector<tvlm_view> vv((size_t)batch * n_surf * 8);
+        for (int b = 0; b < batch; b++) {
+            tvlm_view *V = &vv[((size_t)b * 8 + 0) * n_surf], *W = &vv[((size_t)b * 8 + 1) * n_surf],
+                      *C = &vv[((size_t)b * 8 + 2) * n_surf], *Nn = &vv[((size_t)b * 8 + 3) * n_surf];
+            for (int m = 0; m < n_surf; m++) {
+                tvlm_view A, VEL, F, GD;
+                views(P, b, m, V[m], W[m], C[m], Nn[m], A, VEL, F, GD);
+            }
+            items[b] = {{P->lhs + b * NN, {N, N, 1}, {1, N, NN}}, n_surf, C, Nn, V, W, 1};
+        }
+        rc = tvlm_lhs_group_create(&P->group, batch, items.data(), stream);
+    }
     if (rc != TVLM_OK) return fail(rc);
     P->alpha.assign(batch, 0.0);
     P->beta.assign(batch, 0.0);
@@ -502,7 +520,9 @@ int tvlm_vlm_solve_n(tvlm_vlm_plan* P, int32_t n_active, const double* alpha, co
             TVLM_TRY(post(P, b));
         }
     } else {
-        for (int b = 0; b < B; b++) TVLM_TRY(assemble(P, b, true));
+        // geometry and rhs request by request, then every request's matrix in one grouped launch
+        for (int b = 0; b < B; b++) TVLM_TRY(assemble(P, b, true, false));
+        TVLM_TRY(tvlm_lhs_group_assemble_f64(P->group, B, 1, 0.0, st));
         hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, st, P->lhs, P->lhs32, NN * B);
         TVLM_LAUNCHED();
         tvlm_view L32 = {P->lhs32, {N, N, B}, {1, N, NN}};
@@ -628,6 +648,7 @@ int tvlm_vlm_footprint(const tvlm_vlm_plan* P, int64_t* device_bytes) {
 int tvlm_vlm_destroy(tvlm_vlm_plan* P) {
     if (!P) return TVLM_OK;
     (void)hipEventSynchronize(P->done);
+    (void)tvlm_lhs_group_destroy(P->group);
     for (void* p : P->allocs) (void)hipFree(p);
     (void)hipEventDestroy(P->done);
     if (P->tr_host) (void)hipHostFree(P->tr_host);
