@@ -508,6 +508,65 @@ def assembly_bench(reps=5):
     out["note"] = ("FP32 rsqrt form against the non-packed FP32 VALU rate (78.6 TFLOP/s: packed FP32 "
                    "instructions are switched off in every kernel); FP64 exact form and the FP64 seeded form (the "
                    "served steady path's) against the FP64 VALU peak")
+    out["grouped"] = grouped_assembly_bench(reps)
+    return out
+
+
+def grouped_assembly_bench(reps=5):
+    """Grouped seeded-FP64 assembly (tvlm_lhs_group_*): one influence matrix of every cell of the served
+    mix (3 families x n in {16, 32, 45, 64, 90, 128}, N = 256 ... 16384, one wake row as the served
+    path assembles them) in ONE launch, against the same 18 matrices assembled one call after another;
+    and the same for the 9 small cells (n <= 45), where one call underfills the device."""
+    import torch
+
+    from tinyvlm_amd import capi as H
+    from tinyvlm_amd import serve as S
+
+    def setup(cells):
+        items, pairs = [], 0
+        for fam, n in cells:
+            v = S.mix_mesh(fam, n)
+            ns, nc = v.shape[0] - 1, v.shape[1] - 1
+            w = np.zeros((ns + 1, 2, 4))
+            w[:, 0, :] = v[:, -1, :]
+            w[:, 1, :] = v[:, -1, :]
+            w[:, 1, 0] += 100.0
+            verts, wake = [H.to_device(v, torch.float64)], [H.to_device(w, torch.float64)]
+            colloc = [H.colmajor((ns, nc, 3), torch.float64)]
+            normals = [H.colmajor((ns, nc, 3), torch.float64)]
+            areas = [H.colmajor((ns, nc), torch.float64)]
+            H.mesh_metrics(0.05, verts, colloc, normals, areas)
+            N = ns * nc
+            items.append((H.colmajor((N, N), torch.float64), colloc, normals, verts, wake, 1))
+            pairs += N * N
+        return items, pairs
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {}
+    for tag, sizes in (("mix_18_cells", S.MIX_SIZES), ("small_9_cells", (16, 32, 45))):
+        items, pairs = setup([(f, n) for f in S.MIX_FAMILIES for n in sizes])
+        g = H.LhsGroup(items)
+        ms_g = timed(lambda: g.assemble(seeded=True))
+        ms_1 = timed(lambda: [H.lhs_assemble_seeded(*it) for it in items])
+        tf = pairs * FLOPS_PER_PAIR / (ms_g * 1e-3) / 1e12
+        out[tag] = {"matrices": len(items), "N": sorted({int(it[0].shape[0]) for it in items}), "pairs": pairs,
+                    "grouped_ms": round(ms_g, 4), "per_matrix_calls_ms": round(ms_1, 4),
+                    "grouped_tflops": round(tf, 2), "frac": round(tf / PEAK_FP64_VALU_TFLOPS, 4),
+                    "speedup_of_grouping": round(ms_1 / ms_g, 3)}
+        g.close()
+        del items
+    out["note"] = ("accumulating launches timed back to back (the values grow; the timing does not depend on "
+                   "them); per_matrix_calls = tvlm_lhs_assemble_seeded_f64 per matrix, on the same stream")
     return out
 
 

@@ -198,7 +198,7 @@ int tvlm_set_tile_targets(int32_t wake_groups, int32_t lattice_groups);
 int tvlm_get_tile_targets(int32_t* wake_groups, int32_t* lattice_groups);
 
 /* Grouped assembly: many FP64 influence matrices of any sizes -- different meshes, different
- * requests -- in ONE persistent launch over a work list of (matrix, section, tile).  Each item is one
+ * requests -- in ONE launch over a work list of (matrix, section, tile).  Each item is one
  * Backend::lhs_assemble call (vlm_backend.hpp:44) with its own arguments; the result of every matrix
  * is bitwise that of tvlm_lhs_assemble_f64 (seeded = 0) or tvlm_lhs_assemble_seeded_f64 (seeded = 1)
  * on the same item.  Like those, it accumulates into lhs (zero it first).  No counterpart in the

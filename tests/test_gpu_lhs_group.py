@@ -1,10 +1,10 @@
 """GPU checks of the grouped influence-matrix assembly (tvlm_lhs_group_*): many matrices of uneven
-size -- different meshes, one and two surfaces, different wake iterations -- in one persistent launch.
+size -- different meshes, one and two surfaces, different wake iterations -- in one launch.
 
 Every matrix of a group must be bitwise the per-section launches' result (tvlm_lhs_assemble_f64 for
 the exact form, tvlm_lhs_assemble_seeded_f64 for the seeded form): the grouped kernel runs the same
-tile code on the same tiles.  Also: a prefix of the items, accumulation, a group whose tiles outnumber
-its work-groups (the persistent loop), and the argument checks.
+tile code on the same tiles.  Also: a prefix of the items, accumulation, large and small matrices in
+one work list, and the argument checks.
 """
 import numpy as np
 import pytest
@@ -117,12 +117,12 @@ def test_group_of_uneven_meshes_is_bitwise_the_per_matrix_assembly(H, golden, se
     g.close()
 
 
-def test_group_with_more_tiles_than_work_groups(H, golden, oracle):
-    """Two wings of 1600 and 2400 panels and the golden meshes: 25 x 50 + 38 x 75 + ... tiles,
-    more than the 2048 work-groups of the launch, so work-groups take several tiles each."""
+def test_group_of_large_and_small_matrices(H, golden, oracle):
+    """Two wings of 1600 and 2400 panels and the golden meshes: 25 x 50 + 38 x 75 + ... tiles, so the
+    work list's section lookup crosses many 64-tile chunks and section boundaries inside chunks."""
     import torch
 
-    print("[case] lhs group, persistent loop (tiles > work-groups)", flush=True)
+    print("[case] lhs group of large and small matrices", flush=True)
     items = [_item(H, _wing(H, oracle, 40, 40), 1), _item(H, _wing(H, oracle, 60, 40), 1),
              _item(H, _surfaces(H, golden, "split2", 2), 1)]
     g = H.LhsGroup(items)

@@ -242,8 +242,6 @@ int scratch_address(void** out) {
     return TVLM_OK;
 }
 
-constexpr int kGroupedGroups = 2048;  // work-groups of a grouped launch: 8 per CU of 256
-
 }  // namespace
 
 int noalloc_scratch_setup() {
@@ -261,6 +259,8 @@ int lhs_assemble_seeded_f64(const tvlm_view* lhs, int32_t n_surf, const tvlm_vie
 // Sections of several FP64 influence matrices, item by item, on the device for grouped launches.
 struct tvlm_lhs_group {
     tvlm::LhsSection<double>* secs = nullptr;  // device
+    int32_t* tile0 = nullptr;                  // device: first tile of every section, then the total
+    int32_t* chunk = nullptr;                  // device: section of tile c * kGroupChunk
     int32_t n_items = 0;
     std::vector<int32_t> item_secs;   // sections of items [0, k): item_secs[k]
     std::vector<int64_t> item_tiles;  // tiles of items [0, k): item_tiles[k]
@@ -284,12 +284,27 @@ int tvlm_lhs_group_create(tvlm_lhs_group** out, int32_t n_items, const tvlm_lhs_
     }
     auto* G = new tvlm_lhs_group();
     const size_t bytes = sizeof(tvlm::LhsSection<double>) * secs.size();
+    std::vector<int32_t> t0(secs.size() + 1);
+    for (size_t k = 0; k < secs.size(); k++) t0[k] = secs[k].tile0;
+    t0[secs.size()] = (int32_t)tiles;
+    std::vector<int32_t> ch((size_t)((tiles + tvlm::kGroupChunk - 1) / tvlm::kGroupChunk));
+    for (size_t c = 0, k = 0; c < ch.size(); c++) {
+        const int64_t t = (int64_t)c * tvlm::kGroupChunk;
+        while (t0[k + 1] <= t) k++;
+        ch[c] = (int32_t)k;
+    }
     // uploaded on the caller's stream (not the null stream, which would wait for every other stream)
     hipStream_t st = tvlm::as_stream(stream);
     if (hipMalloc((void**)&G->secs, bytes) != hipSuccess ||
+        hipMalloc((void**)&G->tile0, sizeof(int32_t) * t0.size()) != hipSuccess ||
+        hipMalloc((void**)&G->chunk, sizeof(int32_t) * (ch.empty() ? 1 : ch.size())) != hipSuccess ||
+        hipMemcpyAsync(G->chunk, ch.data(), sizeof(int32_t) * ch.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(G->secs, secs.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(G->tile0, t0.data(), sizeof(int32_t) * t0.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
         if (G->secs) (void)hipFree(G->secs);
+        if (G->tile0) (void)hipFree(G->tile0);
+        if (G->chunk) (void)hipFree(G->chunk);
         delete G;
         tvlm::set_error("group table of %zu sections: device allocation or upload failed", secs.size());
         return TVLM_ERR_HIP;
@@ -307,19 +322,18 @@ int tvlm_lhs_group_assemble_f64(tvlm_lhs_group* G, int32_t n_items, int32_t seed
     TVLM_REQUIRE(!(seeded && sigma != 0.0), "the seeded form has no core model (sigma must be 0)");
     const int64_t tiles = G->item_tiles[n_items];
     if (tiles == 0) return TVLM_OK;
-    const int nsec = G->item_secs[n_items];
-    const unsigned grid = (unsigned)(tiles < tvlm::kGroupedGroups ? tiles : tvlm::kGroupedGroups);
     hipStream_t st = tvlm::as_stream(stream);
     using tvlm::k_lhs_grouped;
+    const dim3 grid((unsigned)tiles);
     if (sigma != 0.0)
-        hipLaunchKernelGGL((k_lhs_grouped<double, 32, 1>), dim3(grid), dim3(tvlm::kThreads), 0, st, G->secs, nsec,
-                           (int)tiles, sigma);
+        hipLaunchKernelGGL((k_lhs_grouped<double, 32, 1>), grid, dim3(tvlm::kThreads), 0, st, G->secs, G->tile0,
+                           G->chunk, (int)tiles, sigma);
     else if (seeded)
-        hipLaunchKernelGGL((k_lhs_grouped<double, 32, 3>), dim3(grid), dim3(tvlm::kThreads), 0, st, G->secs, nsec,
-                           (int)tiles, sigma);
+        hipLaunchKernelGGL((k_lhs_grouped<double, 32, 3>), grid, dim3(tvlm::kThreads), 0, st, G->secs, G->tile0,
+                           G->chunk, (int)tiles, sigma);
     else
-        hipLaunchKernelGGL((k_lhs_grouped<double, 32, 0>), dim3(grid), dim3(tvlm::kThreads), 0, st, G->secs, nsec,
-                           (int)tiles, sigma);
+        hipLaunchKernelGGL((k_lhs_grouped<double, 32, 0>), grid, dim3(tvlm::kThreads), 0, st, G->secs, G->tile0,
+                           G->chunk, (int)tiles, sigma);
     TVLM_LAUNCHED();
     return TVLM_OK;
 }
@@ -327,6 +341,8 @@ int tvlm_lhs_group_assemble_f64(tvlm_lhs_group* G, int32_t n_items, int32_t seed
 int tvlm_lhs_group_destroy(tvlm_lhs_group* G) {
     if (!G) return TVLM_OK;
     if (G->secs) (void)hipFree(G->secs);
+    if (G->tile0) (void)hipFree(G->tile0);
+    if (G->chunk) (void)hipFree(G->chunk);
     delete G;
     return TVLM_OK;
 }

@@ -138,33 +138,67 @@ struct LhsSection {
     int32_t tiles_x, tile0;
 };
 
+// A value every lane of the wave holds alike, moved to a scalar register (the section fields the
+// grouped kernel reads from LDS would otherwise sit in vector registers and cost it occupancy).
+__device__ __forceinline__ int32_t wave_uniform(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t wave_uniform(int64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <typename P>
+__device__ __forceinline__ P* wave_uniform(P* p) {
+    return reinterpret_cast<P*>(wave_uniform((int64_t)reinterpret_cast<uintptr_t>(p)));
+}
+template <typename T>
+__device__ __forceinline__ LhsSection<T> wave_uniform(const LhsSection<T>& a) {
+    LhsSection<T> b;
+    b.lhs = wave_uniform(a.lhs);
+    b.ld = wave_uniform(a.ld);
+    b.rc.colloc = wave_uniform(a.rc.colloc);
+    b.rc.cs2 = wave_uniform(a.rc.cs2);
+    b.rc.normals = wave_uniform(a.rc.normals);
+    b.rc.ns2 = wave_uniform(a.rc.ns2);
+    b.rc.m = wave_uniform(a.rc.m);
+    b.src.verts = wave_uniform(a.src.verts);
+    b.src.vs2 = wave_uniform(a.src.vs2);
+    b.src.ns = wave_uniform(a.src.ns);
+    b.src.nc = wave_uniform(a.src.nc);
+    b.src.wake = wave_uniform(a.src.wake);
+    b.src.ws2 = wave_uniform(a.src.ws2);
+    b.src.nwv = wave_uniform(a.src.nwv);
+    b.src.iteration = wave_uniform(a.src.iteration);
+    b.tiles_x = wave_uniform(a.tiles_x);
+    b.tile0 = wave_uniform(a.tile0);
+    return b;
+}
+
 // Grouped assembly: many influence matrices of any sizes (different meshes, requests) in one
-// launch.  Persistent: work-group g takes tiles g, g + gridDim.x, ... of the first `tiles` tiles
-// (a prefix of whole matrices), finding each tile's section by binary search over tile0.  Every
+// launch, one tile per work-group over the work list of all their tiles (the first `tiles`: a
+// prefix of whole matrices), numbered section by section; the hardware hands the work-groups out in
+// order, so the ones running together work on neighbouring tiles, as in one per-section launch.
+// tile0[s] is the first tile of section s (tile0[nsec] = tiles); chunk[c] is the section holding tile
+// c * kGroupChunk, so a tile's section is chunk[t / kGroupChunk] or one of the few after it.  Every
 // tile runs lhs_tile, so each matrix is bitwise the per-section launches' result.
+constexpr int kGroupChunk = 64;
+
 template <typename T, int TC, int MODE>
-__global__ __launch_bounds__(kThreads) void k_lhs_grouped(const LhsSection<T>* __restrict__ secs, int nsec, int tiles,
-                                                          T sigma) {
+__global__ __launch_bounds__(kThreads) void k_lhs_grouped(const LhsSection<T>* __restrict__ secs,
+                                                          const int32_t* __restrict__ tile0,
+                                                          const int32_t* __restrict__ chunk, int tiles, T sigma) {
     __shared__ T sv[TC][12];
     __shared__ LhsSection<T> s_sec;
-    for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
-        if (threadIdx.x == 0) {
-            int lo = 0, hi = nsec - 1;  // the last section with tile0 <= t
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (secs[mid].tile0 <= t)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            s_sec = secs[lo];
-        }
-        __syncthreads();
-        const LhsSection<T> S = s_sec;
-        const int local = t - S.tile0;
-        lhs_tile<T, TC, MODE>(sv, S.lhs, S.ld, S.rc, S.src, sigma, local % S.tiles_x, local / S.tiles_x);
-        __syncthreads();  // sv and s_sec are rewritten for the next tile
+    const int t = blockIdx.x;
+    if (t >= tiles) return;
+    if (threadIdx.x == 0) {
+        int s = chunk[t / kGroupChunk];
+        while (tile0[s + 1] <= t) s++;  // sections are non-empty and tile0 ends with `tiles`
+        s_sec = secs[s];
     }
+    __syncthreads();
+    const LhsSection<T> S = wave_uniform(s_sec);
+    const int local = t - S.tile0;
+    lhs_tile<T, TC, MODE>(sv, S.lhs, S.ld, S.rc, S.src, sigma, local % S.tiles_x, local / S.tiles_x);
 }
 
 // ---- wake influence ---------------------------------------------------------------------------
