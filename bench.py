@@ -585,10 +585,11 @@ HB_OMEGA = 0.8
 HB_U_REF = 6.0
 
 
-def hb_served(wing, flap, X, n_requests, workers):
+def hb_served(wing, flap, X, n_requests, workers, processes=True):
     """The harmonic-balance configuration served: n_requests airspeeds U = linspace(0.5, 1.5) U_ref as
-    concurrent cold-start requests through Server on `workers` HB worker threads (one HBVLM and stream
-    each), against the same cold solves one after another on one solver."""
+    concurrent cold-start requests through Server on `workers` HB worker processes (threads with
+    processes=False; one HBVLM and stream each), against the same cold solves one after another on
+    one solver."""
     from tinyvlm_amd import serve as S
     from tinyvlm_amd.hb import HBVLM
 
@@ -614,14 +615,15 @@ def hb_served(wing, flap, X, n_requests, workers):
         seq_F[float(U)] = F
     seq_rate = len(sample) / (time.perf_counter() - t0)
     reqs = [req(i, U) for i, U in enumerate(speeds)]
-    with S.Server({}, lanes=1, hb=cfg, hb_workers=workers) as srv:
+    with S.Server({}, lanes=1, hb=cfg, hb_workers=workers, hb_processes=processes) as srv:
         srv.run([req(-1 - k, speeds[0]) for k in range(2 * workers)], realtime=False)  # solvers, scratch
         t0 = time.perf_counter()
         srv.run(reqs, realtime=False)
         wall = time.perf_counter() - t0
     lat = [r.latency * 1e3 for r in reqs]
     same = all(np.array_equal(r.forces, seq_F[r.hb["U"]]) for r in reqs if r.hb["U"] in seq_F)
-    return {"requests": n_requests, "workers": workers, "value": round(n_requests / wall, 3), "unit": "HB solves/s",
+    return {"requests": n_requests, "workers": workers, "worker_kind": "processes" if processes else "threads",
+            "value": round(n_requests / wall, 3), "unit": "HB solves/s",
             "sequential_cold": {"value": round(seq_rate, 3), "unit": "HB solves/s", "solves": len(sample)},
             "speedup_vs_sequential_cold": round(n_requests / wall / seq_rate, 3),
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1), "p99": round(float(np.percentile(lat, 99)), 1)},
@@ -629,7 +631,7 @@ def hb_served(wing, flap, X, n_requests, workers):
             "served_equals_alone_bitwise": bool(same)}
 
 
-def hb_bench(n_speeds, with_cpu, n_served=64, workers=1):
+def hb_bench(n_speeds, with_cpu, n_served=64, workers=4, processes=True):
     import tempfile
 
     import torch
@@ -671,7 +673,7 @@ def hb_bench(n_speeds, with_cpu, n_served=64, workers=1):
         "note": "continuation: each airspeed warm-started from the previous solution; the Anderson map (replayed from a hipGraph) is bound by its 11 exact-form FP64 wake inductions",
     }
     if n_served:
-        out["served"] = hb_served(wing, flap, X, n_served, workers)
+        out["served"] = hb_served(wing, flap, X, n_served, workers, processes)
     if with_cpu:
         from oracle import hb as ohb
         from oracle import cpuref as C
@@ -887,7 +889,8 @@ def main():
     ap.add_argument("--hb-speeds", type=int, default=8, help="airspeeds of the timed HB continuation")
     ap.add_argument("--no-hb", action="store_true")
     ap.add_argument("--hb-served", type=int, default=64, help="airspeeds served as concurrent HB requests (0: none)")
-    ap.add_argument("--hb-workers", type=int, default=1, help="HB worker threads of the served run")
+    ap.add_argument("--hb-workers", type=int, default=4, help="HB worker processes of the served run")
+    ap.add_argument("--hb-threads", action="store_true", help="HB workers as threads of this process")
     ap.add_argument("--hb-only", action="store_true", help="only the hb section")
     ap.add_argument("--two-dof-steps", type=int, default=400, help="timed coupled pitch/heave steps")
     ap.add_argument("--two-dof-large-steps", type=int, default=100, help="timed steps of the 600-panel section")
@@ -918,7 +921,8 @@ def main():
         print(json.dumps(serve_mix_bench(args.mix_span_hi, args.mix_span_lo, args.mix_saturation, args.mix_lanes)))
         return
     if args.hb_only:
-        print(json.dumps(hb_bench(args.hb_speeds, not args.no_cpu, args.hb_served, args.hb_workers)))
+        print(json.dumps(hb_bench(args.hb_speeds, not args.no_cpu, args.hb_served, args.hb_workers,
+                                  not args.hb_threads)))
         return
     if args.flutter_only:
         print(json.dumps(flutter_bench(args.flutter_airspeeds, args.flutter_steps, 20, args.flutter_plain_steps,
@@ -1119,7 +1123,7 @@ def main():
         out["lu"] = lu
         out["served_mix"] = serve_mix_bench(args.mix_span_hi, args.mix_span_lo, args.mix_saturation, args.mix_lanes)
     if not args.no_hb:
-        out["hb"] = hb_bench(args.hb_speeds, not args.no_cpu, args.hb_served, args.hb_workers)
+        out["hb"] = hb_bench(args.hb_speeds, not args.no_cpu, args.hb_served, args.hb_workers, not args.hb_threads)
     if not args.no_two_dof:
         out["two_dof"] = two_dof_bench(args.two_dof_steps, not args.no_cpu)
         out["two_dof_600"] = two_dof_bench(args.two_dof_large_steps, not args.no_cpu, large=True)

@@ -244,11 +244,12 @@ def test_large_mix_meshes_through_the_server(tmp_path):
         assert abs(r.cl - plain["cl"][0]) <= TOL, r.rid
 
 
-def test_hb_requests_served_beside_steady(root, oracle):
+@pytest.mark.parametrize("processes", [False, True], ids=["threads", "processes"])
+def test_hb_requests_served_beside_steady(root, oracle, processes):
     """Harmonic-balance requests (the reference script's wing 9x5 + flap 3x5, H = 2, several
-    airspeeds) served on two worker threads beside steady requests: every request starts cold, so its
-    loads equal (bit for bit) the same solve on a fresh HBVLM alone; the steady results match their
-    alone solves as in the other mix tests."""
+    airspeeds) served on worker threads, or on two worker processes, beside steady requests: every
+    request starts cold, so its loads equal (bit for bit) the same solve on a fresh HBVLM alone; the
+    steady results match their alone solves as in the other mix tests."""
     import os
 
     from tinyvlm_amd import capi
@@ -264,7 +265,7 @@ def test_hb_requests_served_beside_steady(root, oracle):
     X[1, 2] = 0.02
     X[2, 1] = 0.05
     speeds = [5.0, 5.5, 6.0, 6.5, 7.0, 7.5]
-    print("[case] serve: hb requests beside steady", flush=True)
+    print(f"[case] serve: hb requests beside steady, worker processes {processes}", flush=True)
     alone = {}
     for U in speeds:
         h = HBVLM("hip", paths)
@@ -277,7 +278,7 @@ def test_hb_requests_served_beside_steady(root, oracle):
     reqs = [S.Request(i, "r16", float(np.deg2rad(a)), arrival=0.0) for i, a in enumerate((1.0, 2.0, 3.0, 4.0))]
     reqs += [S.Request(100 + k, "wf", kind="hb", hb={"omega": 0.8, "U": U, "dyn": X}) for k, U in enumerate(speeds)]
     with S.Server({"r16": [v]}, lanes=2, hb={"wf": {"meshes": paths, "harmonics": H, "scaling": scale}},
-                  hb_workers=4) as srv:
+                  hb_workers=2 if processes else 4, hb_processes=processes) as srv:
         srv.run(reqs, realtime=False)
     for r in reqs[4:]:
         F, it = alone[r.hb["U"]]

@@ -49,7 +49,7 @@ import tempfile
 import threading
 import time
 from collections import deque
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import ProcessPoolExecutor, ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -131,6 +131,36 @@ def mix_mesh(family: str, n: int, cache_dir: str | None = None) -> np.ndarray:
         x = x + y * math.tan(math.radians(MIX_SWEEP_DEG))
     write_grid_plot3d(path, x, y)
     return read_plot3d(path)
+
+
+# ---- harmonic-balance worker processes ----------------------------------------------------------
+_HB_SOLVERS: dict = {}  # in a worker process: configuration name -> its HBVLM
+
+
+def _hb_proc_init(cfgs: dict) -> None:
+    """Worker-process start: the HIP context and one HBVLM per configuration (graph replay of the map
+    allowed: no other thread of the process captures)."""
+    from . import capi as H
+    from .hb import HBVLM
+
+    H.init(0)
+    for name, cfg in cfgs.items():
+        h = HBVLM("hip", list(cfg["meshes"]))
+        h.init(int(cfg["harmonics"]), float(cfg["scaling"]))
+        _HB_SOLVERS[name] = h
+
+
+def _hb_proc_job(name: str, omega: float, U: float, dyn: np.ndarray):
+    """One cold-start HB solve in a worker process: (forces [3, K], iterations, wall-clock start)."""
+    import torch
+
+    start = time.time()
+    h = _HB_SOLVERS[name]
+    h.reset()
+    forces = np.zeros((3, h.K))
+    with torch.cuda.stream(h.stream):
+        h.run(omega, U, dyn, forces)
+    return forces, int(h.iterations), start
 
 
 # ---- requests and lanes -------------------------------------------------------------------------
@@ -217,7 +247,8 @@ class Server:
     def __init__(self, meshes: dict, lanes: int = 3, max_batch: int = 32, plan_bytes: int = 16 << 30,
                  budget_bytes: int | None = None, plans_per_mesh: int | None = None, u_inf: float = 1.0,
                  rho: float = 1.0, keep_gamma: bool = False, unsteady: dict | None = None,
-                 hb: dict | None = None, hb_workers: int = 1, unsteady_batch: int = 1):
+                 hb: dict | None = None, hb_workers: int = 1, unsteady_batch: int = 1,
+                 hb_processes: bool = False):
         """meshes: name -> list of verts arrays (one per surface, as read_plot3d returns them).
         unsteady: name -> (t_steps, dt) or (t_steps, dt, far_wake) for the meshes that take unsteady
         requests; far_wake = [(age, row_group, span_group), ...] aggregates the wake beyond `age` steps
@@ -227,7 +258,10 @@ class Server:
         configurations that take kind = "hb" requests; they run on hb_workers worker threads, each
         with its own HBVLM (and stream) per configuration, beside the lanes.  An HB solve's Anderson
         loop is driven from Python launch by launch, so more workers in one process contend for the
-        interpreter (measured: 1 worker 8.8, 2 workers 7.6, 4 workers 6.9 solves/s)."""
+        interpreter (measured: 1 worker 8.8, 2 workers 7.6, 4 workers 6.9 solves/s).  hb_processes:
+        the workers are processes instead (spawned, each with its own HIP context, solvers and graph
+        of the HB map; at most 15, one GPU process each beside this one), whose Anderson loops run in
+        parallel; a request's loads are the same as from a thread worker's solver."""
         if lanes < 1 or max_batch < 1 or unsteady_batch < 1:
             raise ValueError("lanes, max_batch and unsteady_batch must be positive")
         self.unsteady_batch = int(unsteady_batch)
@@ -254,10 +288,20 @@ class Server:
         self._hb_pool = None
         self._hb_local = threading.local()
         self._hb_all: list = []  # every worker's HBVLM, kept until close
+        self._hb_procs = bool(hb_processes)
         if self.hb:
-            if hb_workers < 1:
-                raise ValueError("hb_workers must be positive")
-            self._hb_pool = ThreadPoolExecutor(max_workers=hb_workers, thread_name_prefix="tvlm-hb")
+            if hb_workers < 1 or (hb_processes and hb_workers > 15):
+                raise ValueError("hb_workers must be positive (and at most 15 worker processes)")
+            if hb_processes:
+                import multiprocessing
+
+                cfgs = {k: {"meshes": list(c["meshes"]), "harmonics": int(c["harmonics"]),
+                            "scaling": float(c["scaling"])} for k, c in self.hb.items()}
+                self._hb_pool = ProcessPoolExecutor(max_workers=hb_workers,
+                                                    mp_context=multiprocessing.get_context("spawn"),
+                                                    initializer=_hb_proc_init, initargs=(cfgs,))
+            else:
+                self._hb_pool = ThreadPoolExecutor(max_workers=hb_workers, thread_name_prefix="tvlm-hb")
 
     # -- memory ------------------------------------------------------------------------------------
     def _estimate(self, key: str) -> int:
@@ -401,18 +445,27 @@ class Server:
         hb_running: list = []  # (future, request)
         st = self.stats
         t0 = time.perf_counter()
+        t0_wall = time.time()
         while finished < len(order):
             now = time.perf_counter() - t0
             while nxt < len(order) and (not realtime or order[nxt].arrival <= now):
                 r = order[nxt]
                 if r.kind == "hb":  # straight to the HB workers (their pool queues the excess)
-                    hb_running.append((self._hb_pool.submit(self._hb_job, r, t0), r))
+                    if self._hb_procs:
+                        fut = self._hb_pool.submit(_hb_proc_job, r.mesh, float(r.hb["omega"]), float(r.hb["U"]),
+                                                   np.asarray(r.hb["dyn"], dtype=np.float64))
+                    else:
+                        fut = self._hb_pool.submit(self._hb_job, r, t0)
+                    hb_running.append((fut, r))
                 else:
                     waiting.setdefault((r.kind, r.mesh), deque()).append(r)
                 nxt += 1
             progressed = False
             for fut, r in [x for x in hb_running if x[0].done()]:
-                r.forces, r.iterations = fut.result()  # re-raises a worker's error
+                res = fut.result()  # re-raises a worker's error
+                r.forces, r.iterations = res[0], res[1]
+                if self._hb_procs:  # the worker's start on the wall clock
+                    r.admitted = res[2] - t0_wall
                 r.finished = time.perf_counter() - t0
                 r.batch = 1
                 r.cl = float(r.forces[0, 0])
