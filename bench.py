@@ -631,7 +631,7 @@ def hb_served(wing, flap, X, n_requests, workers, processes=True):
             "served_equals_alone_bitwise": bool(same)}
 
 
-def hb_bench(n_speeds, with_cpu, n_served=64, workers=4, processes=True):
+def hb_bench(n_speeds, with_cpu, n_served=64, workers=5, processes=True):
     import tempfile
 
     import torch
@@ -889,7 +889,7 @@ def main():
     ap.add_argument("--hb-speeds", type=int, default=8, help="airspeeds of the timed HB continuation")
     ap.add_argument("--no-hb", action="store_true")
     ap.add_argument("--hb-served", type=int, default=64, help="airspeeds served as concurrent HB requests (0: none)")
-    ap.add_argument("--hb-workers", type=int, default=4, help="HB worker processes of the served run")
+    ap.add_argument("--hb-workers", type=int, default=5, help="HB worker processes of the served run")
     ap.add_argument("--hb-threads", action="store_true", help="HB workers as threads of this process")
     ap.add_argument("--hb-only", action="store_true", help="only the hb section")
     ap.add_argument("--two-dof-steps", type=int, default=400, help="timed coupled pitch/heave steps")
