@@ -475,8 +475,10 @@ __device__ __forceinline__ void lattice_chunk(const double* sverts, const double
 
 // zs: simulations (blockIdx.z) of one launch find their receivers, wake and circulations z * zs
 // elements after simulation 0's (the tables' pointers) and write their partials z * wzs after.
+// At least 4 waves per SIMD: the 4-row form needs 130 registers unhinted (3 waves) and fits 128
+// with the hint, without scratch (the same instructions, allocated tighter).
 template <typename TS, int RH>
-__global__ __launch_bounds__(kThreads) void k_wake_lattice(TS* __restrict__ partial, RecvTable<TS> rt,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_wake_lattice(TS* __restrict__ partial, RecvTable<TS> rt,
                                                            WakeTable<TS> wt, int64_t zs = 0, int64_t wzs = 0) {
     const int64_t zo = (int64_t)blockIdx.z * zs;
     partial += (int64_t)blockIdx.z * wzs;
