@@ -1036,10 +1036,12 @@ struct UpdCfg<double> {
 
 constexpr int kUpdDK = 16;  // k-slice depth
 
+// The FP32 128 x 64 tile is held to 6 waves per SIMD (its accumulators then sit in 80 vector registers
+// beside the operands instead of 32 accumulation registers: 5 -> 6 waves; measured ~1 % on getrf).
 template <typename T, int TI, int TJ, int WAVES_I>
-__global__ __launch_bounds__(256) void k_update(T* __restrict__ C, int64_t ldc, const T* __restrict__ A, int64_t lda,
-                                                const T* __restrict__ B, int64_t ldb, int m, int n, int kb,
-                                                int64_t sbatch) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && TI * TJ == 8192 ? 6 : 1)))
+void k_update(T* __restrict__ C, int64_t ldc, const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
+              int64_t ldb, int m, int n, int kb, int64_t sbatch) {
     using G = UpdCfg<T>;
     constexpr int BM = G::BM, KS = G::KS, DK = kUpdDK;
     constexpr int WI = TI / WAVES_I, WJ = TJ / (4 / WAVES_I), NBI = WI / BM, NBJ = WJ / BM;
