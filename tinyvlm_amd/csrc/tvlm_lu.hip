@@ -149,54 +149,6 @@ __global__ __launch_bounds__(256) void k_trsm(T* __restrict__ a, int64_t lda, in
         if (r < kb) cj[r] = x[r];
 }
 
-// C(m x n) -= A(m x k) B(k x n), all column-major; 64x64 tiles, 256 threads, 4x4 per thread.
-template <typename T>
-__global__ __launch_bounds__(256) void k_update(T* __restrict__ C, int64_t ldc, const T* __restrict__ A, int64_t lda,
-                                                const T* __restrict__ B, int64_t ldb, int m, int n, int k) {
-    constexpr int BM = 64, BN = 64, BK = 16;
-    __shared__ T sa[BK][BM + 1];
-    __shared__ T sb[BK][BN + 1];
-    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-    const int row0 = blockIdx.x * BM, col0 = blockIdx.y * BN;
-    T acc[4][4] = {};
-    for (int kk = 0; kk < k; kk += BK) {
-        for (int e = threadIdx.x; e < BM * BK; e += 256) {
-            const int r = e % BM, q = e / BM;
-            const int gr = row0 + r, gq = kk + q;
-            sa[q][r] = (gr < m && gq < k) ? A[gr + (int64_t)gq * lda] : T(0);
-        }
-        for (int e = threadIdx.x; e < BN * BK; e += 256) {
-            const int q = e % BK, c = e / BK;
-            const int gq = kk + q, gc = col0 + c;
-            sb[q][c] = (gq < k && gc < n) ? B[gq + (int64_t)gc * ldb] : T(0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < BK; q++) {
-            T ra[4], rb[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) ra[u] = sa[q][tx + 16 * u];
-#pragma unroll
-            for (int v = 0; v < 4; v++) rb[v] = sb[q][ty + 16 * v];
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) acc[u][v] += ra[u] * rb[v];
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int gr = row0 + tx + 16 * u;
-        if (gr >= m) continue;
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int gc = col0 + ty + 16 * v;
-            if (gc < n) C[gr + (int64_t)gc * ldc] -= acc[u][v];
-        }
-    }
-}
-
 // ---- fast getrf: register-resident panel, fused interchange + TRSM, MFMA trailing update ------
 //
 // k_panel_reg: one work-group of NT threads factors the panel rows [k0, n) x columns [k0, k0+kb),
