@@ -585,23 +585,24 @@ HB_OMEGA = 0.8
 HB_U_REF = 6.0
 
 
-def hb_served(wing, flap, X, n_requests, workers, processes=True):
+def hb_served(wing, flap, X, n_requests, workers, processes=True, wake="lattice"):
     """The harmonic-balance configuration served: n_requests airspeeds U = linspace(0.5, 1.5) U_ref as
     concurrent cold-start requests through Server on `workers` HB worker processes (threads with
     processes=False; one HBVLM and stream each), against the same cold solves one after another on
-    one solver."""
+    one solver; both with the wake induction `wake` (HBVLM.wake_form)."""
     from tinyvlm_amd import serve as S
     from tinyvlm_amd.hb import HBVLM
 
     K = 2 * HB_H + 1
     speeds = np.linspace(0.5, 1.5, n_requests) * HB_U_REF
-    cfg = {"wf": {"meshes": [wing, flap], "harmonics": HB_H, "scaling": 1.0 / HB_B}}
+    cfg = {"wf": {"meshes": [wing, flap], "harmonics": HB_H, "scaling": 1.0 / HB_B, "wake": wake}}
 
     def req(i, U):
         return S.Request(i, "wf", kind="hb", hb={"omega": HB_OMEGA, "U": float(U), "dyn": X})
 
     # one after another, cold (a fresh start for every airspeed): a sample of the same requests
     seq = HBVLM("hip", [wing, flap])
+    seq.wake_form = wake
     seq.init(HB_H, 1.0 / HB_B)
     F = np.zeros((3, K))
     seq.run(HB_OMEGA, float(speeds[0]), X, F)  # warm-up
@@ -623,7 +624,7 @@ def hb_served(wing, flap, X, n_requests, workers, processes=True):
     lat = [r.latency * 1e3 for r in reqs]
     same = all(np.array_equal(r.forces, seq_F[r.hb["U"]]) for r in reqs if r.hb["U"] in seq_F)
     return {"requests": n_requests, "workers": workers, "worker_kind": "processes" if processes else "threads",
-            "value": round(n_requests / wall, 3), "unit": "HB solves/s",
+            "wake_form": wake, "value": round(n_requests / wall, 3), "unit": "HB solves/s",
             "sequential_cold": {"value": round(seq_rate, 3), "unit": "HB solves/s", "solves": len(sample)},
             "speedup_vs_sequential_cold": round(n_requests / wall / seq_rate, 3),
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1), "p99": round(float(np.percentile(lat, 99)), 1)},
@@ -631,7 +632,7 @@ def hb_served(wing, flap, X, n_requests, workers, processes=True):
             "served_equals_alone_bitwise": bool(same)}
 
 
-def hb_bench(n_speeds, with_cpu, n_served=64, workers=5, processes=True):
+def hb_bench(n_speeds, with_cpu, n_served=64, workers=5, processes=True, wake="lattice"):
     import tempfile
 
     import torch
@@ -650,30 +651,43 @@ def hb_bench(n_speeds, with_cpu, n_served=64, workers=5, processes=True):
     X[1, 2] = math.radians(1.0)  # pitch 1 deg
     speeds = np.linspace(0.5, 1.5, n_speeds) * HB_U_REF
     hb = HBVLM("hip", [wing, flap])
-    hb.init(HB_H, 1.0 / HB_B)
-    F_cold = np.zeros((3, K))
-    hb.run(HB_OMEGA, float(speeds[0]), X, F_cold)  # cold start at the first airspeed (also warm-up)
-    hb.init(HB_H, 1.0 / HB_B)
-    torch.cuda.synchronize()
-    lat, iters = [], []
-    t0 = time.perf_counter()
-    for U in speeds:
-        t1 = time.perf_counter()
-        F = np.zeros((3, K))
-        hb.run(HB_OMEGA, float(U), X, F)
-        lat.append((time.perf_counter() - t1) * 1e3)
-        iters.append(hb.iterations)
-    wall = time.perf_counter() - t0
+
+    def continuation(form):
+        """Cold start at the first airspeed (also the warm-up), then the timed continuation."""
+        hb.wake_form = form
+        hb.init(HB_H, 1.0 / HB_B)
+        F_cold = np.zeros((3, K))
+        hb.run(HB_OMEGA, float(speeds[0]), X, F_cold)
+        hb.init(HB_H, 1.0 / HB_B)
+        torch.cuda.synchronize()
+        lat, iters, loads = [], [], []
+        t0 = time.perf_counter()
+        for U in speeds:
+            t1 = time.perf_counter()
+            F = np.zeros((3, K))
+            hb.run(HB_OMEGA, float(U), X, F)
+            lat.append((time.perf_counter() - t1) * 1e3)
+            iters.append(hb.iterations)
+            loads.append(F)
+        return n_speeds / (time.perf_counter() - t0), lat, iters, F_cold, loads
+
+    rate_x, lat_x, iters_x, F_cold_x, loads_x = continuation("exact")
+    rate, lat, iters, F_cold, loads = continuation(wake) if wake != "exact" else \
+        (rate_x, lat_x, iters_x, F_cold_x, loads_x)
+    d_forms = max(float(np.max(np.abs(a - b))) for a, b in zip(loads, loads_x))
     out = {
         "workload": "hb_rect_wing_flap_20+10c_x_20s_H5", "panels": hb.N, "harmonics": HB_H, "time_instances": K,
-        "value": round(n_speeds / wall, 3), "unit": "HB solves/s", "airspeeds": n_speeds,
+        "wake_form": wake, "value": round(rate, 3), "unit": "HB solves/s", "airspeeds": n_speeds,
         "latency_ms_per_solve": {"p50": round(float(np.percentile(lat, 50)), 2),
                                  "p99": round(float(np.percentile(lat, 99)), 2)},
         "anderson_iterations": iters, "dtype": "f64",
-        "note": "continuation: each airspeed warm-started from the previous solution; the Anderson map (replayed from a hipGraph) is bound by its 11 exact-form FP64 wake inductions",
+        "exact_wake": {"value": round(rate_x, 3), "unit": "HB solves/s", "anderson_iterations": iters_x,
+                       "max_abs_dF_vs_this_form": d_forms},
+        "note": "continuation: each airspeed warm-started from the previous solution; the Anderson map "
+                "(replayed from a hipGraph) is bound by its 11 FP64 wake inductions",
     }
     if n_served:
-        out["served"] = hb_served(wing, flap, X, n_served, workers, processes)
+        out["served"] = hb_served(wing, flap, X, n_served, workers, processes, wake)
     if with_cpu:
         from oracle import hb as ohb
         from oracle import cpuref as C
@@ -688,6 +702,7 @@ def hb_bench(n_speeds, with_cpu, n_served=64, workers=5, processes=True):
                                "kind": "port", "sample": f"one cold-start oracle HBVLM::run at the first airspeed "
                                                          f"({el:.1f} s, {o.iterations} Anderson iterations)"}
         out["accuracy"] = {"max_abs_dF_cold_gpu_vs_cpu": float(np.max(np.abs(F_cold - Fo))),
+                           "max_abs_dF_cold_exact_wake_gpu_vs_cpu": float(np.max(np.abs(F_cold_x - Fo))),
                            "max_abs_F": float(np.max(np.abs(Fo)))}
     return out
 
@@ -891,6 +906,8 @@ def main():
     ap.add_argument("--hb-served", type=int, default=64, help="airspeeds served as concurrent HB requests (0: none)")
     ap.add_argument("--hb-workers", type=int, default=5, help="HB worker processes of the served run")
     ap.add_argument("--hb-threads", action="store_true", help="HB workers as threads of this process")
+    ap.add_argument("--hb-wake", choices=("exact", "lattice"), default="lattice",
+                    help="wake induction of the HB map (the other form's continuation is timed beside it)")
     ap.add_argument("--hb-only", action="store_true", help="only the hb section")
     ap.add_argument("--two-dof-steps", type=int, default=400, help="timed coupled pitch/heave steps")
     ap.add_argument("--two-dof-large-steps", type=int, default=100, help="timed steps of the 600-panel section")
@@ -922,7 +939,7 @@ def main():
         return
     if args.hb_only:
         print(json.dumps(hb_bench(args.hb_speeds, not args.no_cpu, args.hb_served, args.hb_workers,
-                                  not args.hb_threads)))
+                                  not args.hb_threads, args.hb_wake)))
         return
     if args.flutter_only:
         print(json.dumps(flutter_bench(args.flutter_airspeeds, args.flutter_steps, 20, args.flutter_plain_steps,
@@ -1123,7 +1140,8 @@ def main():
         out["lu"] = lu
         out["served_mix"] = serve_mix_bench(args.mix_span_hi, args.mix_span_lo, args.mix_saturation, args.mix_lanes)
     if not args.no_hb:
-        out["hb"] = hb_bench(args.hb_speeds, not args.no_cpu, args.hb_served, args.hb_workers, not args.hb_threads)
+        out["hb"] = hb_bench(args.hb_speeds, not args.no_cpu, args.hb_served, args.hb_workers, not args.hb_threads,
+                             args.hb_wake)
     if not args.no_two_dof:
         out["two_dof"] = two_dof_bench(args.two_dof_steps, not args.no_cpu)
         out["two_dof_600"] = two_dof_bench(args.two_dof_large_steps, not args.no_cpu, large=True)

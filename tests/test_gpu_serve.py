@@ -244,12 +244,14 @@ def test_large_mix_meshes_through_the_server(tmp_path):
         assert abs(r.cl - plain["cl"][0]) <= TOL, r.rid
 
 
-@pytest.mark.parametrize("processes", [False, True], ids=["threads", "processes"])
-def test_hb_requests_served_beside_steady(root, oracle, processes):
+@pytest.mark.parametrize("processes,wake", [(False, "exact"), (True, "exact"), (True, "lattice")],
+                         ids=["threads", "processes", "processes-lattice"])
+def test_hb_requests_served_beside_steady(root, oracle, processes, wake):
     """Harmonic-balance requests (the reference script's wing 9x5 + flap 3x5, H = 2, several
     airspeeds) served on worker threads, or on two worker processes, beside steady requests: every
     request starts cold, so its loads equal (bit for bit) the same solve on a fresh HBVLM alone; the
-    steady results match their alone solves as in the other mix tests."""
+    steady results match their alone solves as in the other mix tests.  With wake = "lattice" the
+    configuration's solvers evaluate the wake induction over the vortex lattice (HBVLM.wake_form)."""
     import os
 
     from tinyvlm_amd import capi
@@ -265,10 +267,11 @@ def test_hb_requests_served_beside_steady(root, oracle, processes):
     X[1, 2] = 0.02
     X[2, 1] = 0.05
     speeds = [5.0, 5.5, 6.0, 6.5, 7.0, 7.5]
-    print(f"[case] serve: hb requests beside steady, worker processes {processes}", flush=True)
+    print(f"[case] serve: hb requests beside steady, worker processes {processes}, wake {wake}", flush=True)
     alone = {}
     for U in speeds:
         h = HBVLM("hip", paths)
+        h.wake_form = wake
         h.init(H, scale)
         h.use_graph = False  # as the serving loop's solvers (the replay gives the same bits anyway)
         F = np.zeros((3, K))
@@ -277,7 +280,7 @@ def test_hb_requests_served_beside_steady(root, oracle, processes):
     v = oracle.rectangular_wing(16, 8, chord=1.0, half_span=4.0)
     reqs = [S.Request(i, "r16", float(np.deg2rad(a)), arrival=0.0) for i, a in enumerate((1.0, 2.0, 3.0, 4.0))]
     reqs += [S.Request(100 + k, "wf", kind="hb", hb={"omega": 0.8, "U": U, "dyn": X}) for k, U in enumerate(speeds)]
-    with S.Server({"r16": [v]}, lanes=2, hb={"wf": {"meshes": paths, "harmonics": H, "scaling": scale}},
+    with S.Server({"r16": [v]}, lanes=2, hb={"wf": {"meshes": paths, "harmonics": H, "scaling": scale, "wake": wake}},
                   hb_workers=2 if processes else 4, hb_processes=processes) as srv:
         srv.run(reqs, realtime=False)
     for r in reqs[4:]:

@@ -248,6 +248,22 @@ def rhs_assemble_wake_influence(rhs, gamma_wake, colloc, normals, verts_wake, li
           "rhs_assemble_wake_influence")
 
 
+def rhs_assemble_wake_influence_lattice(rhs, gamma_wake, colloc, normals, verts_wake, lifting, iteration,
+                                        stream=None):
+    """tvlm_rhs_assemble_wake_influence_lattice_f64: the same wake induction evaluated on the wake's vortex
+    lattice in the FP64 seeded form (no core model); float64 tensors only."""
+    torch = _torch()
+    if rhs.dtype != torch.float64:
+        raise TvlmError("rhs_assemble_wake_influence_lattice: float64 only")
+    n = len(colloc)
+    lift = (ctypes.c_uint8 * n)(*[1 if x else 0 for x in lifting])
+    check(lib().tvlm_rhs_assemble_wake_influence_lattice_f64(ctypes.byref(view(rhs)), ctypes.c_int32(n),
+                                                            views(gamma_wake), views(colloc), views(normals),
+                                                            views(verts_wake), lift, ctypes.c_int32(iteration),
+                                                            stream_ptr(stream)),
+          "rhs_assemble_wake_influence_lattice")
+
+
 def rhs_assemble_wake_influence_noalloc(rhs, gamma_wake, colloc, normals, verts_wake, lifting, iteration, sigma=0.0,
                                         stream=None):
     """tvlm_rhs_assemble_wake_influence_noalloc_f64: the FP64 wake induction with its tile partials in

@@ -131,10 +131,13 @@ int lhs_assemble(const tvlm_view* lhs, int32_t n_surf, const tvlm_view* colloc, 
 // fixed_ws: when non-null, the tile partials go to this caller-provided device buffer of
 // fixed_bytes (nothing is allocated; a launch that needs more is rejected before any work is
 // queued); when null, to the library's per-stream workspace (grown on demand).
+// lattice (FP64 only): the wake's vortex lattice in the FP64 seeded form (k_wake_lattice, the served
+// UVLM loop's kernel: every lattice segment once, with the difference of the circulations of the
+// two rings beside it) instead of ring by ring.
 template <typename T>
 int wake_influence(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_wake, const tvlm_view* colloc,
                    const tvlm_view* normals, const tvlm_view* verts_wake, const uint8_t* lifting, int32_t iteration,
-                   T sigma, void* stream, void* fixed_ws = nullptr, size_t fixed_bytes = 0) {
+                   T sigma, void* stream, void* fixed_ws = nullptr, size_t fixed_bytes = 0, bool lattice = false) {
     TVLM_REQUIRE(rhs && gamma_wake && colloc && normals && verts_wake && lifting, "null argument");
     TVLM_REQUIRE(n_surf >= 1 && n_surf <= TVLM_MAX_SURFACES, "n_surf %d out of range", n_surf);
     TVLM_REQUIRE(iteration >= 0, "negative iteration");
@@ -183,6 +186,23 @@ int wake_influence(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_
     }
     wt.total_tiles = tiles;
     if (tiles == 0 || total == 0) return TVLM_OK;
+    const int recv_blocks = (total + kRecv - 1) / kRecv;
+    int lat_rh = 1;
+    if (lattice) {  // one rows-per-tile for the launch, spans split while it holds few work-groups
+        TVLM_REQUIRE(sizeof(T) == 8 && sigma == T(0), "the lattice form is FP64 without a core model");
+        int split = 1;
+        lattice_launch_shape(wt, recv_blocks, lattice_target_groups(), lat_rh, split);
+        tiles = 0;
+        for (int m = 0; m < wt.n; m++) {
+            WakeSurface<T>& w = wt.s[m];
+            w.rh = lat_rh;
+            w.sw = lattice_span_tile(w.ns, split);
+            w.tiles_span = (w.ns + w.sw - 1) / w.sw;
+            w.tile_begin = tiles;
+            tiles += w.tiles_span * ((w.row_end - w.row_begin + lat_rh - 1) / lat_rh);
+        }
+        wt.total_tiles = tiles;
+    }
     void* ws = nullptr;
     hipStream_t st = as_stream(stream);
     const size_t need = sizeof(T) * (size_t)tiles * (size_t)total;
@@ -196,6 +216,20 @@ int wake_influence(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_
         TVLM_TRY(device_workspace(need, &ws, kWsWakePartials, st));
     }
     dim3 grid((total + kRecv - 1) / kRecv, tiles);
+    if constexpr (sizeof(T) == 8) {
+        if (lattice) {
+            if (lat_rh == 1)
+                hipLaunchKernelGGL((k_wake_lattice<T, 1>), grid, dim3(kThreads), 0, st, (T*)ws, rt, wt, 0, 0);
+            else if (lat_rh == 2)
+                hipLaunchKernelGGL((k_wake_lattice<T, 2>), grid, dim3(kThreads), 0, st, (T*)ws, rt, wt, 0, 0);
+            else
+                hipLaunchKernelGGL((k_wake_lattice<T, 4>), grid, dim3(kThreads), 0, st, (T*)ws, rt, wt, 0, 0);
+            TVLM_LAUNCHED();
+            launch_wake_reduce<T>(static_cast<T*>(rhs->ptr), rhs->stride[0], (const T*)ws, tiles, total, st);
+            TVLM_LAUNCHED();
+            return TVLM_OK;
+        }
+    }
     if constexpr (sizeof(T) == 4) {
         if (sigma != T(0))
             hipLaunchKernelGGL((k_wake_partials<T, T, 1>), grid, dim3(kThreads), 0, st, (T*)ws, rt, wt, sigma);
@@ -377,6 +411,13 @@ int tvlm_rhs_assemble_wake_influence_noalloc_f64(const tvlm_view* rhs, int32_t n
     TVLM_TRY(tvlm::scratch_address(&ws));
     return tvlm::wake_influence<double>(rhs, n_surf, gamma_wake, colloc, normals, verts_wake, lifting, iteration, sigma,
                                         stream, ws, tvlm::kNoAllocScratchBytes);
+}
+int tvlm_rhs_assemble_wake_influence_lattice_f64(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_wake,
+                                                 const tvlm_view* colloc, const tvlm_view* normals,
+                                                 const tvlm_view* verts_wake, const uint8_t* lifting, int32_t iteration,
+                                                 void* stream) {
+    return tvlm::wake_influence<double>(rhs, n_surf, gamma_wake, colloc, normals, verts_wake, lifting, iteration, 0.0,
+                                        stream, nullptr, 0, true);
 }
 int tvlm_rhs_assemble_wake_influence_f32(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_wake,
                                          const tvlm_view* colloc, const tvlm_view* normals, const tvlm_view* verts_wake,

@@ -162,6 +162,10 @@ class HBVLM:
         # replayed from a hipGraph (use_graph = False: launched call by call, the same kernels)
         self.stream = torch.cuda.Stream()
         self.use_graph = True
+        # wake induction of the map: "exact" (Backend::rhs_assemble_wake_influence ring by ring, the
+        # reference's evaluation) or "lattice" (the same sum over the wake's vortex lattice, every
+        # segment once, FP64 seeded; agrees with "exact" to rounding)
+        self.wake_form = "exact"
 
     # ---- HBVLM::init (lib/libhbvlm3.cpp:168-233) ------------------------------------------------
     def init(self, harmonics: int, scaling: float):
@@ -240,6 +244,8 @@ class HBVLM:
     def solve(self, t_start, omega, v: Vars, fs_node, nodes, force_t):
         """HBVLM::run (lib/libhbvlm3.cpp:262-552)."""
         torch = self.torch
+        if self.wake_form not in ("exact", "lattice"):
+            raise ValueError(f"wake_form {self.wake_form!r}: expected 'exact' or 'lattice'")
         Kc, Hh, N = self.K, self.harmonics, self.N
         period = 2.0 * PI_F / omega
         period_interval = period / Kc
@@ -304,6 +310,8 @@ class HBVLM:
         n0 = self.dims[0][0] * self.dims[0][1]
         ns1, nc1 = self.dims[1]
         lifting = [False, True]
+        wake_rhs = H.rhs_assemble_wake_influence if self.wake_form == "exact" else \
+            H.rhs_assemble_wake_influence_lattice
 
         def hb_vlm_iter(x_in, x_out):
             g_in = x_in.view(Kc, N).t()  # {N, K} column-major
@@ -314,8 +322,7 @@ class HBVLM:
                 ts, t = steps[s]
                 I = self.inst[s]
                 H.gamma_wake_from_coeffs(gamma_wake[s][1], te, Hh, t, omega, dt, ts)
-                H.rhs_assemble_wake_influence(self.rhs[:, s], gamma_wake[s], I["colloc"], I["normals"], wake,
-                                              lifting, ts)
+                wake_rhs(self.rhs[:, s], gamma_wake[s], I["colloc"], I["normals"], wake, lifting, ts)
             # A G D^T = R  <=>  A G = R D  (D orthogonal)
             H.gemm(1.0, self.rhs, dft_d, 0.0, g_out)
             H.getrs(self.lhs, self.ipiv, g_out)

@@ -146,6 +146,7 @@ def _hb_proc_init(cfgs: dict) -> None:
     H.init(0)
     for name, cfg in cfgs.items():
         h = HBVLM("hip", list(cfg["meshes"]))
+        h.wake_form = cfg.get("wake", "exact")
         h.init(int(cfg["harmonics"]), float(cfg["scaling"]))
         _HB_SOLVERS[name] = h
 
@@ -254,9 +255,10 @@ class Server:
         requests; far_wake = [(age, row_group, span_group), ...] aggregates the wake beyond `age` steps
         (UVLMPlan.set_far_wake; accept it only where it keeps |dCL| within the bar).
         unsteady_batch: most simulations of one mesh a lane runs together in one plan.
-        hb: name -> {"meshes": [wing_path, flap_path], "harmonics": H, "scaling": s} for the HB
-        configurations that take kind = "hb" requests; they run on hb_workers worker threads, each
-        with its own HBVLM (and stream) per configuration, beside the lanes.  An HB solve's Anderson
+        hb: name -> {"meshes": [wing_path, flap_path], "harmonics": H, "scaling": s[, "wake": form]}
+        for the HB configurations (form: HBVLM.wake_form, "exact" unless given) that take kind = "hb"
+        requests; they run on hb_workers worker threads, each with its own HBVLM (and stream) per
+        configuration, beside the lanes.  An HB solve's Anderson
         loop is driven from Python launch by launch, so more workers in one process contend for the
         interpreter (measured: 1 worker 8.8, 2 workers 7.6, 4 workers 6.9 solves/s).  hb_processes:
         the workers are processes instead (spawned, each with its own HIP context, solvers and graph
@@ -296,7 +298,8 @@ class Server:
                 import multiprocessing
 
                 cfgs = {k: {"meshes": list(c["meshes"]), "harmonics": int(c["harmonics"]),
-                            "scaling": float(c["scaling"])} for k, c in self.hb.items()}
+                            "scaling": float(c["scaling"]), "wake": str(c.get("wake", "exact"))}
+                        for k, c in self.hb.items()}
                 self._hb_pool = ProcessPoolExecutor(max_workers=hb_workers,
                                                     mp_context=multiprocessing.get_context("spawn"),
                                                     initializer=_hb_proc_init, initargs=(cfgs,))
@@ -402,6 +405,7 @@ class Server:
         if name not in cache:
             cfg = self.hb[name]
             h = HBVLM("hip", list(cfg["meshes"]))
+            h.wake_form = cfg.get("wake", "exact")
             h.init(int(cfg["harmonics"]), float(cfg["scaling"]))
             # no graph capture here: capturing synchronizes the device first, which another thread's
             # capture (a UVLM lane, another worker) forbids; the map's launches run one by one
