@@ -245,8 +245,9 @@ int tvlm_rhs_assemble_wake_influence_noalloc_f64(const tvlm_view* rhs, int32_t n
  * in the FP64 seeded form (1/sqrt and 1/x from the FP64 hardware estimates plus one Newton step;
  * every lattice segment once, with the difference of the circulations of the rings beside it): the
  * same arguments (no core model: sigma = 0) and the same rhs as tvlm_rhs_assemble_wake_influence_f64
- * to about 1e-14 relative, in fewer operations.  No counterpart in the reference; the served HB
- * solvers use it. */
+ * to rounding (within 2e-12 of the largest entry: the estimates' few ulp are amplified where a
+ * receiver lies near the extension of a segment), in fewer operations.  No counterpart in the
+ * reference; the HB solver takes it with wake_form "lattice". */
 int tvlm_rhs_assemble_wake_influence_lattice_f64(const tvlm_view* rhs, int32_t n_surf, const tvlm_view* gamma_wake,
                                                  const tvlm_view* colloc, const tvlm_view* normals,
                                                  const tvlm_view* verts_wake, const uint8_t* lifting, int32_t iteration,
