@@ -53,7 +53,7 @@ inline bool plane_contiguous(const tvlm_view& v) {
 // tile partials, LU row-interchange lists, triangular-solve block inverses, the exchange buffers of
 // the cooperative LU panels.  Work queued on different streams never shares a buffer, so concurrent
 // solves on separate streams are safe; work on one stream is ordered by the stream.
-enum WorkspaceSlot { kWsWakePartials = 0, kWsLuPerm = 1, kWsSolveInv = 2, kWsLuCoop = 3, kWorkspaceSlots = 4 };
+enum WorkspaceSlot { kWsWakePartials = 0, kWsLuPerm = 1, kWsSolveInv = 2, kWsLuCoop = 3, kWsLsq = 4, kWorkspaceSlots = 5 };
 int device_workspace(size_t bytes, void** out, int slot, hipStream_t stream);
 // Frees what the library keeps for one stream on the current device (its scratch buffers, its
 // rocBLAS handle, its look-ahead side stream); the stream's work must have finished.

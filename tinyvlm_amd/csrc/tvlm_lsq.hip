@@ -14,18 +14,31 @@
 // unblocked ?orm2r order H(1) first), then X solves R X = B(0:N) by column-oriented back substitution
 // (?trsm Left/Upper/NoTrans).
 //
-// One work-group of 1024 threads does the whole factorisation: the problems this serves are
-// tall and thin (Anderson histories of <= 10 columns over N x K rows).  Each column step is a
-// strided pass over the remaining columns with per-thread partial dot products held in registers
-// (statically indexed, N + NRHS <= 32), one fixed-order block reduction, and one rank-1 update.
+// The rows are spread over up to 256 work-groups of 256 threads (row i in group (i / 256) % groups),
+// with one launch per column: launch j finishes reflector j - 1 and prepares reflector j.
+//  - From the previous launch's per-group partial sums (fixed order over the groups, the same in every
+//    group), every group forms reflector j - 1's tau, beta and the scale of its vector, and
+//    w_k = tau (C(j-1,k) + scal sum_{i>j-1} A(i,j-1) C(i,k)) for each target column k (the columns
+//    right of j - 1, then the B columns); the row j - 1 values come from a head record that the
+//    group owning that row wrote in the previous launch.  It then applies H = I - tau v v^T to its
+//    own rows: v_i = A(i,j-1) scal stored below the diagonal, C(i,k) -= w_k v_i, A(j-1,j-1) = beta.
+//  - In the same pass over its rows, each group sums A(i,j)^2 and A(i,j) C(i,k) over its rows i > j
+//    (fixed order: rows, then a butterfly in each wave, then the waves in order) into its partial
+//    record; the group owning row j writes that row's values into the head record.
+// Partial and head records alternate between two buffers, so a launch reads one while it writes the
+// other.  Launch 0 also records each group's max |A|; the last launch (j = n) then zeroes B when A
+// is all zero (as ?gels does) or, in the group that owns row n - 1 (whose last update it has just
+// made), sets info and solves R X = (Q^T B)(0:N).
 #include "tvlm_internal.hpp"
 
 namespace tvlm {
 namespace {
 
-constexpr int kLsqThreads = 1024;
+constexpr int kLsqThreads = 256;
 constexpr int kLsqWaves = kLsqThreads / 64;
-constexpr int kLsqMaxCols = 32;  // N + NRHS
+constexpr int kLsqMaxCols = 32;        // N + NRHS
+constexpr int kLsqMaxGroups = 256;
+constexpr int kLsqRec = kLsqMaxCols + 1;  // partial / head record: one value, then one per target
 
 template <typename T>
 __device__ __forceinline__ T wave_sum_xor(T v) {
@@ -41,110 +54,145 @@ __device__ __forceinline__ T wave_max_xor(T v) {
     return v;
 }
 
-// target column k of the step at column j: the A columns right of j, then the B columns
+// target column k of the reflector of column c: the A columns right of c, then the B columns
 template <typename T>
-__device__ __forceinline__ T* target(T* a, int64_t lda, T* b, int64_t ldb, int j, int na, int k) {
-    return k < na ? a + (int64_t)(j + 1 + k) * lda : b + (int64_t)(k - na) * ldb;
+__device__ __forceinline__ T* target(T* a, int64_t lda, T* b, int64_t ldb, int c, int na, int k) {
+    return k < na ? a + (int64_t)(c + 1 + k) * lda : b + (int64_t)(k - na) * ldb;
 }
 
+// ws: part[2][groups][kLsqRec], head[2][kLsqRec], amax[groups]
 template <typename T>
-__global__ __launch_bounds__(kLsqThreads) void k_gels(T* __restrict__ a, int64_t lda, int m, int n, T* __restrict__ b,
-                                                      int64_t ldb, int nrhs, int32_t* __restrict__ info) {
-    __shared__ T red[kLsqWaves][kLsqMaxCols];
+__global__ __launch_bounds__(kLsqThreads) void k_qr_step(T* __restrict__ a, int64_t lda, int m, int n,
+                                                         T* __restrict__ b, int64_t ldb, int nrhs, T* __restrict__ ws,
+                                                         int groups, int j, int32_t* __restrict__ info) {
+    __shared__ T sum_prev[kLsqRec];
     __shared__ T wsh[kLsqMaxCols];
-    __shared__ T s_tau, s_beta, s_scal, s_amax;
-    __shared__ int s_info;
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    __shared__ T red[kLsqWaves][kLsqRec];
+    __shared__ T s_tau, s_beta, s_scal;
+    T* part = ws;
+    T* head = ws + (int64_t)2 * groups * kLsqRec;
+    T* amaxg = head + 2 * kLsqRec;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, g = blockIdx.x;
+    const int64_t step = (int64_t)groups * kLsqThreads;
 
-    // all-zero A: B = 0
-    T amax = 0;
-    for (int j = 0; j < n; j++)
-        for (int i = t; i < m; i += kLsqThreads) amax = fmax(amax, fabs(a[i + (int64_t)j * lda]));
-    amax = wave_max_xor(amax);
-    if (lane == 0) red[wv][0] = amax;
-    __syncthreads();
-    if (t == 0) {
-        T x = 0;
-        for (int w = 0; w < kLsqWaves; w++) x = fmax(x, red[w][0]);
-        s_amax = x;
-    }
-    __syncthreads();
-    if (s_amax == T(0)) {
-        for (int r = 0; r < nrhs; r++)
-            for (int i = t; i < m; i += kLsqThreads) b[i + (int64_t)r * ldb] = 0;
-        if (t == 0) info[0] = 0;
-        return;
-    }
-
-    for (int j = 0; j < n; j++) {
-        T* cj = a + (int64_t)j * lda;
-        // reflector: sum of squares below the diagonal
-        T s = 0;
-        for (int i = j + 1 + t; i < m; i += kLsqThreads) s += cj[i] * cj[i];
-        s = wave_sum_xor(s);
-        __syncthreads();  // red[] of the previous step is consumed
-        if (lane == 0) red[wv][0] = s;
+    // reflector of column c = j - 1
+    const int c = j - 1;
+    const int nac = n - c - 1, ntc = nac + nrhs;  // A targets, all targets of column c
+    T tau = 0, scal = 1, beta = 0;
+    if (j > 0) {
+        const T* pp = part + (int64_t)(c & 1) * groups * kLsqRec;
+        const T* hp = head + (c & 1) * kLsqRec;
+        if (t <= ntc) {
+            T v = 0;
+            for (int q = 0; q < groups; q++) v += pp[(int64_t)q * kLsqRec + t];
+            sum_prev[t] = v;
+        }
         __syncthreads();
         if (t == 0) {
-            T tot = 0;
-            for (int w = 0; w < kLsqWaves; w++) tot += red[w][0];
-            const T alpha = cj[j];
+            const T tot = sum_prev[0], alpha = hp[0];
             if (tot == T(0)) {
                 s_tau = 0;
                 s_beta = alpha;
                 s_scal = 1;
             } else {
                 const T mag = sqrt(alpha * alpha + tot);
-                const T beta = alpha >= T(0) ? -mag : mag;
-                s_tau = (beta - alpha) / beta;
-                s_beta = beta;
-                s_scal = T(1) / (alpha - beta);
+                const T bt = alpha >= T(0) ? -mag : mag;
+                s_tau = (bt - alpha) / bt;
+                s_beta = bt;
+                s_scal = T(1) / (alpha - bt);
             }
         }
         __syncthreads();
-        const T tau = s_tau;
-        if (tau != T(0)) {
-            const T scal = s_scal;
-            for (int i = j + 1 + t; i < m; i += kLsqThreads) cj[i] *= scal;
-            const int na = n - j - 1;
-            const int K = na + nrhs;
-            // w_k = C(j,k) + sum_{i>j} v_i C(i,k) for every target column k
-            T acc[kLsqMaxCols];
-#pragma unroll
-            for (int k = 0; k < kLsqMaxCols; k++) acc[k] = 0;
-            for (int i = j + 1 + t; i < m; i += kLsqThreads) {
-                const T vi = cj[i];
-#pragma unroll
-                for (int k = 0; k < kLsqMaxCols; k++)
-                    if (k < K) acc[k] += vi * target(a, lda, b, ldb, j, na, k)[i];
-            }
-#pragma unroll
-            for (int k = 0; k < kLsqMaxCols; k++) {
-                if (k < K) {
-                    const T v = wave_sum_xor(acc[k]);
-                    if (lane == 0) red[wv][k] = v;
-                }
-            }
-            __syncthreads();
-            if (t < K) {
-                T w = target(a, lda, b, ldb, j, na, t)[j];
-                for (int q = 0; q < kLsqWaves; q++) w += red[q][t];
-                wsh[t] = tau * w;
-            }
-            __syncthreads();
-            for (int i = j + t; i < m; i += kLsqThreads) {
-                const T vi = (i == j) ? T(1) : cj[i];
-#pragma unroll
-                for (int k = 0; k < kLsqMaxCols; k++)
-                    if (k < K) target(a, lda, b, ldb, j, na, k)[i] -= wsh[k] * vi;
-            }
-        }
-        __syncthreads();
-        if (t == 0) cj[j] = s_beta;
+        tau = s_tau;
+        scal = s_scal;
+        beta = s_beta;
+        if (t < ntc) wsh[t] = tau * (hp[1 + t] + scal * sum_prev[1 + t]);
         __syncthreads();
     }
 
-    // R X = (Q^T B)(0:N), skipped when R has an exactly zero diagonal entry
+    // this group's rows: finish reflector c, then the partial sums of column j
+    const bool next = j < n;
+    const int naj = n - j - 1, ntj = next ? naj + nrhs : 0;
+    T acc[kLsqMaxCols];
+#pragma unroll
+    for (int k = 0; k < kLsqMaxCols; k++) acc[k] = 0;
+    T ss = 0, amax = 0;
+    for (int64_t i = (int64_t)g * kLsqThreads + t; i < m; i += step) {
+        if (j == 0)
+            for (int q = 0; q < n; q++) amax = fmax(amax, fabs(a[i + (int64_t)q * lda]));
+        if (j > 0 && i >= c) {
+            T* ac = a + (int64_t)c * lda;
+            if (tau != T(0)) {
+                T v = T(1);
+                if (i > c) {
+                    v = ac[i] * scal;
+                    ac[i] = v;
+                }
+#pragma unroll
+                for (int k = 0; k < kLsqMaxCols; k++)
+                    if (k < ntc) target(a, lda, b, ldb, c, nac, k)[i] -= wsh[k] * v;
+            }
+            if (i == c) ac[i] = beta;
+        }
+        if (next && i > j) {
+            const T aij = a[i + (int64_t)j * lda];
+            ss += aij * aij;
+#pragma unroll
+            for (int k = 0; k < kLsqMaxCols; k++)
+                if (k < ntj) acc[k] += aij * target(a, lda, b, ldb, j, naj, k)[i];
+        }
+        if (next && i == j) {
+            T* hw = head + (j & 1) * kLsqRec;
+            hw[0] = a[i + (int64_t)j * lda];
+            for (int k = 0; k < ntj; k++) hw[1 + k] = target(a, lda, b, ldb, j, naj, k)[i];
+        }
+    }
+    if (j == 0) {
+        amax = wave_max_xor(amax);
+        if (lane == 0) red[wv][0] = amax;
+        __syncthreads();
+        if (t == 0) {
+            T x = 0;
+            for (int w = 0; w < kLsqWaves; w++) x = fmax(x, red[w][0]);
+            amaxg[g] = x;
+        }
+        __syncthreads();
+    }
+    if (next) {
+        ss = wave_sum_xor(ss);
+        if (lane == 0) red[wv][0] = ss;
+#pragma unroll
+        for (int k = 0; k < kLsqMaxCols; k++) {
+            if (k < ntj) {
+                const T v = wave_sum_xor(acc[k]);
+                if (lane == 0) red[wv][1 + k] = v;
+            }
+        }
+        __syncthreads();
+        if (t <= ntj) {
+            T v = 0;
+            for (int w = 0; w < kLsqWaves; w++) v += red[w][t];
+            part[((int64_t)(j & 1) * groups + g) * kLsqRec + t] = v;
+        }
+        return;
+    }
+
+    // last launch (j = n): all-zero A -> B = 0, info = 0; else back substitution
+    bool zero = n == 0;
+    if (!zero) {
+        T x = 0;
+        for (int q = 0; q < groups; q++) x = fmax(x, amaxg[q]);
+        zero = x == T(0);
+    }
+    if (zero) {
+        for (int64_t i = (int64_t)g * kLsqThreads + t; i < m; i += step)
+            for (int r = 0; r < nrhs; r++) b[i + (int64_t)r * ldb] = 0;
+        if (g == 0 && t == 0) info[0] = 0;
+        return;
+    }
+    if (g != ((n - 1) / kLsqThreads) % groups) return;
+    __syncthreads();  // row n - 1's last update (one thread of this group) is seen by the others
+    __shared__ int s_info;
     if (t == 0) {
         int inf = 0;
         for (int i = 0; i < n; i++)
@@ -185,9 +233,17 @@ int gels(const tvlm_view* av, const tvlm_view* bv, int32_t* info, void* stream) 
     }
     const int64_t lda = n > 1 ? av->stride[1] : m, ldb = nrhs > 1 ? bv->stride[1] : m;
     TVLM_REQUIRE(lda >= m && ldb >= m, "gels: leading dimensions must be >= M");
-    hipLaunchKernelGGL(k_gels<T>, dim3(1), dim3(kLsqThreads), 0, as_stream(stream), static_cast<T*>(av->ptr), lda,
-                       (int)m, (int)n, static_cast<T*>(bv->ptr), ldb, (int)nrhs, info);
-    TVLM_LAUNCHED();
+    int64_t groups = (m + kLsqThreads - 1) / kLsqThreads;
+    groups = groups < 1 ? 1 : groups > kLsqMaxGroups ? kLsqMaxGroups : groups;
+    hipStream_t st = as_stream(stream);
+    void* ws = nullptr;
+    TVLM_TRY(device_workspace(sizeof(T) * (size_t)(2 * groups * kLsqRec + 2 * kLsqRec + groups), &ws, kWsLsq, st));
+    for (int j = 0; j <= n; j++) {
+        hipLaunchKernelGGL(k_qr_step<T>, dim3((unsigned)groups), dim3(kLsqThreads), 0, st, static_cast<T*>(av->ptr),
+                           lda, (int)m, (int)n, static_cast<T*>(bv->ptr), ldb, (int)nrhs, static_cast<T*>(ws),
+                           (int)groups, j, info);
+        TVLM_LAUNCHED();
+    }
     return TVLM_OK;
 }
 
