@@ -397,33 +397,43 @@ __device__ __forceinline__ LatVertex lat_vertex(double cx, double cy, double cz,
     return f;
 }
 
-// segment a->b of strength g for the receiver (image = false) or its image (image = true)
-template <bool IMAGE>
-__device__ __forceinline__ void lat_segment(double& ax, double& ay, double& az, const LatVertex& a,
-                                            const LatVertex& b, double g) {
-    const double r1x = a.x, r1y = IMAGE ? a.ym : a.y, r1z = a.z, ri1 = IMAGE ? a.rim : a.ri;
-    const double r2x = b.x, r2y = IMAGE ? b.ym : b.y, r2z = b.z, ri2 = IMAGE ? b.rim : b.ri;
-    const double crx = r1y * r2z - r1z * r2y;
-    const double cry = r1z * r2x - r1x * r2z;
-    const double crz = r1x * r2y - r1y * r2x;
-    const double sq = crx * crx + cry * cry + crz * crz;
-    const double r0x = r1x - r2x, r0y = r1y - r2y, r0z = r1z - r2z;
-    const double d1 = r0x * r1x + r0y * r1y + r0z * r1z;
-    const double d2 = r0x * r2x + r0y * r2y + r0z * r2z;
-    // |r| < RCUT  <=>  1/|r| > 1/RCUT
-    constexpr double rimax = 1.0 / BS<double>::cut_norm;
-    const bool ok = (sq >= BS<double>::cut_cross) & (ri1 <= rimax) & (ri2 <= rimax);
-    const double t = (d1 * ri1 - d2 * ri2) * rcp_seeded(BS<double>::fourpi * sq) * g;
-    const double coeff = ok ? t : 0.0;
-    ax += coeff * crx;
-    ay += (IMAGE ? -1.0 : 1.0) * (coeff * cry);
-    az += coeff * crz;
-}
-
+// Segment a->b of strength g for the receiver and for its image in y = 0.  The two evaluations share
+// every term without a y component: r0 = b - a (taken as (c - a) - (c - b), whose y part is the same
+// for the image), the y component of r1 x r2 and its square, and the x and z parts of r0.r1 and r0.r2.
 __device__ __forceinline__ void lat_pair(double& ax, double& ay, double& az, const LatVertex& a, const LatVertex& b,
                                          double g) {
-    lat_segment<false>(ax, ay, az, a, b, g);
-    lat_segment<true>(ax, ay, az, a, b, g);
+    const double r0x = a.x - b.x, r0y = a.y - b.y, r0z = a.z - b.z;
+    const double cry = a.z * b.x - a.x * b.z;
+    const double cry2 = cry * cry;
+    const double d1xz = r0x * a.x + r0z * a.z;
+    const double d2xz = r0x * b.x + r0z * b.z;
+    constexpr double cut = BS<double>::cut_cross, fourpi = BS<double>::fourpi;
+    // |r| < RCUT  <=>  1/|r| > 1/RCUT
+    constexpr double rimax = 1.0 / BS<double>::cut_norm;
+    {
+        const double crx = a.y * b.z - a.z * b.y;
+        const double crz = a.x * b.y - a.y * b.x;
+        const double sq = crx * crx + crz * crz + cry2;
+        const double d1 = d1xz + r0y * a.y, d2 = d2xz + r0y * b.y;
+        const bool ok = (sq >= cut) & (a.ri <= rimax) & (b.ri <= rimax);
+        const double t = (d1 * a.ri - d2 * b.ri) * rcp_seeded(fourpi * sq) * g;
+        const double coeff = ok ? t : 0.0;
+        ax += coeff * crx;
+        ay += coeff * cry;
+        az += coeff * crz;
+    }
+    {
+        const double crx = a.ym * b.z - a.z * b.ym;
+        const double crz = a.x * b.ym - a.ym * b.x;
+        const double sq = crx * crx + crz * crz + cry2;
+        const double d1 = d1xz + r0y * a.ym, d2 = d2xz + r0y * b.ym;
+        const bool ok = (sq >= cut) & (a.rim <= rimax) & (b.rim <= rimax);
+        const double t = (d1 * a.rim - d2 * b.rim) * rcp_seeded(fourpi * sq) * g;
+        const double coeff = ok ? t : 0.0;
+        ax += coeff * crx;
+        ay -= coeff * cry;
+        az += coeff * crz;
+    }
 }
 
 constexpr int kLatChunkMax = 16;  // spans per wave
