@@ -397,9 +397,10 @@ __device__ __forceinline__ LatVertex lat_vertex(double cx, double cy, double cz,
     return f;
 }
 
-// Segment a->b of strength g for the receiver and for its image in y = 0.  The two evaluations share
-// every term without a y component: r0 = b - a (taken as (c - a) - (c - b), whose y part is the same
-// for the image), the y component of r1 x r2 and its square, and the x and z parts of r0.r1 and r0.r2.
+// Segment a->b of strength g for the receiver and for its image in y = 0, without the 1/(4 pi) factor
+// (lattice_chunk applies it once to its sum).  The two evaluations share every term without a y
+// component: r0 = b - a (taken as (c - a) - (c - b), whose y part is the same for the image), the y
+// component of r1 x r2 and its square, and the x and z parts of r0.r1 and r0.r2.
 __device__ __forceinline__ void lat_pair(double& ax, double& ay, double& az, const LatVertex& a, const LatVertex& b,
                                          double g) {
     const double r0x = a.x - b.x, r0y = a.y - b.y, r0z = a.z - b.z;
@@ -407,7 +408,7 @@ __device__ __forceinline__ void lat_pair(double& ax, double& ay, double& az, con
     const double cry2 = cry * cry;
     const double d1xz = r0x * a.x + r0z * a.z;
     const double d2xz = r0x * b.x + r0z * b.z;
-    constexpr double cut = BS<double>::cut_cross, fourpi = BS<double>::fourpi;
+    constexpr double cut = BS<double>::cut_cross;
     // |r| < RCUT  <=>  1/|r| > 1/RCUT
     constexpr double rimax = 1.0 / BS<double>::cut_norm;
     {
@@ -416,7 +417,7 @@ __device__ __forceinline__ void lat_pair(double& ax, double& ay, double& az, con
         const double sq = crx * crx + crz * crz + cry2;
         const double d1 = d1xz + r0y * a.y, d2 = d2xz + r0y * b.y;
         const bool ok = (sq >= cut) & (a.ri <= rimax) & (b.ri <= rimax);
-        const double t = (d1 * a.ri - d2 * b.ri) * rcp_seeded(fourpi * sq) * g;
+        const double t = (d1 * a.ri - d2 * b.ri) * rcp_seeded(sq) * g;
         const double coeff = ok ? t : 0.0;
         ax += coeff * crx;
         ay += coeff * cry;
@@ -428,7 +429,7 @@ __device__ __forceinline__ void lat_pair(double& ax, double& ay, double& az, con
         const double sq = crx * crx + crz * crz + cry2;
         const double d1 = d1xz + r0y * a.ym, d2 = d2xz + r0y * b.ym;
         const bool ok = (sq >= cut) & (a.rim <= rimax) & (b.rim <= rimax);
-        const double t = (d1 * a.rim - d2 * b.rim) * rcp_seeded(fourpi * sq) * g;
+        const double t = (d1 * a.rim - d2 * b.rim) * rcp_seeded(sq) * g;
         const double coeff = ok ? t : 0.0;
         ax += coeff * crx;
         ay -= coeff * cry;
@@ -443,11 +444,12 @@ constexpr int kLatChunkMax = 16;  // spans per wave
 // the receiver (cx, cy, cz): chordwise segments of each vertex column with the circulation
 // difference of the two ring columns beside it, spanwise segments of each vertex row with the
 // difference of the two ring rows, the previous vertex column's terms kept in registers; the tile's
-// right edge only when sb == swe.  Accumulates the induced velocity into (ax, ay, az).
+// right edge only when sb == swe.  Adds the induced velocity to (vx, vy, vz).
 template <int RH>
 __device__ __forceinline__ void lattice_chunk(const double* sverts, const double* sgam, int swe, int rhe, int vw,
-                                              int sa, int sb, double cx, double cy, double cz, double& ax, double& ay,
-                                              double& az) {
+                                              int sa, int sb, double cx, double cy, double cz, double& vx, double& vy,
+                                              double& vz) {
+    double ax = 0.0, ay = 0.0, az = 0.0;  // 4 pi x the induced velocity (lat_pair leaves the factor out)
     LatVertex prev[RH + 1], cur[RH + 1];
 #pragma unroll
     for (int q = 0; q <= RH; q++)
@@ -481,6 +483,10 @@ __device__ __forceinline__ void lattice_chunk(const double* sverts, const double
         for (int q = 0; q < RH; q++)
             if (q < rhe) lat_pair(ax, ay, az, prev[q], prev[q + 1], sgam[q * swe + swe - 1]);
     }
+    constexpr double inv_fourpi = 1.0 / BS<double>::fourpi;
+    vx += ax * inv_fourpi;
+    vy += ay * inv_fourpi;
+    vz += az * inv_fourpi;
 }
 
 // zs: simulations (blockIdx.z) of one launch find their receivers, wake and circulations z * zs
