@@ -131,6 +131,15 @@ int tvlm_fill_f64(int32_t location, double* ptr, int64_t stride, double value, i
     int tvlm_displace_wing_##S(int32_t n_surf, const tvlm_view* transforms,                        \
                                const tvlm_view* verts_wing, const tvlm_view* verts_wing_init,       \
                                void* stream);                                                       \
+    /* The wake geometry of a prescribed motion (no reference counterpart: one launch per surface  \
+     * for the per-step displace_wing + wake_shed loop of lib/libhbvlm3.cpp:306-319).  For every    \
+     * step i < steps, surface m's wake vertex row shape[1]-1-i becomes its wing trailing-edge row   \
+     * of verts_wing_init displaced by transforms[:, :, i*n_surf + m] ({4,4,>= steps*n_surf}): the  \
+     * values steps alternating displace_wing / wake_shed(i) calls leave there.  The wing vertices   \
+     * are not touched; other wake rows are left as they are. */                                    \
+    int tvlm_wake_geometry_##S(int32_t n_surf, const tvlm_view* transforms,                        \
+                               const tvlm_view* verts_wing_init, const tvlm_view* verts_wake,       \
+                               int32_t steps, void* stream);                                        \
     /* LU::factorize (vlm_backend.hpp:105/:112): in-place L\U with row pivots; ipiv_dev{N} is      \
      * 1-based like LAPACK; info_dev[0] = 0 or k (1-based) of the first exactly zero pivot.         \
      * a->shape[2] = B > 1 factors B independent matrices in one pass (matrix b at                  \

@@ -131,3 +131,52 @@ def test_cpp_anderson_matches_oracle(oracle, m):
                                               np.float32(1e-5), m)
     assert k32 < 100 and ko32 < 100
     assert np.max(np.abs(x32 - exact)) < 1e-4
+
+
+@pytest.mark.parametrize("m", [1, 3, 10])
+def test_python_anderson_matches_oracle(oracle, m):
+    """hb.AndersonSolver (device vectors, the HB driver's loop) on the same contraction: the oracle's
+    iteration count +-1 and the closed-form fixed point within 1e-10."""
+    import torch
+
+    from tinyvlm_amd import capi as H
+    from tinyvlm_amd.hb import AndersonSolver
+
+    H.init(0)
+    M, c = _anderson_problem()
+    exact = np.linalg.solve(np.eye(64) - M, c)
+    Md, cd = H.to_device(np.asfortranarray(M), torch.float64), H.to_device(c, torch.float64)
+
+    def f(x, y):
+        y.copy_(Md @ x + cd)
+
+    print(f"python anderson m={m}", flush=True)
+    s = AndersonSolver(64, m)
+    x = torch.zeros(64, dtype=torch.float64, device="cuda")
+    k = s.run(x, f, 100, 1e-12)
+    xo, ko = oracle.anderson_acceleration(np.zeros(64), lambda v: M @ v + c, 100, 1e-12, m)
+    assert abs(k - ko) <= 1, (k, ko)
+    assert np.max(np.abs(H.to_host(x) - exact)) < 1e-10
+
+
+def test_python_anderson_rank_deficient_history_raises():
+    """A history whose first column is zero and second is not (the map leaves the residual unchanged
+    once, then changes it) makes R(0,0) = 0: gels reports info = 1 and the solver raises."""
+    import torch
+
+    from tinyvlm_amd import capi as H
+    from tinyvlm_amd.hb import AndersonSolver
+
+    H.init(0)
+    c = torch.linspace(0.5, 1.5, 64, dtype=torch.float64, device="cuda")
+    calls = [0]
+
+    def f(x, y):
+        calls[0] += 1
+        # x0 = 0 -> c -> 2c with residual c twice (first history column 0); then f(2c) = 4c, residual
+        # 2c, second column c: the k = 2 least squares has A = [0, c]
+        y.copy_(x + c if calls[0] <= 2 else 0.5 * x + 3.0 * c)
+
+    s = AndersonSolver(64, 3)
+    with pytest.raises(H.TvlmError, match="rank-deficient"):
+        s.run(torch.zeros(64, dtype=torch.float64, device="cuda"), f, 100, 1e-12)

@@ -335,6 +335,14 @@ def displace_wing(transforms, verts_wing, verts_wing_init, stream=None):
                                              views(verts_wing_init), stream_ptr(stream)), "displace_wing")
 
 
+def wake_geometry(transforms, verts_wing_init, verts_wake, steps, stream=None):
+    """tvlm_wake_geometry_*: wake rows of `steps` prescribed steps in one launch per surface
+    (transforms {4, 4, steps * n_surf}, step-major)."""
+    check(_fn("wake_geometry", transforms)(ctypes.c_int32(len(verts_wing_init)), ctypes.byref(view(transforms)),
+                                           views(verts_wing_init), views(verts_wake), ctypes.c_int32(steps),
+                                           stream_ptr(stream)), "wake_geometry")
+
+
 def gamma_wake_from_coeffs(gamma_wake, gamma_coeffs, harmonics, tn, omega, dt, iteration, stream=None):
     R = _real(gamma_wake)
     check(_fn("gamma_wake_from_coeffs", gamma_wake)(ctypes.byref(view(gamma_wake)), ctypes.byref(view(gamma_coeffs)),

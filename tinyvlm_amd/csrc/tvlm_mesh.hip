@@ -263,6 +263,24 @@ __global__ void k_displace(const T* __restrict__ M, int64_t m0, int64_t m1, cons
     for (int r = 0; r < 4; r++) Y[p + r * ys2] = displace_row(M, m0, m1, r, x);
 }
 
+// Wake vertex rows of a whole prescribed motion: element e = (span vertex j, step i) of one surface
+// writes wake row nwv-1-i with that surface's trailing-edge vertex j of X displaced by the step's
+// transform M + (i n_surf + m) m2 -- the values k_displace then k_wake_shed leave there.
+template <typename T>
+__global__ void k_wake_geometry(const T* __restrict__ M, int64_t m0, int64_t m1, int64_t m2, int n_surf, int m,
+                                const T* __restrict__ X, int64_t xs2, int ns, int nc, T* __restrict__ W, int64_t ws2,
+                                int nwv, int steps) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= (int64_t)steps * (ns + 1)) return;
+    const int j = (int)(e % (ns + 1)), i = (int)(e / (ns + 1));
+    const T* Mi = M + ((int64_t)i * n_surf + m) * m2;
+    const int64_t p = j + (int64_t)nc * (ns + 1);
+    const T x[4] = {X[p], X[p + xs2], X[p + 2 * xs2], X[p + 3 * xs2]};
+    const int64_t w = j + (int64_t)(nwv - 1 - i) * (ns + 1);
+#pragma unroll
+    for (int r = 0; r < 4; r++) W[w + r * ws2] = displace_row(Mi, m0, m1, r, x);
+}
+
 // ---- gamma_wake_from_coeffs (vlm_backend_cpu.cpp:470-503) -----------------------------------------
 template <typename T>
 __global__ void k_gamma_wake_coeffs(T* __restrict__ gw, int64_t g1, const T* __restrict__ gc, int64_t c1, int ns,
@@ -670,6 +688,31 @@ int tvlm_fill_f64(int32_t location, double* ptr, int64_t stride, double value, i
             hipLaunchKernelGGL(k_displace<T>, blocks_for(nv), dim3(256), 0, as_stream(st), (const T*)tr[m].ptr, \
                                tr[m].stride[0], tr[m].stride[1], (const T*)vi[m].ptr, vi[m].stride[2],        \
                                (T*)vw[m].ptr, vw[m].stride[2], nv);                                           \
+            TVLM_LAUNCHED();                                                                                  \
+        }                                                                                                     \
+        return TVLM_OK;                                                                                       \
+    }                                                                                                         \
+    int tvlm_wake_geometry_##S(int32_t n_surf, const tvlm_view* tr, const tvlm_view* vi, const tvlm_view* wk,  \
+                               int32_t steps, void* st) {                                                     \
+        TVLM_REQUIRE(tr && vi && wk && n_surf >= 1 && n_surf <= TVLM_MAX_SURFACES && steps >= 0, "bad argument"); \
+        TVLM_REQUIRE(tr->ptr && tr->shape[0] == 4 && tr->shape[1] == 4 &&                                     \
+                         tr->shape[2] >= (int64_t)steps * n_surf,                                             \
+                     "transforms must be {4, 4, >= steps * n_surf}");                                         \
+        for (int m = 0; m < n_surf; m++) {                                                                    \
+            TVLM_REQUIRE(plane_contiguous(vi[m]) && plane_contiguous(wk[m]) && vi[m].shape[2] == 4 &&         \
+                             wk[m].shape[2] == 4 && wk[m].shape[0] == vi[m].shape[0] && vi[m].shape[1] >= 2,  \
+                         "verts_wing_init/verts_wake[%d] shape", m);                                          \
+            TVLM_REQUIRE(steps <= wk[m].shape[1], "%d steps outside the wake (%lld rows)", steps,             \
+                         (long long)wk[m].shape[1]);                                                          \
+        }                                                                                                     \
+        for (int m = 0; m < n_surf; m++) {                                                                    \
+            const int ns = (int)vi[m].shape[0] - 1, nc = (int)vi[m].shape[1] - 1;                             \
+            const int64_t cnt = (int64_t)steps * (ns + 1);                                                    \
+            if (cnt == 0) continue;                                                                           \
+            hipLaunchKernelGGL(k_wake_geometry<T>, blocks_for(cnt), dim3(256), 0, as_stream(st),              \
+                               (const T*)tr->ptr, tr->stride[0], tr->stride[1], tr->stride[2], n_surf, m,     \
+                               (const T*)vi[m].ptr, vi[m].stride[2], ns, nc, (T*)wk[m].ptr, wk[m].stride[2],  \
+                               (int)wk[m].shape[1], steps);                                                   \
             TVLM_LAUNCHED();                                                                                  \
         }                                                                                                     \
         return TVLM_OK;                                                                                       \

@@ -76,21 +76,37 @@ def build_kinematics(dyn_f, omega, v: Vars, harmonics: int):
 class AndersonSolver:
     """anderson_acceleration (vlm/src/vlm_solvers.cpp:6-115) on device vectors: the same sequence
     of BLAS copies/axpy/gemv and the LSQ solve (gels) of the C ABI; the residual norm is read back
-    once per iteration, the by-value contract of BLAS::norm."""
+    once per iteration, the by-value contract of BLAS::norm.  The least-squares info (which the
+    reference does not check) shares the norm's device buffer: the norm read of the next iteration
+    also returns it, so a rank-deficient history still raises, one map evaluation later, without a
+    second host read per iteration."""
 
     def __init__(self, n, m):
         import torch
 
         z = lambda *s: H.colmajor(s, torch.float64)  # noqa: E731
+        self.torch = torch
         self.n, self.m = n, m
         self.X, self.G, self.Gk = z(n, m), z(n, m), z(n, m)
         self.x_curr, self.x_new, self.g_curr, self.g_new, self.gamma = z(n), z(n), z(n), z(n), z(n)
-        self.info = torch.zeros(1, dtype=torch.int32, device="cuda")
-        self.scal = torch.zeros(1, dtype=torch.float64, device="cuda")
+        # [0]: the residual norm (nrm2), [1]: the last gels info (int32 in its first four bytes)
+        self.flags = torch.zeros(2, dtype=torch.float64, device="cuda")
+        self.scal = self.flags[0:1]
+        self.info = self.flags[1:2].view(torch.int32)[0:1]
+
+    def _read(self):
+        """(norm, info) from one host read of the flags buffer."""
+        v = self.flags.cpu()
+        return float(v[0]), int(v[1:2].view(self.torch.int32)[0])
+
+    @staticmethod
+    def _check(info):
+        if info != 0:
+            raise H.TvlmError(f"Anderson least squares: rank-deficient history (gels info {info})")
 
     def norm(self, x):
         H.nrm2(x, self.scal)
-        return float(self.scal.item())
+        return self._read()[0]
 
     def run(self, x0, f, max_iter=100, tol_res=1e-6):
         """f(x, y) writes the map's value at x into y.  x0 is overwritten with the fixed point;
@@ -111,14 +127,20 @@ class AndersonSolver:
         H.axpy(-1.0, g_curr, G[:, 0])
         g_curr.copy_(g_new)
         k = 1
-        while k < max_iter and self.norm(g_curr) > tol_res:
+        pending = False  # a gels whose info has not been read yet
+        while k < max_iter:
+            H.nrm2(g_curr, self.scal)
+            res, info = self._read()
+            if pending:
+                self._check(info)
+                pending = False
+            if not res > tol_res:
+                break
             m_k = min(m, k)
             gamma.copy_(g_curr)
             Gk[:, :m_k].copy_(G[:, :m_k])
             H.gels(Gk[:, :m_k], gamma, self.info)
-            info = int(self.info.item())
-            if info != 0:
-                raise H.TvlmError(f"Anderson least squares: rank-deficient history (gels info {info})")
+            pending = True
             x_new.copy_(x_curr)
             H.axpy(1.0, g_curr, x_new)
             H.gemv(-1.0, X[:, :m_k], gamma[:m_k], 1.0, x_new)
@@ -134,6 +156,8 @@ class AndersonSolver:
             H.axpy(-1.0, g_curr, Gb)
             g_curr.copy_(g_new)
             k += 1
+        if pending:
+            self._check(self._read()[1])
         x0.copy_(x_curr)
         return k
 
@@ -276,9 +300,10 @@ class HBVLM:
         times = np.arange(nsteps, dtype=np.float64) * dt
         T_all = np.stack([n.transform(times) for n in nodes], axis=2)  # (4, 4, 2, nsteps)
         T_dev = H.to_device(np.asfortranarray(T_all.reshape(4, 4, 2 * nsteps, order="F")), torch.float64)
-        for i in range(nsteps):
-            H.displace_wing([T_dev[:, :, 2 * i], T_dev[:, :, 2 * i + 1]], self.verts, self.verts_init)
-            H.wake_shed(self.verts, wake, i)
+        # (the per-step displace_wing + wake_shed pairs in one launch per surface; the wing is then
+        # left at the last step's pose, as after the loop)
+        H.wake_geometry(T_dev, self.verts_init, wake, nsteps)
+        H.displace_wing([T_dev[:, :, 2 * nsteps - 2], T_dev[:, :, 2 * nsteps - 1]], self.verts, self.verts_init)
         # orthogonal DFT matrix of the K times and its time derivative (lib/libhbvlm3.cpp:321-346)
         s0 = 1.0 / math.sqrt(float(Kc))
         s2 = math.sqrt(2.0 / float(Kc))
