@@ -60,3 +60,24 @@ def test_kinematics_angular_velocity_kat():
     th = 0.5 * float(F32(3.141593)) * math.sin(om * t)
     thd = 0.5 * float(F32(3.141593)) * om * math.cos(om * t)
     assert np.allclose(v, [-u - math.sin(th) * thd, 0.0, -math.cos(th) * thd], atol=1e-12)
+
+
+def test_hb_poses_over_an_array_of_times_equal_one_time_at_a_time():
+    """HBVLM._poses evaluates the HB kinematics tree (free stream, heave, pitch, flap) over all the
+    collocation times in one pass of the dual-number arithmetic on numpy arrays; each time's
+    transform and dT/dt equal the scalar evaluation of KinematicNode.transform_dual at that time
+    (elementwise the same operations; 1e-15 allows a vectorised sin/cos that differs in the last
+    bit)."""
+    from tinyvlm_amd.hb import HBVLM, Vars, build_kinematics
+
+    H = 5
+    X = np.zeros((3, 2 * H + 1))
+    X[0, 1], X[1, 2], X[2, 3], X[0, 4] = 0.1, math.radians(1.0), 0.02, 0.01
+    fs, nodes = build_kinematics(X, 0.8, Vars(6.0), H)
+    times = [0.7 + 0.013 * i for i in range(2 * H + 1)]
+    poses = HBVLM._poses(nodes + [fs], times)
+    for i, t in enumerate(times):
+        for m, n in enumerate(nodes + [fs]):
+            d = n.transform_dual(t)
+            assert np.max(np.abs(poses[i][m][0] - d.value())) <= 1e-15
+            assert np.max(np.abs(poses[i][m][1] - d.grad())) <= 1e-15

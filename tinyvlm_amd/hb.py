@@ -255,15 +255,20 @@ class HBVLM:
         self.stream.synchronize()
         return out
 
-    def _pose(self, nodes, t, verts, colloc, normals, areas):
-        """displace_wing to the pose at t and the panel metrics there; returns the dual transform
-        of each surface (value and dT/dt)."""
+    @staticmethod
+    def _poses(nodes, times):
+        """The dual transform of every node at every time of `times`, evaluated over the array of
+        times in one pass: [[(T, dT/dt) per node] per time], 4 x 4 numpy arrays."""
+        tt = np.asarray(times, dtype=np.float64)
+        vg = [(d.value(), d.grad()) for d in (n.transform_dual(tt) for n in nodes)]
+        return [[(v[:, :, i], g[:, :, i]) for v, g in vg] for i in range(len(tt))]
+
+    def _pose(self, pose, verts, colloc, normals, areas):
+        """displace_wing to `pose` (one (T, dT/dt) per surface) and the panel metrics there."""
         torch = self.torch
-        duals = [n.transform_dual(t) for n in nodes]
-        tr = [H.to_device(np.asfortranarray(d.value()), torch.float64) for d in duals]
+        tr = [H.to_device(np.asfortranarray(T), torch.float64) for T, _ in pose]
         H.displace_wing(tr, verts, self.verts_init)
         H.mesh_metrics(0.0, verts, colloc, normals, areas)
-        return duals
 
     def solve(self, t_start, omega, v: Vars, fs_node, nodes, force_t):
         """HBVLM::run (lib/libhbvlm3.cpp:262-552)."""
@@ -322,16 +327,17 @@ class HBVLM:
         ddft_d = H.to_device(np.asfortranarray(ddft), torch.float64)
         # circulation-independent part of each collocation time: pose, metrics, kinematic rhs
         steps = []
-        self.rhs_vel.zero_()
         for s in range(Kc):
             tf = t_start + period * float(s) / float(Kc)
             ts = int(round(tf / dt))
-            t = float(ts) * dt
+            steps.append((ts, float(ts) * dt))
+        poses = self._poses(nodes, [t for _, t in steps])
+        self.rhs_vel.zero_()
+        for s in range(Kc):
             I = self.inst[s]
-            duals = self._pose(nodes, t, I["verts"], I["colloc"], I["normals"], I["areas"])
-            H.kinematic_velocities([d.grad() for d in duals], self.colloc0, self.vel)
+            self._pose(poses[s], I["verts"], I["colloc"], I["normals"], I["areas"])
+            H.kinematic_velocities([g for _, g in poses[s]], self.colloc0, self.vel)
             H.rhs_assemble_velocities(self.rhs_vel[:, s], I["normals"], self.vel)
-            steps.append((ts, t))
         n0 = self.dims[0][0] * self.dims[0][1]
         ns1, nc1 = self.dims[1]
         lifting = [False, True]
@@ -386,13 +392,15 @@ class HBVLM:
         out = force_t
         verts = self.verts
         colloc, normals, areas = self.inst[0]["colloc"], self.inst[0]["normals"], self.inst[0]["areas"]
+        times = [float(int(round(period * float(s) / float(Kc) / dt))) * dt for s in range(Kc)]
+        poses = self._poses(nodes, times)
+        fs_all = self._poses([fs_node], times)
         for s in range(Kc):
-            tf = period * float(s) / float(Kc)
-            ts = int(round(tf / dt))
-            t = float(ts) * dt
-            duals = self._pose(nodes, t, verts, colloc, normals, areas)
-            H.kinematic_velocities([d.grad() for d in duals], self.colloc0, self.vel)
-            fs = -fs_node.linear_velocity(t, (0.0, 0.0, 0.0))
+            pose = poses[s]
+            self._pose(pose, verts, colloc, normals, areas)
+            H.kinematic_velocities([g for _, g in pose], self.colloc0, self.vel)
+            g_fs = fs_all[s][0][1]  # KinematicNode.linear_velocity at the origin
+            fs = -(g_fs[:3, :3] @ np.zeros(3) + g_fs[:3, 3])
             g0 = self.gamma_wing[:n0, s].view(nc0, ns0).t()
             g1 = self.gamma_wing[n0:, s].view(nc1, ns1).t()
             self.gd[0].copy_(g0)
@@ -406,7 +414,7 @@ class HBVLM:
                 H.axpy(-1.0, g1[:, : nc1 - 1], self.gd[1][:, 1:])
             for m in range(2):
                 H.forces_unsteady(verts[m], self.gd[m], self.dg[m], self.vel[m], areas[m], normals[m], self.forces[m])
-            T0, T1 = duals[0].value(), duals[1].value()
+            T0, T1 = pose[0][0], pose[1][0]
             ref_a = (T0 @ np.array([1.0 + v.a, 0.0, 0.0, 1.0]))[:3]
             ref_b = (T1 @ np.array([1.0 + v.c, 0.0, 0.0, 1.0]))[:3]
             out[0, s] = self._cl_multibody(fs, rho, areas)
