@@ -77,12 +77,13 @@ COPY_PROBE_BYTES = 1 << 30
 # "*_sims64": one plan of 64 simulations (the bench step), each launch covering all 64, over
 # `bench.py --probe` and `bench.py --probe --exact-wake` (one batch of 64).
 # The executed flops per pair were measured again after a segment and its image came to share their
-# y-free terms (the traffic passes precede that change, which moved no memory access).
+# y-free terms and the 1/(4 pi) factor moved out of the segment loop (the traffic passes precede
+# those changes, which moved no memory access).
 WAKE_COUNTERS = {
-    "exact": {"traffic_bytes": 3008144, "algorithmic_bytes": 349384, "executed_flops_per_pair": 242.9},
-    "far": {"traffic_bytes": 2465922, "algorithmic_bytes": 188327, "executed_flops_per_pair": 290.0},
-    "far_sims64": {"traffic_bytes": 121996697, "algorithmic_bytes": 12052939, "executed_flops_per_pair": 230.9},
-    "exact_sims64": {"traffic_bytes": 174869909, "algorithmic_bytes": 22360576, "executed_flops_per_pair": 227.6},
+    "exact": {"traffic_bytes": 3008144, "algorithmic_bytes": 349384, "executed_flops_per_pair": 237.9},
+    "far": {"traffic_bytes": 2465922, "algorithmic_bytes": 188327, "executed_flops_per_pair": 284.9},
+    "far_sims64": {"traffic_bytes": 121996697, "algorithmic_bytes": 12052939, "executed_flops_per_pair": 226.0},
+    "exact_sims64": {"traffic_bytes": 174869909, "algorithmic_bytes": 22360576, "executed_flops_per_pair": 222.7},
 }
 NS, NC, HALF_SPAN, CHORD = 40, 10, 2.0, 1.0
 T_STEPS = 501  # UVLM::run loops t_steps-1 times -> 500 time steps
