@@ -491,8 +491,9 @@ __device__ __forceinline__ void lattice_chunk(const double* sverts, const double
 
 // zs: simulations (blockIdx.z) of one launch find their receivers, wake and circulations z * zs
 // elements after simulation 0's (the tables' pointers) and write their partials z * wzs after.
-// At least 4 waves per SIMD: the 4-row form needs 130 registers unhinted (3 waves) and fits 128
-// with the hint, without scratch (the same instructions, allocated tighter).
+// At least 4 waves per SIMD: the hint once held the 4-row form at 128 registers when it needed 130
+// unhinted (3 waves); since a segment and its image share their y-free terms it takes 126 either way,
+// and the hint keeps a later change from dropping a wave unnoticed.
 template <typename TS, int RH>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_wake_lattice(TS* __restrict__ partial, RecvTable<TS> rt,
                                                            WakeTable<TS> wt, int64_t zs = 0, int64_t wzs = 0) {
