@@ -220,7 +220,6 @@ class HBVLM:
         self.forces = [z(ns, nc, 3) for ns, nc in self.dims]
         self.ipiv = torch.zeros(N, dtype=torch.int32, device="cuda")
         self.info = torch.zeros(1, dtype=torch.int32, device="cuda")
-        self.scal_d = torch.zeros(16, dtype=torch.float64, device="cuda")
         self.anderson = AndersonSolver(N * Kc, 10)
         # per collocation time: pose and panel metrics
         self.inst = [{
@@ -390,14 +389,24 @@ class HBVLM:
         H.gemm(1.0, self.gamma_coeffs, ddft_d, 0.0, self.dgamma_wing, trans_b=True)
         (ns0, nc0) = self.dims[0]
         out = force_t
-        verts = self.verts
-        colloc, normals, areas = self.inst[0]["colloc"], self.inst[0]["normals"], self.inst[0]["areas"]
         times = [float(int(round(period * float(s) / float(Kc) / dt))) * dt for s in range(Kc)]
         poses = self._poses(nodes, times)
         fs_all = self._poses([fs_node], times)
+        # Two passes over the K times, the poses kept in the instance buffers (free once the fixed
+        # point is found) and every area, MAC and coefficient in a device row per time (area0, mac0,
+        # area1, mac1, CL0, CL1, CM0, CM1, flap CM): two host reads in all, the same kernels and host
+        # arithmetic as coeff_cl_multibody / coeff_cm_multibody one time after another.
+        coef = torch.zeros(Kc, 16, dtype=torch.float64, device="cuda")
         for s in range(Kc):
-            pose = poses[s]
-            self._pose(pose, verts, colloc, normals, areas)
+            I = self.inst[s]
+            self._pose(poses[s], I["verts"], I["colloc"], I["normals"], I["areas"])
+            for m in range(2):
+                H.sum_(I["areas"][m], coef[s, 2 * m: 2 * m + 1])
+                H.mesh_mac(I["verts"][m], I["areas"][m], coef[s, 2 * m + 1: 2 * m + 2])
+        am = coef[:, :4].tolist()
+        for s in range(Kc):
+            pose, I = poses[s], self.inst[s]
+            verts, normals, areas = I["verts"], I["normals"], I["areas"]
             H.kinematic_velocities([g for _, g in pose], self.colloc0, self.vel)
             g_fs = fs_all[s][0][1]  # KinematicNode.linear_velocity at the origin
             fs = -(g_fs[:3, :3] @ np.zeros(3) + g_fs[:3, 3])
@@ -417,40 +426,45 @@ class HBVLM:
             T0, T1 = pose[0][0], pose[1][0]
             ref_a = (T0 @ np.array([1.0 + v.a, 0.0, 0.0, 1.0]))[:3]
             ref_b = (T1 @ np.array([1.0 + v.c, 0.0, 0.0, 1.0]))[:3]
-            out[0, s] = self._cl_multibody(fs, rho, areas)
-            out[1, s] = self._cm_multibody(ref_a, fs, rho, verts, areas)[1]
-            area1, mac1 = self._area_mac(verts[1], areas[1])
-            H.coeff_cm(self.forces[1], verts[1], ref_b, fs, 1.0, area1, mac1, self.scal_d[8:11])
-            out[2, s] = float(self.scal_d[9].item())
+            la = self._lift_axis(fs)
+            for m in range(2):
+                area, mac = am[s][2 * m], am[s][2 * m + 1]
+                H.coeff_cl(self.forces[m], la, fs, rho, area, coef[s, 4 + m: 5 + m])
+                H.coeff_cm(self.forces[m], verts[m], ref_a, fs, rho, area, mac, coef[s, 6 + 3 * m: 9 + 3 * m])
+            H.coeff_cm(self.forces[1], verts[1], ref_b, fs, 1.0, am[s][2], am[s][3], coef[s, 12:15])
+        co = coef.tolist()
+        for s in range(Kc):
+            out[0, s] = self._cl_multibody(am[s], co[s])
+            out[1, s] = self._cm_multibody(am[s], co[s])[1]
+            out[2, s] = co[s][13]
         return out
 
-    def _area_mac(self, verts, areas):
-        H.sum_(areas, self.scal_d[0:1])
-        H.mesh_mac(verts, areas, self.scal_d[1:2])
-        a, mac = self.scal_d[0:2].tolist()
-        return a, mac
-
-    def _cl_multibody(self, fs, rho, areas):
-        """coeff_cl_multibody (vlm/src/vlm_backend.cpp:71-98)."""
+    @staticmethod
+    def _lift_axis(fs):
         la = np.cross(np.asarray(fs, dtype=np.float64), np.array([0.0, 1.0, 0.0]))
-        la = la / math.sqrt(float(la[0] * la[0] + la[1] * la[1] + la[2] * la[2]))
+        return la / math.sqrt(float(la[0] * la[0] + la[1] * la[1] + la[2] * la[2]))
+
+    @staticmethod
+    def _cl_multibody(am, co):
+        """coeff_cl_multibody (vlm/src/vlm_backend.cpp:71-98) from the surfaces' areas (am[0], am[2])
+        and CL (co[4], co[5])."""
         cl = total = 0.0
         for m in range(2):
-            H.sum_(areas[m], self.scal_d[0:1])
-            area = float(self.scal_d[0].item())
-            H.coeff_cl(self.forces[m], la, fs, rho, area, self.scal_d[2:3])
-            cl += float(self.scal_d[2].item()) * area
+            area = am[2 * m]
+            cl += co[4 + m] * area
             total += area
         return cl / total
 
-    def _cm_multibody(self, ref, fs, rho, verts, areas):
-        """coeff_cm_multibody (vlm/src/vlm_backend.cpp:100-134)."""
+    @staticmethod
+    def _cm_multibody(am, co):
+        """coeff_cm_multibody (vlm/src/vlm_backend.cpp:100-134) from the surfaces' areas and MACs and
+        CM vectors (co[6:9], co[9:12])."""
         cm = np.zeros(3)
         ta = tm = 0.0
         for m in range(2):
-            area, mac = self._area_mac(verts[m], areas[m])
-            H.coeff_cm(self.forces[m], verts[m], ref, fs, rho, area, mac, self.scal_d[4:7])
-            cm += np.array(self.scal_d[4:7].tolist()) * area * mac
+            area, mac = am[2 * m], am[2 * m + 1]
+            cm += np.array(co[6 + 3 * m: 9 + 3 * m]) * area * mac
             ta += area
             tm += mac
         return cm / (ta * tm)
+
