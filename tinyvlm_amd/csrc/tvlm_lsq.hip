@@ -16,7 +16,7 @@
 //
 // The rows are spread over up to 256 work-groups of 256 threads (row i in group (i / 256) % groups),
 // with one launch per column: launch j finishes reflector j - 1 and prepares reflector j.
-//  - From the previous launch's per-group partial sums (fixed order over the groups, the same in every
+//  - From the previous launch's per-group partial sums (summed in a fixed order, the same in every
 //    group), every group forms reflector j - 1's tau, beta and the scale of its vector, and
 //    w_k = tau (C(j-1,k) + scal sum_{i>j-1} A(i,j-1) C(i,k)) for each target column k (the columns
 //    right of j - 1, then the B columns); the row j - 1 values come from a head record that the
@@ -82,10 +82,22 @@ __global__ __launch_bounds__(kLsqThreads) void k_qr_step(T* __restrict__ a, int6
     if (j > 0) {
         const T* pp = part + (int64_t)(c & 1) * groups * kLsqRec;
         const T* hp = head + (c & 1) * kLsqRec;
-        if (t <= ntc) {
-            T v = 0;
-            for (int q = 0; q < groups; q++) v += pp[(int64_t)q * kLsqRec + t];
-            sum_prev[t] = v;
+        // column sums of the groups' records: wave w takes columns w, w + 4, ...; lane l adds groups
+        // l, l + 64, l + 128, l + 192 in that order, then a butterfly (the same order in every group;
+        // the loads of a wave's columns are all issued before any is added)
+#pragma unroll
+        for (int k = 0; k < kLsqRec / kLsqWaves + 1; k++) {
+            const int col = wv + k * kLsqWaves;
+            if (col <= ntc) {
+                T v = 0;
+#pragma unroll
+                for (int r = 0; r < kLsqMaxGroups / 64; r++) {
+                    const int q = lane + 64 * r;
+                    if (q < groups) v += pp[(int64_t)q * kLsqRec + col];
+                }
+                v = wave_sum_xor(v);
+                if (lane == 0) sum_prev[col] = v;
+            }
         }
         __syncthreads();
         if (t == 0) {
@@ -179,10 +191,14 @@ __global__ __launch_bounds__(kLsqThreads) void k_qr_step(T* __restrict__ a, int6
 
     // last launch (j = n): all-zero A -> B = 0, info = 0; else back substitution
     bool zero = n == 0;
-    if (!zero) {
+    if (!zero) {  // every wave forms the same maximum over the groups' records
         T x = 0;
-        for (int q = 0; q < groups; q++) x = fmax(x, amaxg[q]);
-        zero = x == T(0);
+#pragma unroll
+        for (int r = 0; r < kLsqMaxGroups / 64; r++) {
+            const int q = lane + 64 * r;
+            if (q < groups) x = fmax(x, amaxg[q]);
+        }
+        zero = wave_max_xor(x) == T(0);
     }
     if (zero) {
         for (int64_t i = (int64_t)g * kLsqThreads + t; i < m; i += step)
