@@ -575,3 +575,14 @@ def test_lu_coop_single_exchange_attempt(H, torch, prec, n, batch):
         else:
             assert rel(H.to_host(a[:, :, b]), lu_ref) < 1e-4
             assert res < 4 * n * np.finfo(np.float32).eps
+
+
+def test_current_stream_handle_matches_torch(H, torch):
+    """capi.current_stream_handle (the launches' default stream) is torch's current stream, outside
+    and inside a stream context."""
+    assert H.current_stream_handle() == torch.cuda.current_stream().cuda_stream
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        assert H.current_stream_handle() == s.cuda_stream == torch.cuda.current_stream().cuda_stream
+    assert H.current_stream_handle() == torch.cuda.current_stream().cuda_stream
+    assert H.current_stream_handle() != s.cuda_stream

@@ -131,14 +131,25 @@ def views(ts) -> ctypes.Array:
     return arr
 
 
+def current_stream_handle() -> int:
+    """hipStream_t of torch's current stream on the current device, as torch.cuda.current_stream()
+    .cuda_stream gives it, through torch's raw-stream lookup where it exists (the public call costs
+    ~10 us of Python per launch of a host-driven loop)."""
+    torch = _torch()
+    C = torch._C
+    if hasattr(C, "_cuda_getCurrentRawStream") and hasattr(C, "_cuda_getDevice"):
+        return int(C._cuda_getCurrentRawStream(C._cuda_getDevice()))
+    return torch.cuda.current_stream().cuda_stream
+
+
 def stream_ptr(stream=None):
     """hipStream_t of a torch stream, of a raw handle (int, e.g. from tvlm_stream_create) or of
     torch's current stream."""
     if isinstance(stream, int):
         return ctypes.c_void_p(stream)
-    torch = _torch()
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return ctypes.c_void_p(s.cuda_stream)
+    if stream is None:
+        return ctypes.c_void_p(current_stream_handle())
+    return ctypes.c_void_p(stream.cuda_stream)
 
 
 def _real(t):
