@@ -193,6 +193,13 @@ class UVLM2DOF:
         offs = np.cumsum([0] + [ns * nc for ns, nc in dims])
         gam = [rhs[offs[m]: offs[m + 1]].view(nc, ns).t() for m, (ns, nc) in enumerate(dims)]
         inv_dt = float(T(1.0) / dt_nd)
+        # per Newton iteration: the section transform goes up through pinned host memory without
+        # blocking, and the surfaces' areas and MACs (host arguments of the coefficients) come back
+        # through pinned memory behind an event while the solve runs: one full wait per iteration
+        Twd = z(4, 4)
+        tw_host = torch.empty((4, 4), dtype=tdt, pin_memory=True).t()  # column-major like Twd
+        am_host = torch.empty(2 * len(dims), dtype=tdt, pin_memory=True)
+        am_ready = torch.cuda.Event()
 
         def solve(dT, it):
             H.kinematic_velocities([dT] * len(dims), colloc_h, vel)
@@ -250,15 +257,22 @@ class UVLM2DOF:
                 Tw, dTw = section_transform(tt, float(u[0, i + 1]), float(v[0, i + 1]), float(u[1, i + 1]),
                                             float(v[1, i + 1]), a_h)
                 ref = (Tw @ np.array([1.0 + a_h, 0.0, 0.0, 1.0]))[:3]
-                Twd = H.to_device(np.asfortranarray(Tw), tdt)
+                tw_host.copy_(torch.from_numpy(np.asarray(Tw)))
+                Twd.copy_(tw_host, non_blocking=True)
                 H.displace_wing([Twd] * len(dims), verts, verts_init)
                 H.mesh_metrics(0.0, verts, colloc, normals, areas)
+                for m in range(len(dims)):
+                    H.sum_(areas[m], scal[2 * m: 2 * m + 1])
+                    H.mesh_mac(verts[m], areas[m], scal[2 * m + 1: 2 * m + 2])
+                am_host.copy_(scal[: 2 * len(dims)], non_blocking=True)
+                am_ready.record()
                 H.wake_shed(verts, wake, i + 1)
                 solve(dTw, i + 1)
                 gamma_post(i + 1, True)
                 for m in range(len(dims)):
                     H.forces_unsteady(verts[m], gdelta[m], gdt[m], vel[m], areas[m], normals[m], forces[m])
-                cl, cmy = self._coefficients(forces, verts, areas, ref, fs, la, vars.rho, scal)
+                am_ready.synchronize()
+                cl, cmy = self._coefficients(forces, verts, ref, fs, la, vars.rho, scal, am_host.tolist())
                 F[0, i + 1] = T(-cl / (float(PI_F) * vars.mu))
                 F[1, i + 1] = T(2.0 * cmy / (float(PI_F) * vars.mu * vars.r_a ** 2))
                 dF = np.array([F[0, i + 1] - F[0, i] - kh * du[0],
@@ -276,14 +290,11 @@ class UVLM2DOF:
                 "t_steps": t_steps}
 
     @staticmethod
-    def _coefficients(forces, verts, areas, ref, fs, la, rho, scal):
-        """coeff_cl_multibody / coeff_cm_multibody (vlm/src/vlm_backend.cpp:71-135).  The areas and
-        MACs of all surfaces are read back together, then all CL/CM together: two host reads."""
+    def _coefficients(forces, verts, ref, fs, la, rho, scal, am):
+        """coeff_cl_multibody / coeff_cm_multibody (vlm/src/vlm_backend.cpp:71-135) from the surfaces'
+        areas and MACs am = [area0, mac0, area1, mac1, ...] (Backend::sum and mesh_mac of the same
+        pose): all CL/CM launched together, one host read."""
         n = len(forces)
-        for m in range(n):
-            H.sum_(areas[m], scal[2 * m: 2 * m + 1])
-            H.mesh_mac(verts[m], areas[m], scal[2 * m + 1: 2 * m + 2])
-        am = scal[: 2 * n].tolist()
         base = 2 * n
         for m in range(n):
             area, mac = am[2 * m], am[2 * m + 1]
