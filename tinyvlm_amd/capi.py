@@ -12,6 +12,7 @@ a reference shape ``(ns, nc, 3)`` is a torch tensor with strides ``(1, ns, ns*nc
 from __future__ import annotations
 
 import ctypes
+import weakref
 import os
 import re
 
@@ -109,13 +110,46 @@ def to_host(t) -> np.ndarray:
     return np.asfortranarray(a)
 
 
+# View of a tensor, kept per tensor object (weakly) while its address, shape and strides stay the
+# same: the host-driven loops pass the same tensors at every launch, and building the ctypes
+# structure costs several microseconds.  Callers only read a View (by reference during a call, or
+# copied into an array or a struct), never change it.
+# Keyed by id(t) with a weak reference to t (tensors compare elementwise, so they cannot be
+# dictionary keys themselves); the reference's callback drops the entry when t is freed.
+_VIEW_CACHE: dict = {}
+
+
 def view(t) -> View:
+    key = (t.data_ptr(), t.shape, t.stride())
+    tid = id(t)
+    hit = _VIEW_CACHE.get(tid)
+    if hit is not None and hit[0]() is t and hit[1] == key:
+        return hit[2]
+    v = _make_view(t)
+    try:
+        ref = weakref.ref(t, lambda _r, k=tid: _VIEW_CACHE.pop(k, None))
+    except TypeError:  # not weakly referenceable: no cache
+        return v
+    _VIEW_CACHE[tid] = (ref, key, v)
+    return v
+
+
+def _make_view(t) -> View:
+    nd = t.dim()
+    s, st = t.shape, t.stride()
+    # unused trailing dimensions: shape 1, stride = extent product (harmless for shape-1 dims)
+    if nd == 3:
+        return View(t.data_ptr(), (s[0], s[1], s[2]), (st[0], st[1], st[2]))
+    if nd == 2:
+        return View(t.data_ptr(), (s[0], s[1], 1), (st[0], st[1], st[1] * s[1]))
+    if nd == 1:
+        e = st[0] * s[0]
+        return View(t.data_ptr(), (s[0], 1, 1), (st[0], e, e))
     v = View()
     v.ptr = t.data_ptr()
     shp = list(t.shape) + [1] * (3 - t.dim())
     st = list(t.stride()) + [0] * (3 - t.dim())
     if t.dim() >= 1:
-        # unused trailing strides: extent product, harmless for shape-1 dims
         for d in range(t.dim(), 3):
             st[d] = st[d - 1] * shp[d - 1] if d > 0 else 1
     for k in range(3):
