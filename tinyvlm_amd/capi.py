@@ -165,15 +165,22 @@ def views(ts) -> ctypes.Array:
     return arr
 
 
+_RAW_STREAM = None  # (getCurrentRawStream, getDevice) of torch, or False where torch lacks them
+
+
 def current_stream_handle() -> int:
     """hipStream_t of torch's current stream on the current device, as torch.cuda.current_stream()
     .cuda_stream gives it, through torch's raw-stream lookup where it exists (the public call costs
     ~10 us of Python per launch of a host-driven loop)."""
-    torch = _torch()
-    C = torch._C
-    if hasattr(C, "_cuda_getCurrentRawStream") and hasattr(C, "_cuda_getDevice"):
-        return int(C._cuda_getCurrentRawStream(C._cuda_getDevice()))
-    return torch.cuda.current_stream().cuda_stream
+    global _RAW_STREAM
+    if _RAW_STREAM is None:
+        C = _torch()._C
+        ok = hasattr(C, "_cuda_getCurrentRawStream") and hasattr(C, "_cuda_getDevice")
+        _RAW_STREAM = (C._cuda_getCurrentRawStream, C._cuda_getDevice) if ok else False
+    if _RAW_STREAM:
+        raw, dev = _RAW_STREAM
+        return int(raw(dev()))
+    return _torch().cuda.current_stream().cuda_stream
 
 
 def stream_ptr(stream=None):
@@ -200,8 +207,14 @@ def _sfx(t):
     raise TvlmError(f"unsupported dtype {t.dtype}")
 
 
+_FNS: dict = {}  # (entry point, dtype) -> the library function
+
+
 def _fn(name, t):
-    return getattr(lib(), f"tvlm_{name}_{_sfx(t)}")
+    f = _FNS.get((name, t.dtype))
+    if f is None:
+        f = _FNS[(name, t.dtype)] = getattr(lib(), f"tvlm_{name}_{_sfx(t)}")
+    return f
 
 
 def _vec3(t, vals):
