@@ -200,6 +200,14 @@ class UVLM2DOF:
         tw_host = torch.empty((4, 4), dtype=tdt, pin_memory=True).t()  # column-major like Twd
         am_host = torch.empty(2 * len(dims), dtype=tdt, pin_memory=True)
         am_ready = torch.cuda.Event()
+        scal_slices: dict = {}
+
+        def S(a, b):
+            """scal[a:b], the same tensor at every call (its view is built once)."""
+            t = scal_slices.get((a, b))
+            if t is None:
+                t = scal_slices[(a, b)] = scal[a:b]
+            return t
 
         def solve(dT, it):
             H.kinematic_velocities([dT] * len(dims), colloc_h, vel)
@@ -214,20 +222,27 @@ class UVLM2DOF:
                                               [w[:, r0: r0 + wake_rows + 1] for w in wake], self.lifting, wake_rows)
             H.getrs(lhs, ipiv, rhs)
 
+        # the slices gamma_post works on, made once (the same tensors at every call keep their views)
+        te = [gam[m][:, nc - 1] for m, (ns, nc) in enumerate(dims)]
+        g_head = [gam[m][:, : nc - 1] if nc > 1 else None for m, (ns, nc) in enumerate(dims)]
+        gd_tail = [gdelta[m][:, 1:] if nc > 1 else None for m, (ns, nc) in enumerate(dims)]
+        gd_first = [gdelta[m][:, 0] for m in range(len(dims))]
+        gdt_flat = [gdt[m].t().reshape(-1) for m in range(len(dims))]
+
         def gamma_post(it, with_dt):
             # update_wake_and_gamma (tests/uvlm_2dof.cpp:289-331, 712-755)
             for m, (ns, nc) in enumerate(dims):
                 g = gam[m]
-                gamma_wake[m][:, t_steps - 2 - it].copy_(g[:, nc - 1])
+                gamma_wake[m][:, t_steps - 2 - it].copy_(te[m])
                 gdelta[m].copy_(g)
                 if m == 1:  # the flap's first row continues the wing's trailing edge
-                    H.axpy(-1.0, gam[0][:, dims[0][1] - 1], gdelta[m][:, 0])
+                    H.axpy(-1.0, te[0], gd_first[m])
                 if nc > 1:
-                    H.axpy(-1.0, g[:, : nc - 1], gdelta[m][:, 1:])
+                    H.axpy(-1.0, g_head[m], gd_tail[m])
                 if with_dt:
                     gdt[m].copy_(g)
                     H.axpy(-1.0, gprev[m], gdt[m])
-                    H.scal(inv_dt, gdt[m].t().reshape(-1))
+                    H.scal(inv_dt, gdt_flat[m])
 
         # t = 0 (tests/uvlm_2dof.cpp:547-559): freestream only
         H.wake_shed(verts, wake, 0)
@@ -262,9 +277,9 @@ class UVLM2DOF:
                 H.displace_wing([Twd] * len(dims), verts, verts_init)
                 H.mesh_metrics(0.0, verts, colloc, normals, areas)
                 for m in range(len(dims)):
-                    H.sum_(areas[m], scal[2 * m: 2 * m + 1])
-                    H.mesh_mac(verts[m], areas[m], scal[2 * m + 1: 2 * m + 2])
-                am_host.copy_(scal[: 2 * len(dims)], non_blocking=True)
+                    H.sum_(areas[m], S(2 * m, 2 * m + 1))
+                    H.mesh_mac(verts[m], areas[m], S(2 * m + 1, 2 * m + 2))
+                am_host.copy_(S(0, 2 * len(dims)), non_blocking=True)
                 am_ready.record()
                 H.wake_shed(verts, wake, i + 1)
                 solve(dTw, i + 1)
@@ -272,7 +287,7 @@ class UVLM2DOF:
                 for m in range(len(dims)):
                     H.forces_unsteady(verts[m], gdelta[m], gdt[m], vel[m], areas[m], normals[m], forces[m])
                 am_ready.synchronize()
-                cl, cmy = self._coefficients(forces, verts, ref, fs, la, vars.rho, scal, am_host.tolist())
+                cl, cmy = self._coefficients(forces, verts, ref, fs, la, vars.rho, S, am_host.tolist())
                 F[0, i + 1] = T(-cl / (float(PI_F) * vars.mu))
                 F[1, i + 1] = T(2.0 * cmy / (float(PI_F) * vars.mu * vars.r_a ** 2))
                 dF = np.array([F[0, i + 1] - F[0, i] - kh * du[0],
@@ -290,18 +305,19 @@ class UVLM2DOF:
                 "t_steps": t_steps}
 
     @staticmethod
-    def _coefficients(forces, verts, ref, fs, la, rho, scal, am):
+    def _coefficients(forces, verts, ref, fs, la, rho, S, am):
         """coeff_cl_multibody / coeff_cm_multibody (vlm/src/vlm_backend.cpp:71-135) from the surfaces'
         areas and MACs am = [area0, mac0, area1, mac1, ...] (Backend::sum and mesh_mac of the same
-        pose): all CL/CM launched together, one host read."""
+        pose): all CL/CM launched together into the device scalars S(a, b) (scal[a:b]), one host
+        read."""
         n = len(forces)
         base = 2 * n
         for m in range(n):
             area, mac = am[2 * m], am[2 * m + 1]
             o = base + 4 * m
-            H.coeff_cl(forces[m], la, fs, rho, area, scal[o: o + 1])
-            H.coeff_cm(forces[m], verts[m], ref, fs, rho, area, mac, scal[o + 1: o + 4])
-        co = scal[base: base + 4 * n].tolist()
+            H.coeff_cl(forces[m], la, fs, rho, area, S(o, o + 1))
+            H.coeff_cm(forces[m], verts[m], ref, fs, rho, area, mac, S(o + 1, o + 4))
+        co = S(base, base + 4 * n).tolist()
         cl = cmy = ta = tm = 0.0
         for m in range(n):
             area, mac = am[2 * m], am[2 * m + 1]
