@@ -89,6 +89,11 @@ class AndersonSolver:
         self.n, self.m = n, m
         self.X, self.G, self.Gk = z(n, m), z(n, m), z(n, m)
         self.x_curr, self.x_new, self.g_curr, self.g_new, self.gamma = z(n), z(n), z(n), z(n), z(n)
+        # the history's column slices, made once (the same tensors at every iteration keep their views)
+        self.lead = {name: [B[:, :j] for j in range(m + 1)] for name, B in (("X", self.X), ("G", self.G),
+                                                                           ("Gk", self.Gk))}
+        self.lead["gamma"] = [self.gamma[:j] for j in range(m + 1)]
+        self.col = {name: [B[:, j] for j in range(m)] for name, B in (("X", self.X), ("G", self.G))}
         # [0]: the residual norm (nrm2), [1]: the last gels info (int32 in its first four bytes)
         self.flags = torch.zeros(2, dtype=torch.float64, device="cuda")
         self.scal = self.flags[0:1]
@@ -137,15 +142,17 @@ class AndersonSolver:
             if not res > tol_res:
                 break
             m_k = min(m, k)
+            # X[:, :m_k], G[:, :m_k], Gk[:, :m_k], gamma[:m_k]
+            Xm, Gm, Gkm, gm = (self.lead[name][m_k] for name in ("X", "G", "Gk", "gamma"))
             gamma.copy_(g_curr)
-            Gk[:, :m_k].copy_(G[:, :m_k])
-            H.gels(Gk[:, :m_k], gamma, self.info)
+            Gkm.copy_(Gm)
+            H.gels(Gkm, gamma, self.info)
             pending = True
             x_new.copy_(x_curr)
             H.axpy(1.0, g_curr, x_new)
-            H.gemv(-1.0, X[:, :m_k], gamma[:m_k], 1.0, x_new)
-            H.gemv(-1.0, G[:, :m_k], gamma[:m_k], 1.0, x_new)
-            Xb, Gb = X[:, k % m], G[:, k % m]
+            H.gemv(-1.0, Xm, gm, 1.0, x_new)
+            H.gemv(-1.0, Gm, gm, 1.0, x_new)
+            Xb, Gb = self.col["X"][k % m], self.col["G"][k % m]
             Xb.copy_(x_new)
             H.axpy(-1.0, x_curr, Xb)
             x_curr.copy_(x_new)
