@@ -109,10 +109,6 @@ class AndersonSolver:
         if info != 0:
             raise H.TvlmError(f"Anderson least squares: rank-deficient history (gels info {info})")
 
-    def norm(self, x):
-        H.nrm2(x, self.scal)
-        return self._read()[0]
-
     def run(self, x0, f, max_iter=100, tol_res=1e-6):
         """f(x, y) writes the map's value at x into y.  x0 is overwritten with the fixed point;
         returns the iteration counter k of the reference's loop."""
