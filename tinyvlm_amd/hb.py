@@ -232,6 +232,9 @@ class HBVLM:
             "areas": [z(ns, nc) for ns, nc in self.dims],
         } for _ in range(Kc)]
         self.rhs_vel = z(N, Kc)
+        # the buffers were filled on the caller's current stream; the solve runs on self.stream, a
+        # stream that does not wait for the null stream or any other: finish the fills here
+        torch.cuda.current_stream().synchronize()
 
     def reset(self):
         """Forget the warm start: the next run starts from zero Fourier coefficients, as the first
@@ -239,6 +242,9 @@ class HBVLM:
         if self.harmonics is None:
             raise RuntimeError("HBVLM.init must be called before reset")
         self.gamma_coeffs.zero_()
+        # on the caller's current stream: done before run() starts the solve on self.stream (a served
+        # worker resets before it makes self.stream current)
+        self.torch.cuda.current_stream().synchronize()
         self.iterations = 0
 
     # ---- the binding's run (lib/libhbvlm3.cpp:552-641) -----------------------------------------
